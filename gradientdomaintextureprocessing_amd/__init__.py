@@ -1,0 +1,14 @@
+"""MI355X-native texel-domain screened-Poisson multigrid (gradient-domain texture processing).
+
+The product is libgdtp.so (HIP kernels for gfx950 + native host set-up behind the C ABI in
+include/gdtp.h); this package is its Python binding and host-side call surface.
+"""
+from .api import (METRIC_EMBEDDING, METRIC_UNIFORM, OPERATOR_MASS, OPERATOR_STIFFNESS, OPERATOR_SYSTEM, AddProlongation,
+                  HierarchicalSystem, InitializeHierarchy, MultigridSystem, MultiplyByRestriction,
+                  MultiplyBySystemMatrix_NoReciprocals, Relaxation, RelaxationAndResidual, TexturedMesh, UpdateLinearSystem,
+                  VCycle)
+
+__all__ = ["TexturedMesh", "HierarchicalSystem", "MultigridSystem", "InitializeHierarchy", "UpdateLinearSystem", "VCycle",
+           "Relaxation", "RelaxationAndResidual", "MultiplyByRestriction", "AddProlongation",
+           "MultiplyBySystemMatrix_NoReciprocals", "OPERATOR_SYSTEM", "OPERATOR_MASS", "OPERATOR_STIFFNESS",
+           "METRIC_EMBEDDING", "METRIC_UNIFORM"]

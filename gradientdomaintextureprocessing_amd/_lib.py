@@ -1,0 +1,72 @@
+"""ctypes binding of libgdtp.so (the C ABI declared in include/gdtp.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdtp.so")
+
+c_int, c_double, c_void_p, c_char_p, c_size_t, c_ull = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p,
+                                                         ctypes.c_size_t, ctypes.c_ulonglong)
+P = ctypes.POINTER
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gdtp_last_error": (c_char_p, []),
+    "gdtp_device_count": (c_int, [P(c_int)]),
+    "gdtp_mesh_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, P(c_void_p)]),
+    "gdtp_mesh_create_cube": (c_int, [c_int, c_int, c_int, P(c_void_p)]),
+    "gdtp_mesh_create_icosphere": (c_int, [c_int, c_int, c_int, c_double, c_ull, P(c_void_p)]),
+    "gdtp_mesh_create_torus": (c_int, [c_int, c_int, c_int, c_int, c_double, c_double, c_double, c_ull, c_int, c_int, P(c_void_p)]),
+    "gdtp_mesh_counts": (c_int, [c_void_p, P(c_int), P(c_int)]),
+    "gdtp_mesh_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gdtp_mesh_destroy": (None, [c_void_p]),
+    "gdtp_hierarchy_create": (c_int, [c_void_p, c_int, c_int, c_int, c_int, P(c_void_p)]),
+    "gdtp_hierarchy_level_info": (c_int, [c_void_p, c_int, c_void_p]),
+    "gdtp_hierarchy_array_i32": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_hierarchy_array_f64": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_hierarchy_destroy": (None, [c_void_p]),
+    "gdtp_system_create": (c_int, [c_void_p, c_double, c_double, c_int, P(c_void_p)]),
+    "gdtp_system_update": (c_int, [c_void_p, c_double, c_double]),
+    "gdtp_system_array_i32": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_system_array_f64": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_system_destroy": (None, [c_void_p]),
+    "gdtp_vcycle": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_relax": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_band_sweep": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_restrict": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_prolong_add": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_boundary_solve": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_coarse_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+}
+
+_lib = None
+
+
+class GdtpError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"gdtp error {code}: {message}")
+        self.code = code
+
+
+def load():
+    """Load libgdtp.so (fails loudly if the HIP library has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m gradientdomaintextureprocessing_amd.build` first")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise GdtpError(rc, load().gdtp_last_error().decode())
+    return rc

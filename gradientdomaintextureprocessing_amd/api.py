@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference's texel-solver call surface, over the libgdtp C ABI.
+
+Reference names are kept where they exist (include/Src/*.inl, TextureFiltering/*.cpp):
+InitializeHierarchy, UpdateLinearSystem, VCycle, Relaxation, RelaxationAndResidual,
+MultiplyByRestriction, AddProlongation, MultiplyBySystemMatrix_NoReciprocals, solve.
+Device vectors are torch tensors on the GPU, float64, shape [ncols, N_level], contiguous.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+OPERATOR_SYSTEM, OPERATOR_MASS, OPERATOR_STIFFNESS = 0, 1, 2
+METRIC_EMBEDDING, METRIC_UNIFORM = 0, 1
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ValueError("device vectors must be contiguous GPU tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class TexturedMesh:
+    """Triangle mesh with per-corner texture coordinates (include/Src/SimpleMesh.h:45-49)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_arrays(cls, vertices, triangles, corner_uv):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        t = np.ascontiguousarray(triangles, dtype=np.int32)
+        uv = np.ascontiguousarray(corner_uv, dtype=np.float64)
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.gdtp_mesh_create(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0], uv.ctypes.data, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def cube(cls, subdiv, width, height):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.gdtp_mesh_create_cube(subdiv, width, height, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def icosphere(cls, levels, width, height, amplitude=0.05, seed=0):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.gdtp_mesh_create_icosphere(levels, width, height, amplitude, seed, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def torus(cls, nu, nv, charts_u, charts_v, width, height, major_radius=1.0, minor_radius=0.35, amplitude=0.05, seed=0):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.gdtp_mesh_create_torus(nu, nv, charts_u, charts_v, major_radius, minor_radius, amplitude, seed, width, height,
+                                              ctypes.byref(h)))
+        return cls(h)
+
+    def arrays(self):
+        lib = _lib.load()
+        nv, nt = ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib.gdtp_mesh_counts(self._h, ctypes.byref(nv), ctypes.byref(nt)))
+        v = np.zeros((nv.value, 3)); t = np.zeros((nt.value, 3), np.int32); uv = np.zeros((nt.value, 3, 2))
+        _lib.check(lib.gdtp_mesh_read(self._h, v.ctypes.data, t.ctypes.data, uv.ctypes.data))
+        return v, t, uv
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.gdtp_mesh_destroy(self._h)
+            self._h = None
+
+
+class HierarchicalSystem:
+    """Texel hierarchy plus finest-level mass and stiffness (Hierarchy.h:265-275)."""
+
+    def __init__(self, handle, levels, width, height):
+        self._h, self.levels, self.width, self.height = handle, levels, width, height
+
+    def level_info(self, level):
+        info = np.zeros(8, np.int32)
+        _lib.check(_lib.load().gdtp_hierarchy_level_info(self._h, level, info.ctypes.data))
+        keys = ["texels", "deep", "boundary", "charts", "raster_lines", "tasks", "blocks", "segments"]
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def array(self, level, name, dtype=np.int32):
+        lib = _lib.load()
+        fn = lib.gdtp_hierarchy_array_i32 if dtype == np.int32 else lib.gdtp_hierarchy_array_f64
+        n = ctypes.c_size_t(0)
+        _lib.check(fn(self._h, level, name.encode(), None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype)
+        if n.value:
+            _lib.check(fn(self._h, level, name.encode(), out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    def texel_coordinates(self, level=0):
+        """(atlas column, atlas row, chart) of every texel of `level` in global order
+        (TextureNodeInfo, HierarchyConstruction.inl:534-555)."""
+        ni = self.array(level, "nodeInfo").reshape(-1, 4)
+        charts = self.array(level, "charts").reshape(-1, 9)
+        ci = charts[ni[:, 0], 2] + ni[:, 1]
+        cj = charts[ni[:, 0], 3] + ni[:, 2]
+        return ci, cj, ni[:, 0]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.gdtp_hierarchy_destroy(self._h)
+            self._h = None
+
+
+def InitializeHierarchy(mesh, width, height, levels=4, metric=METRIC_EMBEDDING):
+    """InitializeHierarchy + InitializeMetric + InitializeMassAndStiffness
+    (HierarchyConstruction.inl:799-849, Metric.inl:448-466, MassAndStiffness.h:490-549)."""
+    h = ctypes.c_void_p()
+    _lib.check(_lib.load().gdtp_hierarchy_create(mesh._h, width, height, levels, metric, ctypes.byref(h)))
+    return HierarchicalSystem(h, levels, width, height)
+
+
+class MultigridSystem:
+    """A = screen*M + stiffness*S over the hierarchy, with device buffers for max_cols columns."""
+
+    def __init__(self, handle, hierarchy, max_cols):
+        self._h, self.hierarchy, self.max_cols = handle, hierarchy, max_cols
+
+    def update(self, stiffness_weight, screen_weight=1.0):
+        _lib.check(_lib.load().gdtp_system_update(self._h, screen_weight, stiffness_weight))
+
+    def array(self, level, name, dtype=np.float64):
+        lib = _lib.load()
+        fn = lib.gdtp_system_array_i32 if dtype == np.int32 else lib.gdtp_system_array_f64
+        n = ctypes.c_size_t(0)
+        _lib.check(fn(self._h, level, name.encode(), None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype)
+        if n.value:
+            _lib.check(fn(self._h, level, name.encode(), out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    # -- device operations
+    def vcycle(self, x, b):
+        _lib.check(_lib.load().gdtp_vcycle(self._h, _ptr(x), _ptr(b), x.shape[0], _stream()))
+
+    def relax(self, level, x, b, residual=None):
+        _lib.check(_lib.load().gdtp_relax(self._h, level, _ptr(x), _ptr(b), _ptr(residual), x.shape[0], _stream()))
+
+    def band_sweep(self, level, x, b, residual=None):
+        _lib.check(_lib.load().gdtp_band_sweep(self._h, level, _ptr(x), _ptr(b), _ptr(residual), x.shape[0], _stream()))
+
+    def restrict(self, level, fine, coarse):
+        _lib.check(_lib.load().gdtp_restrict(self._h, level, _ptr(fine), _ptr(coarse), fine.shape[0], _stream()))
+
+    def prolong_add(self, level, coarse, fine):
+        _lib.check(_lib.load().gdtp_prolong_add(self._h, level, _ptr(coarse), _ptr(fine), fine.shape[0], _stream()))
+
+    def boundary_solve(self, level, rhs, x):
+        _lib.check(_lib.load().gdtp_boundary_solve(self._h, level, _ptr(rhs), _ptr(x), rhs.shape[0], _stream()))
+
+    def coarse_solve(self, rhs, x):
+        _lib.check(_lib.load().gdtp_coarse_solve(self._h, _ptr(rhs), _ptr(x), rhs.shape[0], _stream()))
+
+    def apply(self, which, x, y):
+        _lib.check(_lib.load().gdtp_apply(self._h, which, _ptr(x), _ptr(y), x.shape[0], _stream()))
+
+    def residual(self, x, b, r):
+        _lib.check(_lib.load().gdtp_residual(self._h, _ptr(x), _ptr(b), _ptr(r), x.shape[0], _stream()))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.gdtp_system_destroy(self._h)
+            self._h = None
+
+
+def UpdateLinearSystem(hierarchy, stiffness_weight, screen_weight=1.0, max_cols=3):
+    """UpdateLinearSystem (UpdateCoefficients.inl:463-513); max_cols=0 builds the host operator only."""
+    h = ctypes.c_void_p()
+    _lib.check(_lib.load().gdtp_system_create(hierarchy._h, screen_weight, stiffness_weight, max_cols, ctypes.byref(h)))
+    return MultigridSystem(h, hierarchy, max_cols)
+
+
+def VCycle(system, x, b):
+    """VCycle (IterativeSolvers.inl:1211-1333) on level-0 device vectors."""
+    system.vcycle(x, b)
+
+
+def RelaxationAndResidual(system, level, x, b, residual):
+    system.relax(level, x, b, residual)
+
+
+def Relaxation(system, level, x, b):
+    system.relax(level, x, b, None)
+
+
+def MultiplyByRestriction(system, level, fine, coarse):
+    system.restrict(level, fine, coarse)
+
+
+def AddProlongation(system, level, coarse, fine):
+    system.prolong_add(level, coarse, fine)
+
+
+def MultiplyBySystemMatrix_NoReciprocals(system, which, x, y):
+    system.apply(which, x, y)

@@ -1,0 +1,232 @@
+// Mass / stiffness assembly over atlas-grid cells clipped to mesh triangles.
+#include "gdtp_assembly.h"
+
+#include <omp.h>
+
+#include "gdtp_boundary.h"
+
+namespace gdtp {
+
+// Degree-4, six-point rule on the unit right triangle (weights sum to one); the same classical
+// symmetric rule the reference uses (RightTriangleQuadrature.h:6-23).
+static const double kQuadPos[6][2] = {{0.816847572980459, 0.0915762135097705}, {0.0915762135097705, 0.816847572980459},
+                                      {0.0915762135097705, 0.0915762135097705}, {0.108103018168070, 0.445948490915965},
+                                      {0.445948490915965, 0.108103018168070},  {0.445948490915965, 0.445948490915965}};
+static const double kQuadW[6] = {0.109951743655322, 0.109951743655322, 0.109951743655322,
+                                 0.223381589678011, 0.223381589678011, 0.223381589678011};
+
+// Bilinear texel basis on the unit cell; corner order (0,0),(1,0),(1,1),(0,1) (Basis.h:165-206).
+static inline double Q1(int k, double x, double y) {
+  switch (k) {
+    case 0: return (1 - x) * (1 - y);
+    case 1: return x * (1 - y);
+    case 2: return x * y;
+    default: return (1 - x) * y;
+  }
+}
+static inline V2 Q1Grad(int k, double x, double y) {
+  switch (k) {
+    case 0: return V2(y - 1, x - 1);
+    case 1: return V2(1 - y, -x);
+    case 2: return V2(y, x);
+    default: return V2(-y, 1 - x);
+  }
+}
+
+void ComputeParameterMetricFromSurface(const TexturedMesh& mesh, const Atlas& atlas, const std::vector<M2>& g,
+                                       std::vector<std::vector<M2>>& metric) {
+  metric.assign(atlas.charts.size(), {});
+  for (size_t c = 0; c < atlas.charts.size(); c++) {
+    const AtlasChart& ch = atlas.charts[c];
+    metric[c].resize(ch.meshTriangleIndices.size());
+    for (size_t k = 0; k < ch.meshTriangleIndices.size(); k++) {
+      int t = ch.meshTriangleIndices[k];
+      V2 d0 = mesh.uv[3 * t + 1] - mesh.uv[3 * t], d1 = mesh.uv[3 * t + 2] - mesh.uv[3 * t];
+      M2 D;  // columns are the UV edge vectors
+      D(0, 0) = d0.x, D(1, 0) = d0.y, D(0, 1) = d1.x, D(1, 1) = d1.y;
+      M2 Di = D.inv();
+      metric[c][k] = Di.T() * g[t] * Di;
+    }
+  }
+}
+
+void ComputeParameterMetric(const TexturedMesh& mesh, const Atlas& atlas, int mode, std::vector<std::vector<M2>>& metric) {
+  const int nT = mesh.numTriangles();
+  std::vector<M2> g(nT);
+  double total = 0;
+  for (int t = 0; t < nT; t++) {
+    M2 m;
+    if (mode == METRIC_UNIFORM) {
+      V2 d[2] = {mesh.uv[3 * t + 1] - mesh.uv[3 * t], mesh.uv[3 * t + 2] - mesh.uv[3 * t]};
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) m(a, b) = dot(d[a], d[b]);
+    } else {
+      const V3 &p0 = mesh.vertices[mesh.triangles[3 * t]], &p1 = mesh.vertices[mesh.triangles[3 * t + 1]],
+               &p2 = mesh.vertices[mesh.triangles[3 * t + 2]];
+      V3 d[2] = {p1 - p0, p2 - p0};
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) m(a, b) = dot(d[a], d[b]);
+    }
+    total += std::sqrt(m.det()) / 2;
+    g[t] = m;
+  }
+  for (auto& m : g)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) m(a, b) /= total;
+  ComputeParameterMetricFromSurface(mesh, atlas, g, metric);
+}
+
+// Sutherland-Hodgman clip of a convex polygon against {p : dot(p, n) > offset}.
+void ClipConvex(std::vector<V2>& poly, V2 n, double offset) {
+  std::vector<V2> out;
+  if (poly.empty()) return;
+  V2 prev = poly.back();
+  double pl = dot(prev, n) - offset;
+  bool pin = pl > 0;
+  for (const V2& cur : poly) {
+    double cl = dot(cur, n) - offset;
+    bool cin = cl > 0;
+    if (cin != pin) {
+      double a = -cl / (pl - cl);
+      out.push_back(prev * a + cur * (1.0 - a));
+    }
+    if (cin) out.push_back(cur);
+    prev = cur, pl = cl, pin = cin;
+  }
+  poly.swap(out);
+}
+
+// Clip to the primal cell [i cw,(i+1) cw] x [j ch,(j+1) ch] (ClipTriangleToPrimalCell, PolygonClipping.h:85-118).
+static bool ClipToCell(std::vector<V2>& poly, int i, int j, double cw, double ch) {
+  ClipConvex(poly, V2(0, 1), j * ch);
+  if (poly.empty()) return false;
+  ClipConvex(poly, V2(0, -1), -(j + 1) * ch);
+  if (poly.empty()) return false;
+  ClipConvex(poly, V2(1, 0), i * cw);
+  if (poly.empty()) return false;
+  ClipConvex(poly, V2(-1, 0), -(i + 1) * cw);
+  return !poly.empty();
+}
+
+// Integrate the 4x4 bilinear mass / stiffness over a convex polygon inside cell (i,j).
+static void IntegrateCellPolygon(const std::vector<V2>& poly, int i, int j, double cw, double ch, const M2& Ginv, double gscale,
+                                 double mass[4][4], double stiff[4][4]) {
+  for (size_t p = 2; p < poly.size(); p++) {
+    V2 d0 = poly[p - 1] - poly[0], d1 = poly[p] - poly[0];
+    double area = std::fabs(cross(d0, d1)) * gscale / 2.0;
+    double val[4][6];
+    V2 grd[4][6];
+    for (int s = 0; s < 6; s++) {
+      V2 q = poly[0] + d0 * kQuadPos[s][0] + d1 * kQuadPos[s][1];
+      double x = q.x / cw - i, y = q.y / ch - j;
+      for (int k = 0; k < 4; k++) {
+        val[k][s] = Q1(k, x, y);
+        V2 g = Q1Grad(k, x, y);
+        grd[k][s] = V2(g.x / cw, g.y / ch);
+      }
+    }
+    for (int k = 0; k < 4; k++)
+      for (int l = 0; l < 4; l++) {
+        double m = 0, st = 0;
+        for (int s = 0; s < 6; s++) {
+          m += val[k][s] * val[l][s] * kQuadW[s];
+          st += dot(grd[l][s], Ginv * grd[k][s]) * kQuadW[s];
+        }
+        mass[l][k] += m * area;
+        stiff[l][k] += st * area;
+      }
+  }
+}
+
+int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric, FineOperator& op,
+                         std::string& err) {
+  const Level& L0 = h.levels[0];
+  const int nCharts = (int)L0.charts.size();
+  op.deepMass.assign(10 * (size_t)L0.numDeep, 0.0);
+  op.deepStiffness.assign(10 * (size_t)L0.numDeep, 0.0);
+  std::vector<std::vector<Triplet>> bbM(nCharts), bbS(nCharts), bdM(nCharts), bdS(nCharts);
+  std::vector<std::string> errs(nCharts);
+  const int offI[4] = {0, 1, 1, 0}, offJ[4] = {0, 0, 1, 1};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int c = 0; c < nCharts; c++) {
+    const GridChart& g = L0.charts[c];
+    const AtlasChart& ac = h.atlas.charts[c];
+    const int nTri = (int)ac.triangles.size() / 3;
+    std::vector<std::array<double, 16>> cellM(g.numInteriorCells, std::array<double, 16>{}), cellS(g.numInteriorCells, std::array<double, 16>{});
+    std::vector<std::array<double, 16>> bcellM(g.numBoundaryCells, std::array<double, 16>{}), bcellS(g.numBoundaryCells, std::array<double, 16>{});
+    std::vector<V2> poly;
+    for (int t = 0; t < nTri; t++) {
+      V2 p[3];
+      for (int k = 0; k < 3; k++) p[k] = ac.vertices[ac.triangles[3 * t + k]] - g.corner;
+      const M2& G = metric[c][t];
+      M2 Ginv = G.inv();
+      double gscale = std::sqrt(G.det());
+      double fmn[2] = {std::min({p[0].x, p[1].x, p[2].x}), std::min({p[0].y, p[1].y, p[2].y})};
+      double fmx[2] = {std::max({p[0].x, p[1].x, p[2].x}), std::max({p[0].y, p[1].y, p[2].y})};
+      int mn0 = (int)std::floor(std::max(fmn[0], 0.0) / g.cellW), mn1 = (int)std::floor(std::max(fmn[1], 0.0) / g.cellH);
+      int mx0 = (int)std::ceil(std::min(fmx[0], 1.0) / g.cellW), mx1 = (int)std::ceil(std::min(fmx[1], 1.0) / g.cellH);
+      for (int j = mn1; j < mx1; j++)
+        for (int i = mn0; i < mx0; i++) {
+          if (i < 0 || j < 0 || i >= g.width - 1 || j >= g.height - 1) continue;
+          size_t q = (size_t)j * (g.width - 1) + i;
+          int li = g.localInteriorCell[q], lb = g.localBoundaryCell[q];
+          if (li < 0 && lb < 0) continue;
+          poly.assign(p, p + 3);
+          if (!ClipToCell(poly, i, j, g.cellW, g.cellH)) continue;
+          double m[4][4] = {}, s[4][4] = {};
+          IntegrateCellPolygon(poly, i, j, g.cellW, g.cellH, Ginv, gscale, m, s);
+          auto& CM = li >= 0 ? cellM[li] : bcellM[lb];
+          auto& CS = li >= 0 ? cellS[li] : bcellS[lb];
+          for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) CM[4 * a + b] += m[a][b], CS[4 * a + b] += s[a][b];
+        }
+    }
+    // interior cells: deep rows and the boundary rows they touch (MassAndStiffness.h:277-353)
+    for (int ic = 0; ic < g.numInteriorCells; ic++) {
+      const auto& gl = g.interiorCellGlobalCorners[ic];
+      for (int k = 0; k < 4; k++) {
+        int bd = L0.boundaryAndDeepIndex[gl[k]];
+        if (bd < 0) {
+          int d = -bd - 1;
+          for (int l = 0; l < 4; l++) {
+            int slot = (offJ[l] - offJ[k] + 1) * 3 + (offI[l] - offI[k] + 1);
+            op.deepMass[10 * (size_t)d + slot] += cellM[ic][4 * k + l];
+            op.deepStiffness[10 * (size_t)d + slot] += cellS[ic][4 * k + l];
+          }
+        } else {
+          int b = bd - 1;
+          for (int l = 0; l < 4; l++) {
+            int nbd = L0.boundaryAndDeepIndex[gl[l]];
+            if (nbd < 0) {
+              bdM[c].push_back({b, gl[l], cellM[ic][4 * k + l]});
+              bdS[c].push_back({b, gl[l], cellS[ic][4 * k + l]});
+            } else {
+              bbM[c].push_back({b, nbd - 1, cellM[ic][4 * k + l]});
+              bbS[c].push_back({b, nbd - 1, cellS[ic][4 * k + l]});
+            }
+          }
+        }
+      }
+    }
+    // boundary cells and chart seams
+    if (!AssembleBoundaryCells(mesh, h, c, metric[c], bcellM, bcellS, bbM[c], bbS[c], errs[c])) continue;
+  }
+  for (int c = 0; c < nCharts; c++)
+    if (!errs[c].empty()) { err = errs[c]; return 0; }
+  auto merge = [&](std::vector<std::vector<Triplet>>& v) {
+    std::vector<Triplet> all;
+    for (auto& x : v) all.insert(all.end(), x.begin(), x.end());
+    return all;
+  };
+  {
+    auto t = merge(bbM); op.bbMass = Csr::FromTriplets(L0.numBoundary, L0.numBoundary, t);
+    t = merge(bbS); op.bbStiffness = Csr::FromTriplets(L0.numBoundary, L0.numBoundary, t);
+    t = merge(bdM); op.bdMass = Csr::FromTriplets(L0.numBoundary, L0.numTexels, t);
+    t = merge(bdS); op.bdStiffness = Csr::FromTriplets(L0.numBoundary, L0.numTexels, t);
+  }
+  // Seam terms couple boundary texels of different charts; AssembleSeams adds them.
+  if (!AssembleSeams(mesh, h, metric, op, err)) return 0;
+  return 1;
+}
+
+}  // namespace gdtp

@@ -1,0 +1,343 @@
+// C ABI of libgdtp (declared in include/gdtp.h).
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gdtp.h"
+#include "gdtp_device.h"
+#include "gdtp_mesh.h"
+
+using namespace gdtp;
+
+struct gdtp_mesh {
+  TexturedMesh m;
+};
+
+struct gdtp_hierarchy {
+  TexturedMesh mesh;
+  Hierarchy h;
+  FineOperator op;
+};
+
+struct gdtp_system {
+  const gdtp_hierarchy* H = nullptr;
+  double screen = 1, stiff = 0;
+  int kcap = 0;
+  OperatorHierarchy oh;
+  std::unique_ptr<DeviceSystem> ds;
+};
+
+static thread_local std::string g_error;
+
+static int Fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define GDTP_TRY(body)                                                        \
+  try {                                                                       \
+    body                                                                      \
+  } catch (const std::bad_alloc&) {                                           \
+    return Fail(GDTP_ERROR_DEVICE, "out of memory");                          \
+  } catch (const std::exception& e) {                                         \
+    return Fail(GDTP_ERROR_DEVICE, e.what());                                 \
+  }
+
+// Definitions take C linkage from the declarations in gdtp.h.
+
+const char* gdtp_last_error(void) { return g_error.c_str(); }
+
+int gdtp_device_count(int* count) {
+  if (!count) return Fail(GDTP_ERROR_INVALID, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return GDTP_OK;
+}
+
+// ---------------------------------------------------------------- meshes
+int gdtp_mesh_create(const double* V, int nv, const int* T, int nt, const double* uv, gdtp_mesh** out) {
+  if (!V || !T || !uv || !out || nv <= 0 || nt <= 0) return Fail(GDTP_ERROR_INVALID, "invalid mesh arrays");
+  GDTP_TRY({
+    auto* m = new gdtp_mesh();
+    m->m.vertices.resize(nv);
+    for (int i = 0; i < nv; i++) m->m.vertices[i] = V3(V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+    m->m.triangles.assign(T, T + 3 * (size_t)nt);
+    for (size_t k = 0; k < 3 * (size_t)nt; k++)
+      if (T[k] < 0 || T[k] >= nv) { delete m; return Fail(GDTP_ERROR_INVALID, "triangle index out of range"); }
+    m->m.uv.resize(3 * (size_t)nt);
+    for (size_t k = 0; k < 3 * (size_t)nt; k++) m->m.uv[k] = V2(uv[2 * k], uv[2 * k + 1]);
+    *out = m;
+    return GDTP_OK;
+  })
+}
+
+int gdtp_mesh_create_cube(int subdiv, int W, int H, gdtp_mesh** out) {
+  if (!out) return Fail(GDTP_ERROR_INVALID, "null output");
+  auto* m = new gdtp_mesh();
+  if (!MakeCube(subdiv, W, H, m->m)) { delete m; return Fail(GDTP_ERROR_INVALID, "invalid cube parameters"); }
+  *out = m;
+  return GDTP_OK;
+}
+
+int gdtp_mesh_create_icosphere(int levels, int W, int H, double amplitude, unsigned long long seed, gdtp_mesh** out) {
+  if (!out) return Fail(GDTP_ERROR_INVALID, "null output");
+  auto* m = new gdtp_mesh();
+  if (!MakeIcosphere(levels, W, H, amplitude, seed, m->m)) { delete m; return Fail(GDTP_ERROR_INVALID, "invalid icosphere parameters"); }
+  *out = m;
+  return GDTP_OK;
+}
+
+int gdtp_mesh_create_torus(int nu, int nv, int cu, int cv, double R, double r, double amplitude, unsigned long long seed, int W, int H,
+                           gdtp_mesh** out) {
+  if (!out) return Fail(GDTP_ERROR_INVALID, "null output");
+  auto* m = new gdtp_mesh();
+  if (!MakeTorus(nu, nv, cu, cv, R, r, amplitude, seed, W, H, m->m)) { delete m; return Fail(GDTP_ERROR_INVALID, "invalid torus parameters"); }
+  *out = m;
+  return GDTP_OK;
+}
+
+int gdtp_mesh_counts(const gdtp_mesh* m, int* nv, int* nt) {
+  if (!m) return Fail(GDTP_ERROR_INVALID, "null mesh");
+  if (nv) *nv = (int)m->m.vertices.size();
+  if (nt) *nt = m->m.numTriangles();
+  return GDTP_OK;
+}
+
+int gdtp_mesh_read(const gdtp_mesh* m, double* V, int* T, double* uv) {
+  if (!m) return Fail(GDTP_ERROR_INVALID, "null mesh");
+  if (V) memcpy(V, m->m.vertices.data(), m->m.vertices.size() * sizeof(V3));
+  if (T) memcpy(T, m->m.triangles.data(), m->m.triangles.size() * sizeof(int));
+  if (uv) memcpy(uv, m->m.uv.data(), m->m.uv.size() * sizeof(V2));
+  return GDTP_OK;
+}
+
+void gdtp_mesh_destroy(gdtp_mesh* m) { delete m; }
+
+// ---------------------------------------------------------------- hierarchy
+int gdtp_hierarchy_create(const gdtp_mesh* mesh, int W, int H, int levels, int metricMode, gdtp_hierarchy** out) {
+  if (!mesh || !out || W < 8 || H < 8 || levels < 1 || levels > 16) return Fail(GDTP_ERROR_INVALID, "invalid hierarchy arguments");
+  GDTP_TRY({
+    auto* h = new gdtp_hierarchy();
+    h->mesh = mesh->m;
+    std::string err;
+    BlockInfo bi;
+    if (!BuildHierarchy(h->mesh, W, H, levels, bi, h->h, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    std::vector<std::vector<M2>> metric;
+    ComputeParameterMetric(h->mesh, h->h.atlas, metricMode == GDTP_METRIC_UNIFORM ? METRIC_UNIFORM : METRIC_EMBEDDING, metric);
+    if (!AssembleFineOperator(h->mesh, h->h, metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    *out = h;
+    return GDTP_OK;
+  })
+}
+
+int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info) {
+  if (!h || !info || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid level");
+  const Level& L = h->h.levels[level];
+  info[0] = L.numTexels, info[1] = L.numDeep, info[2] = L.numBoundary, info[3] = (int)L.charts.size();
+  info[4] = (int)L.rasterLines.size(), info[5] = (int)L.tasks.taskPtr.size() - 1, info[6] = (int)L.tasks.blockBegin.size();
+  info[7] = (int)L.tasks.segments.size();
+  return GDTP_OK;
+}
+
+template <class T>
+static int Emit(const std::vector<T>& v, T* out, size_t* count) {
+  if (!count) return Fail(GDTP_ERROR_INVALID, "null count");
+  if (out) {
+    if (*count < v.size()) return Fail(GDTP_ERROR_INVALID, "output buffer too small");
+    if (!v.empty()) memcpy(out, v.data(), v.size() * sizeof(T));
+  }
+  *count = v.size();
+  return GDTP_OK;
+}
+
+static bool CsrI32(const Csr& m, const std::string& name, const std::string& base, std::vector<int>& v) {
+  if (name == base + ".rowptr") return v = m.rowptr, true;
+  if (name == base + ".col") return v = m.col, true;
+  return false;
+}
+
+int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cname, int* out, size_t* count) {
+  if (!h || !cname || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  const Level& L = h->h.levels[level];
+  const std::string name(cname);
+  std::vector<int> v;
+  if (name == "boundaryGlobalIndex") v = L.boundaryGlobalIndex;
+  else if (name == "deepGlobalIndex") v = L.deepGlobalIndex;
+  else if (name == "boundaryAndDeepIndex") v = L.boundaryAndDeepIndex;
+  else if (name == "nodeInfo") { for (auto& n : L.nodeInfo) v.insert(v.end(), {n.chart, n.ci, n.cj, n.type}); }
+  else if (name == "rasterLines") { for (auto& x : L.rasterLines) v.insert(v.end(), {x.start, x.end, x.prev, x.next, x.coeff}); }
+  else if (name == "restrictionLines") { for (auto& x : L.restrictionLines) v.insert(v.end(), {x.start, x.end, x.prev, x.next, x.coeff}); }
+  else if (name == "deepLines") { for (auto& x : L.deepLines) v.insert(v.end(), {x.coarseStart, x.coarseEnd, x.finePrev, x.fineCurrent, x.fineNext}); }
+  else if (name == "prolongationLines") { for (auto& x : L.prolongationLines) v.insert(v.end(), {x.start, x.length, x.center, x.prev, x.next, x.aligned}); }
+  else if (name == "tasks.taskPtr") v = L.tasks.taskPtr;
+  else if (name == "tasks.blockBegin") v = L.tasks.blockBegin;
+  else if (name == "tasks.blockDeepEnd") v = L.tasks.blockDeepEnd;
+  else if (name == "tasks.blockPadEnd") v = L.tasks.blockPadEnd;
+  else if (name == "tasks.linePtr") v = L.tasks.linePtr;
+  else if (name == "tasks.segments") { for (auto& s : L.tasks.segments) v.insert(v.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart}); }
+  else if (name == "boundaryDeepIndices") { for (auto& x : L.boundaryDeepIndices) v.insert(v.end(), {x.boundary, x.deepGlobal, x.deep, x.offset}); }
+  else if (name == "boundaryBoundaryIndices") { for (auto& x : L.boundaryBoundaryIndices) v.insert(v.end(), {x.principal, x.secondary, x.fineDeep, x.offset}); }
+  else if (name == "charts") {
+    for (auto& c : L.charts)
+      v.insert(v.end(), {c.width, c.height, c.cornerCoords[0], c.cornerCoords[1], c.centerOffset[0], c.centerOffset[1], c.texelOffset, c.deepOffset, c.boundaryOffset});
+  } else if (name == "nodeType") { for (auto& c : L.charts) for (auto t : c.nodeType) v.push_back(t); }
+  else if (name == "cellType") { for (auto& c : L.charts) for (auto t : c.cellType) v.push_back(t); }
+  else if (name == "chartTexelIndex") { for (auto& c : L.charts) v.insert(v.end(), c.globalTexel.begin(), c.globalTexel.end()); }
+  else if (CsrI32(L.boundaryRestriction, name, "boundaryRestriction", v) || CsrI32(L.boundaryCoarseFineProlongation, name, "boundaryCoarseFineProlongation", v) ||
+           CsrI32(L.boundaryFineCoarseRestriction, name, "boundaryFineCoarseRestriction", v)) {
+  } else if (level == 0 && (CsrI32(h->op.bbMass, name, "mass.bb", v) || CsrI32(h->op.bbStiffness, name, "stiffness.bb", v) ||
+                            CsrI32(h->op.bdMass, name, "mass.bd", v) || CsrI32(h->op.bdStiffness, name, "stiffness.bd", v))) {
+  } else return Fail(GDTP_ERROR_INVALID, "unknown hierarchy array " + name);
+  return Emit(v, out, count);
+}
+
+int gdtp_hierarchy_array_f64(const gdtp_hierarchy* h, int level, const char* cname, double* out, size_t* count) {
+  if (!h || !cname || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  const Level& L = h->h.levels[level];
+  const std::string name(cname);
+  std::vector<double> v;
+  if (name == "boundaryRestriction.val") v = L.boundaryRestriction.val;
+  else if (name == "boundaryCoarseFineProlongation.val") v = L.boundaryCoarseFineProlongation.val;
+  else if (name == "boundaryFineCoarseRestriction.val") v = L.boundaryFineCoarseRestriction.val;
+  else if (name == "boundaryBoundaryWeights") { for (auto& x : L.boundaryBoundaryIndices) v.push_back(x.weight); }
+  else if (name == "chartsGeom") { for (auto& c : L.charts) v.insert(v.end(), {c.corner.x, c.corner.y, c.cellW, c.cellH}); }
+  else if (level == 0 && name == "mass.deep") v = h->op.deepMass;
+  else if (level == 0 && name == "stiffness.deep") v = h->op.deepStiffness;
+  else if (level == 0 && name == "mass.bb.val") v = h->op.bbMass.val;
+  else if (level == 0 && name == "stiffness.bb.val") v = h->op.bbStiffness.val;
+  else if (level == 0 && name == "mass.bd.val") v = h->op.bdMass.val;
+  else if (level == 0 && name == "stiffness.bd.val") v = h->op.bdStiffness.val;
+  else return Fail(GDTP_ERROR_INVALID, "unknown hierarchy array " + name);
+  return Emit(v, out, count);
+}
+
+void gdtp_hierarchy_destroy(gdtp_hierarchy* h) { delete h; }
+
+// ---------------------------------------------------------------- system
+static int BuildSystem(gdtp_system* s, double screen, double stiff) {
+  std::string err;
+  OperatorHierarchy oh;
+  if (!BuildOperatorHierarchy(s->H->h, s->H->op, screen, stiff, true, oh, err)) return Fail(GDTP_ERROR_NUMERIC, err);
+  std::unique_ptr<DeviceSystem> ds;
+  if (s->kcap > 0) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return Fail(GDTP_ERROR_DEVICE, "no HIP device available");
+    ds.reset(new DeviceSystem());
+    UploadSystem(s->H->h, oh, s->H->op, s->kcap, *ds);
+  }
+  s->oh = std::move(oh);
+  s->ds = std::move(ds);
+  s->screen = screen, s->stiff = stiff;
+  return GDTP_OK;
+}
+
+int gdtp_system_create(const gdtp_hierarchy* h, double screen, double stiff, int maxCols, gdtp_system** out) {
+  if (!h || !out || maxCols < 0 || maxCols > 4096) return Fail(GDTP_ERROR_INVALID, "invalid system arguments");
+  GDTP_TRY({
+    auto* s = new gdtp_system();
+    s->H = h;
+    s->kcap = maxCols;
+    int rc = BuildSystem(s, screen, stiff);
+    if (rc != GDTP_OK) { delete s; return rc; }
+    *out = s;
+    return GDTP_OK;
+  })
+}
+
+int gdtp_system_update(gdtp_system* s, double screen, double stiff) {
+  if (!s) return Fail(GDTP_ERROR_INVALID, "null system");
+  GDTP_TRY({ return BuildSystem(s, screen, stiff); })
+}
+
+int gdtp_system_array_i32(const gdtp_system* s, int level, const char* cname, int* out, size_t* count) {
+  if (!s || !cname || level < 0 || level >= (int)s->oh.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  const std::string name(cname);
+  const LevelOperator& O = s->oh.levels[level];
+  std::vector<int> v;
+  if (CsrI32(O.bd, name, "bd", v) || CsrI32(O.bb, name, "bb", v) || CsrI32(s->oh.coarseMatrix, name, "coarse", v)) {
+  } else if (name == "boundaryFactor.perm" && level + 1 < (int)s->oh.levels.size()) v = s->oh.boundaryFactor[level].perm;
+  else if (name == "coarseFactor.perm") v = s->oh.coarseFactor.perm;
+  else return Fail(GDTP_ERROR_INVALID, "unknown system array " + name);
+  return Emit(v, out, count);
+}
+
+int gdtp_system_array_f64(const gdtp_system* s, int level, const char* cname, double* out, size_t* count) {
+  if (!s || !cname || level < 0 || level >= (int)s->oh.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  const std::string name(cname);
+  const LevelOperator& O = s->oh.levels[level];
+  std::vector<double> v;
+  if (name == "deepRaw") v = O.deepRaw;
+  else if (name == "deepRecip") v = O.deepRecip;
+  else if (name == "bd.val") v = O.bd.val;
+  else if (name == "bb.val") v = O.bb.val;
+  else if (name == "coarse.val") v = s->oh.coarseMatrix.val;
+  else return Fail(GDTP_ERROR_INVALID, "unknown system array " + name);
+  return Emit(v, out, count);
+}
+
+void gdtp_system_destroy(gdtp_system* s) { delete s; }
+
+// ---------------------------------------------------------------- device operations
+static int Ready(gdtp_system* s, int ncols) {
+  if (!s) return Fail(GDTP_ERROR_INVALID, "null system");
+  if (!s->ds) return Fail(GDTP_ERROR_INVALID, "system was created without device buffers (max_cols = 0)");
+  if (ncols < 1 || ncols > s->kcap) return Fail(GDTP_ERROR_INVALID, "ncols outside [1, max_cols]");
+  return GDTP_OK;
+}
+
+#define GDTP_DEVICE_OP(cond, body)                                      \
+  int rc_ = Ready(s, ncols);                                            \
+  if (rc_ != GDTP_OK) return rc_;                                       \
+  if (!(cond)) return Fail(GDTP_ERROR_INVALID, "invalid arguments");    \
+  GDTP_TRY({                                                            \
+    hipStream_t st = (hipStream_t)stream;                               \
+    DeviceSystem& ds = *s->ds;                                          \
+    body;                                                               \
+    GDTP_HIP(hipGetLastError());                                        \
+    return GDTP_OK;                                                     \
+  })
+
+int gdtp_vcycle(gdtp_system* s, double* x, const double* b, int ncols, void* stream) {
+  GDTP_DEVICE_OP(x && b, VCycleReference(ds, x, b, ncols, ds.L[0].N, st))
+}
+
+int gdtp_relax(gdtp_system* s, int level, double* x, const double* b, double* residual, int ncols, void* stream) {
+  GDTP_DEVICE_OP(x && b && level >= 0 && level + 1 < s->ds->levels,
+                 RelaxLevel(ds, level, x, b, residual, ncols, ds.L[level].N, residual != nullptr, st))
+}
+
+int gdtp_band_sweep(gdtp_system* s, int level, double* x, const double* b, double* residual, int ncols, void* stream) {
+  GDTP_DEVICE_OP(x && b && level >= 0 && level + 1 < s->ds->levels,
+                 BandSweepLevel(ds, level, x, b, residual, ncols, ds.L[level].N, st))
+}
+
+int gdtp_restrict(gdtp_system* s, int level, const double* fine, double* coarse, int ncols, void* stream) {
+  GDTP_DEVICE_OP(fine && coarse && level >= 0 && level + 1 < s->ds->levels,
+                 RestrictLevel(ds, level, fine, coarse, ncols, ds.L[level].N, ds.L[level + 1].N, st))
+}
+
+int gdtp_prolong_add(gdtp_system* s, int level, const double* coarse, double* fine, int ncols, void* stream) {
+  GDTP_DEVICE_OP(fine && coarse && level >= 0 && level + 1 < s->ds->levels,
+                 ProlongAddLevel(ds, level, coarse, fine, ncols, ds.L[level + 1].N, ds.L[level].N, st))
+}
+
+int gdtp_boundary_solve(gdtp_system* s, int level, const double* rhs, double* x, int ncols, void* stream) {
+  GDTP_DEVICE_OP(rhs && x && level >= 0 && level + 1 < s->ds->levels,
+                 BoundarySolveLevel(ds, level, rhs, x, ncols, ds.L[level].B, st))
+}
+
+int gdtp_coarse_solve(gdtp_system* s, const double* rhs, double* x, int ncols, void* stream) {
+  GDTP_DEVICE_OP(rhs && x, CoarseSolve(ds, rhs, x, ncols, ds.L[ds.levels - 1].N, st))
+}
+
+int gdtp_apply(gdtp_system* s, int which, const double* x, double* y, int ncols, void* stream) {
+  GDTP_DEVICE_OP(x && y && which >= 0 && which <= 2,
+                 ApplyOperator(ds, which == 0 ? ds.L[0] : (which == 1 ? ds.mass0 : ds.stiff0), x, y, ncols, ds.L[0].N, st))
+}
+
+int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, int ncols, void* stream) {
+  GDTP_DEVICE_OP(x && b && r, ResidualLevel(ds, 0, x, b, r, ncols, ds.L[0].N, st))
+}
+

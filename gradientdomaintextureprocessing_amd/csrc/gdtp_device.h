@@ -1,0 +1,83 @@
+// Device-resident multigrid system and the HIP V-cycle (internal interface of libgdtp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "gdtp_galerkin.h"
+
+namespace gdtp {
+
+#define GDTP_HIP(call)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Exact triangular solves of an LDL^T factor, split into independent elimination-subtree
+// teams (one workgroup each) and a top part solved by a single workgroup.
+struct DevLdlt {
+  int n = 0;
+  int *perm = nullptr;
+  int *lrRowptr = nullptr, *lrCol = nullptr, *lcRowptr = nullptr, *lcCol = nullptr;
+  double *lrVal = nullptr, *lcVal = nullptr, *dinv = nullptr;
+  // teams: team t owns forward levels [teamLevPtr[t], teamLevPtr[t+1]) of levRowPtr -> levRows
+  int nTeams = 0;
+  int *teamFwdLev = nullptr, *teamBwdLev = nullptr;  // nTeams+1 each, indices into the level tables
+  int *fwdLevPtr = nullptr, *fwdRows = nullptr, *bwdLevPtr = nullptr, *bwdRows = nullptr;
+  int topFwdLevBegin = 0, topFwdLevEnd = 0, topBwdLevBegin = 0, topBwdLevEnd = 0;
+  int maxTeamRows = 0;
+};
+
+struct DevCsr {
+  int rows = 0, nnz = 0;
+  int *rowptr = nullptr, *col = nullptr;
+  double* val = nullptr;
+};
+
+struct DevLevel {
+  int N = 0, D = 0, B = 0;
+  double* coef = nullptr;      // reciprocal (levels < L-1) deep stencil, SoA [10][D]
+  double* coefRaw = nullptr;   // raw deep stencil A, SoA [10][D]
+  int* bgi = nullptr;          // boundary -> global
+  DevCsr bd, bb, brestr;       // A_BD, A_BB, boundary restriction to level+1
+  // band tasks
+  int nTasks = 0;
+  int *taskPtr = nullptr, *blockBegin = nullptr, *blockDeepEnd = nullptr, *blockPadEnd = nullptr, *linePtr = nullptr;
+  int* segs = nullptr;         // 5 ints per segment
+  int nRaster = 0, nRestr = 0, nProl = 0;
+  int *raster = nullptr, *restr = nullptr, *prol = nullptr;  // 5, 5, 6 ints per line
+  DevLdlt factor;              // boundary factor (levels < L-1) or full coarse factor (level L-1)
+  // work vectors for up to `kcap` columns
+  double *x = nullptr, *b = nullptr, *r = nullptr, *vb = nullptr, *y = nullptr;
+};
+
+struct DeviceSystem {
+  int levels = 0, kcap = 0;
+  std::vector<DevLevel> L;
+  DevLevel mass0, stiff0;      // level-0 mass and stiffness (raw stencils and boundary rows only)
+  std::vector<void*> allocations;
+  hipStream_t stream = nullptr;
+  ~DeviceSystem();
+};
+
+void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds);
+
+// One V-cycle of the reference schedule on level-0 vectors x (in/out) and b, K <= kcap columns,
+// column stride ld (IterativeSolvers.inl:1211-1333).
+void VCycleReference(DeviceSystem& ds, double* x, const double* b, int K, int ld, hipStream_t s);
+
+// Building blocks, exported for kernel-level parity tests.
+void RelaxLevel(DeviceSystem& ds, int level, double* x, const double* b, double* r, int K, int ld, bool withResidual, hipStream_t s);
+void BandSweepLevel(DeviceSystem& ds, int level, double* x, const double* b, double* r, int K, int ld, hipStream_t s);
+void RestrictLevel(DeviceSystem& ds, int level, const double* rFine, double* bCoarse, int K, int ldF, int ldC, hipStream_t s);
+void ProlongAddLevel(DeviceSystem& ds, int level, const double* xCoarse, double* xFine, int K, int ldC, int ldF, hipStream_t s);
+void BoundarySolveLevel(DeviceSystem& ds, int level, const double* rhs, double* sol, int K, int ld, hipStream_t s);
+void CoarseSolve(DeviceSystem& ds, const double* rhs, double* sol, int K, int ld, hipStream_t s);
+// y = A x with raw (non-reciprocal) coefficients of `level`; residual r = b - A x.
+void ApplyLevel(DeviceSystem& ds, int level, const double* x, double* y, int K, int ld, hipStream_t s);
+void ApplyOperator(DeviceSystem& ds, DevLevel& op, const double* x, double* y, int K, int ld, hipStream_t s);
+void ResidualLevel(DeviceSystem& ds, int level, const double* x, const double* b, double* r, int K, int ld, hipStream_t s);
+
+}  // namespace gdtp

@@ -1,0 +1,626 @@
+// HIP kernels of the texel multigrid (CDNA4 / gfx950) and their host orchestration.
+//
+// Vectors are column-major: column c of a level-l vector starts at c*ld.  Deep stencils are
+// stored structure-of-arrays, [10][D], so every stencil slot is a coalesced stream.
+#include <stdexcept>
+
+#include "gdtp_device.h"
+
+namespace gdtp {
+
+static constexpr int kWave = 64;
+static constexpr int kKc = 4;  // channels processed together in registers
+
+// ------------------------------------------------------------------ band Gauss-Seidel
+// The reference smoother (IterativeSolvers.inl:398-558 with residual, :174-395 without):
+// per band task, blocks left to right; inside a block, three lagged passes over the block's deep
+// rows (at step j rows j, j-1, j-2), each row a left-to-right Gauss-Seidel sweep of its deep
+// segments, and (pre-smoother) the true residual of padded row j-3.  One wavefront runs one task.
+// The in-row dependence x_i = beta_i - c3_i x_{i-1} is a first-order linear recurrence, evaluated
+// with a wavefront-wide scan (two texels per lane, 128-texel rows).
+
+struct SegRef {
+  const int* s;
+  __device__ int cs(int i) const { return s[5 * i + 0]; }
+  __device__ int ce(int i) const { return s[5 * i + 1]; }
+  __device__ int ps(int i) const { return s[5 * i + 2]; }
+  __device__ int ns(int i) const { return s[5 * i + 3]; }
+  __device__ int ds(int i) const { return s[5 * i + 4]; }
+};
+
+// Locate element e of a row made of segments [s0, s1); returns the segment or -1.
+__device__ __forceinline__ int FindElem(SegRef S, int s0, int s1, int e, int& off) {
+  int base = 0;
+  for (int s = s0; s < s1; s++) {
+    int len = S.ce(s) - S.cs(s) + 1;
+    if (e < base + len) { off = e - base; return s; }
+    base += len;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void GsRow(SegRef S, int s0, int s1, const double* __restrict__ coef, int D, double* x,
+                                      const double* __restrict__ b, int K, int ld) {
+  const int lane = threadIdx.x;
+  int off[2], seg[2], g[2] = {0, 0}, p[2] = {0, 0}, n[2] = {0, 0}, d[2] = {0, 0};
+  double a[2] = {0, 0}, c[2][9];
+  for (int q = 0; q < 2; q++) {
+    seg[q] = FindElem(S, s0, s1, 2 * lane + q, off[q]);
+    if (seg[q] >= 0) {
+      g[q] = S.cs(seg[q]) + off[q], p[q] = S.ps(seg[q]) + off[q], n[q] = S.ns(seg[q]) + off[q], d[q] = S.ds(seg[q]) + off[q];
+      for (int k = 0; k < 9; k++) c[q][k] = coef[(size_t)k * D + d[q]];
+      a[q] = off[q] == 0 ? 0.0 : -c[q][3];
+    }
+  }
+  for (int c0 = 0; c0 < K; c0 += kKc) {
+    const int kc = min(kKc, K - c0);
+    double beta[2][kKc];
+    for (int q = 0; q < 2; q++)
+      for (int cc = 0; cc < kKc; cc++) {
+        beta[q][cc] = 0.0;
+        if (seg[q] < 0 || cc >= kc) continue;
+        const double* X = x + (size_t)(c0 + cc) * ld;
+        const double* B = b + (size_t)(c0 + cc) * ld;
+        double s = X[p[q] - 1] * c[q][0] + X[p[q]] * c[q][1] + X[p[q] + 1] * c[q][2] + X[g[q] + 1] * c[q][5] +
+                   X[n[q] - 1] * c[q][6] + X[n[q]] * c[q][7] + X[n[q] + 1] * c[q][8];
+        double v = B[g[q]] * c[q][4] - s;
+        if (off[q] == 0) v -= X[g[q] - 1] * c[q][3];
+        beta[q][cc] = v;
+      }
+    // lane-local composition of the two elements, then an inclusive wavefront scan
+    double A = a[1] * a[0];
+    double Bs[kKc];
+    for (int cc = 0; cc < kKc; cc++) Bs[cc] = a[1] * beta[0][cc] + beta[1][cc];
+    for (int o = 1; o < kWave; o <<= 1) {
+      double Au = __shfl_up(A, o, kWave);
+      double Bu[kKc];
+      for (int cc = 0; cc < kKc; cc++) Bu[cc] = __shfl_up(Bs[cc], o, kWave);
+      if (lane >= o) {
+        for (int cc = 0; cc < kKc; cc++) Bs[cc] = A * Bu[cc] + Bs[cc];
+        A = A * Au;
+      }
+    }
+    for (int cc = 0; cc < kKc; cc++) {
+      double prev = __shfl_up(Bs[cc], 1, kWave);
+      if (lane == 0) prev = 0.0;
+      double x0 = a[0] * prev + beta[0][cc];
+      double x1 = a[1] * x0 + beta[1][cc];
+      if (cc < kc) {
+        double* X = x + (size_t)(c0 + cc) * ld;
+        if (seg[0] >= 0) X[g[0]] = x0;
+        if (seg[1] >= 0) X[g[1]] = x1;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void ResRow(SegRef S, int s0, int s1, const double* __restrict__ coef, int D, const double* x,
+                                       const double* __restrict__ b, double* r, int K, int ld) {
+  int total = 0;
+  for (int s = s0; s < s1; s++) total += S.ce(s) - S.cs(s) + 1;
+  for (int e = threadIdx.x; e < total; e += kWave) {
+    int off, sg = FindElem(S, s0, s1, e, off);
+    int g = S.cs(sg) + off, p = S.ps(sg) + off, n = S.ns(sg) + off, d = S.ds(sg) + off;
+    double c[10];
+    for (int k = 0; k < 10; k++) c[k] = coef[(size_t)k * D + d];
+    for (int cc = 0; cc < K; cc++) {
+      const double* X = x + (size_t)cc * ld;
+      double s = X[p - 1] * c[0] + X[p] * c[1] + X[p + 1] * c[2] + X[g - 1] * c[3] + X[g] + X[g + 1] * c[5] + X[n - 1] * c[6] +
+                 X[n] * c[7] + X[n + 1] * c[8];
+      r[(size_t)cc * ld + g] = b[(size_t)cc * ld + g] - s * c[9];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr, const int* __restrict__ blockBegin,
+                                                 const int* __restrict__ blockDeepEnd, const int* __restrict__ blockPadEnd,
+                                                 const int* __restrict__ linePtr, const int* __restrict__ segs, const double* __restrict__ coef,
+                                                 int D, double* x, const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
+  const int t = 2 * blockIdx.x + color;
+  if (t >= nTasks) return;
+  SegRef S{segs};
+  for (int blk = taskPtr[t]; blk < taskPtr[t + 1]; blk++) {
+    const int r0 = blockBegin[blk], p0 = blockDeepEnd[blk];
+    const int nDeep = p0 - r0, nPad = blockPadEnd[blk] - p0;
+    for (int j = 0; j < nDeep + 3; j++) {
+      for (int gi = 0; gi < 3; gi++) {
+        int row = j - gi;
+        if (row >= 0 && row < nDeep) {
+          GsRow(S, linePtr[r0 + row], linePtr[r0 + row + 1], coef, D, x, b, K, ld);
+          __syncthreads();
+        }
+      }
+      if (withResidual) {
+        int pr = j - 3;
+        if (pr >= 0 && pr < nPad) {
+          ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
+          __syncthreads();
+        }
+      }
+    }
+    if (withResidual)
+      for (int pr = nDeep; pr < nPad; pr++) {
+        ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
+        __syncthreads();
+      }
+  }
+}
+
+// ------------------------------------------------------------------ boundary rows
+// vb = b_B - A_BD x   (IterativeSolvers.inl:410-419)
+__global__ void KBoundaryRhs(int B, const int* __restrict__ bgi, const int* __restrict__ rp, const int* __restrict__ col,
+                             const double* __restrict__ val, const double* __restrict__ x, const double* __restrict__ b, double* vb, int K,
+                             int ld) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)B * K) return;
+  int i = idx % B, c = idx / B;
+  const double* X = x + (size_t)c * ld;
+  double s = 0;
+  for (int k = rp[i]; k < rp[i + 1]; k++) s += X[col[k]] * val[k];
+  vb[(size_t)c * B + i] = b[(size_t)c * ld + bgi[i]] - s;
+}
+
+// r_B = b_B - A_BD x - A_BB x_B   (IterativeSolvers.inl:549-555)
+__global__ void KBoundaryResidual(int B, const int* __restrict__ bgi, const int* __restrict__ rpd, const int* __restrict__ cold,
+                                  const double* __restrict__ vald, const int* __restrict__ rpb, const int* __restrict__ colb,
+                                  const double* __restrict__ valb, const double* __restrict__ x, const double* __restrict__ b, double* r,
+                                  int K, int ld, int ldr) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)B * K) return;
+  int i = idx % B, c = idx / B;
+  const double* X = x + (size_t)c * ld;
+  double s = 0;
+  for (int k = rpd[i]; k < rpd[i + 1]; k++) s += X[cold[k]] * vald[k];
+  double v = b[(size_t)c * ld + bgi[i]] - s;
+  double t = 0;
+  for (int k = rpb[i]; k < rpb[i + 1]; k++) t += X[bgi[colb[k]]] * valb[k];
+  r[(size_t)c * ldr + bgi[i]] = v - t;
+}
+
+// ------------------------------------------------------------------ exact LDL^T solves
+__global__ void KLdltGather(int n, const int* __restrict__ perm, const double* __restrict__ src, int lds, double* y, int K) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * K) return;
+  int k = idx % n, c = idx / n;
+  y[(size_t)c * n + k] = src[(size_t)c * lds + perm[k]];
+}
+
+__global__ void KLdltScatter(int n, const int* __restrict__ perm, const int* __restrict__ map, const double* __restrict__ y, double* dst,
+                             int ldd, int K) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * K) return;
+  int k = idx % n, c = idx / n;
+  int t = perm[k];
+  if (map) t = map[t];
+  dst[(size_t)c * ldd + t] = y[(size_t)c * n + k];
+}
+
+__global__ void KLdltDiag(int n, const double* __restrict__ dinv, double* y, int K) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * K) return;
+  y[idx] *= dinv[idx % n];
+}
+
+// Teams: one workgroup per elimination subtree, its dependency levels separated by barriers.
+__global__ __launch_bounds__(256) void KLdltTeams(const int* __restrict__ teamLev, const int* __restrict__ levPtr,
+                                                  const int* __restrict__ rows, const int* __restrict__ rp, const int* __restrict__ cl,
+                                                  const double* __restrict__ vl, double* y, int n, int K) {
+  const int t = blockIdx.x;
+  for (int lv = teamLev[t]; lv < teamLev[t + 1]; lv++) {
+    const int r0 = levPtr[lv], cnt = levPtr[lv + 1] - r0;
+    for (int q = threadIdx.x; q < cnt * K; q += blockDim.x) {
+      int row = rows[r0 + q % cnt], c = q / cnt;
+      double* Y = y + (size_t)c * n;
+      double s = Y[row];
+      for (int k = rp[row]; k < rp[row + 1]; k++) s -= vl[k] * Y[cl[k]];
+      Y[row] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Top of the elimination tree: one workgroup, a wavefront per (row, column) dot product.
+__global__ __launch_bounds__(1024) void KLdltTop(int lev0, int lev1, const int* __restrict__ levPtr, const int* __restrict__ rows,
+                                                 const int* __restrict__ rp, const int* __restrict__ cl, const double* __restrict__ vl, double* y,
+                                                 int n, int K) {
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  for (int lv = lev0; lv < lev1; lv++) {
+    const int r0 = levPtr[lv], cnt = levPtr[lv + 1] - r0;
+    for (int q = wv; q < cnt * K; q += nw) {
+      int row = rows[r0 + q % cnt], c = q / cnt;
+      double* Y = y + (size_t)c * n;
+      double s = 0;
+      for (int k = rp[row] + lane; k < rp[row + 1]; k += kWave) s += vl[k] * Y[cl[k]];
+      for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+      if (lane == 0) Y[row] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ transfers
+// coarse boundary rhs = R_B r   (IterativeSolvers.inl:1027-1030)
+__global__ void KCsrGatherScatter(int rows, const int* __restrict__ outMap, const int* __restrict__ rp, const int* __restrict__ cl,
+                                  const double* __restrict__ vl, const double* __restrict__ in, int ldi, double* out, int ldo, int K) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)rows * K) return;
+  int i = idx % rows, c = idx / rows;
+  const double* X = in + (size_t)c * ldi;
+  double s = 0;
+  for (int k = rp[i]; k < rp[i + 1]; k++) s += X[cl[k]] * vl[k];
+  out[(size_t)c * ldo + outMap[i]] = s;
+}
+
+// coarse deep rhs: 3x3 full weighting around the fine texel 2c (IterativeSolvers.inl:1034-1050)
+__global__ void KRestrictDeep(int nLines, const int* __restrict__ lines, const double* __restrict__ in, int ldi, double* out, int ldo, int K) {
+  const int l = blockIdx.x;
+  if (l >= nLines) return;
+  const int* L = lines + 5 * l;
+  const int start = L[0], end = L[1], prev = L[2], next = L[3], outStart = L[4];
+  const int len = (end - start) / 2 + 1;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    const int j = 2 * i;
+    for (int c = 0; c < K; c++) {
+      const double* cu = in + (size_t)c * ldi + start;
+      const double* pv = in + (size_t)c * ldi + prev;
+      const double* nx = in + (size_t)c * ldi + next;
+      out[(size_t)c * ldo + outStart + i] = cu[j] + (pv[j] + cu[j - 1] + cu[j + 1] + nx[j]) * 0.5 +
+                                            (pv[j - 1] + pv[j + 1] + nx[j - 1] + nx[j + 1]) * 0.25;
+    }
+  }
+}
+
+// x_fine += P x_coarse, bilinear along prolongation lines (IterativeSolvers.inl:1122-1168)
+__global__ void KProlongAdd(int nLines, const int* __restrict__ lines, const double* __restrict__ in, int ldi, double* out, int ldo, int K) {
+  const int l = blockIdx.x;
+  if (l >= nLines) return;
+  const int* L = lines + 6 * l;
+  const int start = L[0], len = L[1], center = L[2], prev = L[3], next = L[4], off = L[5] ? 0 : 1;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    const int a = (i + off) / 2, bq = (i + off + 1) / 2;
+    for (int c = 0; c < K; c++) {
+      const double* X = in + (size_t)c * ldi;
+      double v = center != -1 ? (X[center + a] + X[center + bq]) * 0.5 : (X[prev + a] + X[prev + bq] + X[next + a] + X[next + bq]) * 0.25;
+      out[(size_t)c * ldo + start + i] += v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ operator application
+// Deep rows of y = A x with raw stencils (MultiplyBySystemMatrix_NoReciprocals, IterativeSolvers.inl:881-950);
+// with `b` given, writes the residual b - A x instead.
+__global__ void KApplyDeep(int nLines, const int* __restrict__ lines, const double* __restrict__ coef, int D, const double* __restrict__ x,
+                           const double* __restrict__ b, double* y, int K, int ld) {
+  const int l = blockIdx.x;
+  if (l >= nLines) return;
+  const int* L = lines + 5 * l;
+  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    double c[9];
+    for (int k = 0; k < 9; k++) c[k] = coef[(size_t)k * D + d0 + i];
+    for (int cc = 0; cc < K; cc++) {
+      const double* X = x + (size_t)cc * ld;
+      double s = X[prev + i - 1] * c[0] + X[prev + i] * c[1] + X[prev + i + 1] * c[2] + X[start + i - 1] * c[3] + X[start + i] * c[4] +
+                 X[start + i + 1] * c[5] + X[next + i - 1] * c[6] + X[next + i] * c[7] + X[next + i + 1] * c[8];
+      y[(size_t)cc * ld + start + i] = b ? b[(size_t)cc * ld + start + i] - s : s;
+    }
+  }
+}
+
+__global__ void KApplyBoundary(int B, const int* __restrict__ bgi, const int* __restrict__ rpd, const int* __restrict__ cold,
+                               const double* __restrict__ vald, const int* __restrict__ rpb, const int* __restrict__ colb,
+                               const double* __restrict__ valb, const double* __restrict__ x, const double* __restrict__ b, double* y, int K,
+                               int ld) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)B * K) return;
+  int i = idx % B, c = idx / B;
+  const double* X = x + (size_t)c * ld;
+  double t = 0;
+  for (int k = rpb[i]; k < rpb[i + 1]; k++) t += X[bgi[colb[k]]] * valb[k];
+  double s = 0;
+  for (int k = rpd[i]; k < rpd[i + 1]; k++) s += X[cold[k]] * vald[k];
+  double v = t + s;
+  y[(size_t)c * ld + bgi[i]] = b ? b[(size_t)c * ld + bgi[i]] - v : v;
+}
+
+__global__ void KZero(long n, double* x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ host side
+static inline unsigned Blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+DeviceSystem::~DeviceSystem() {
+  for (void* p : allocations) (void)hipFree(p);
+}
+
+template <class T>
+static T* Upload(DeviceSystem& ds, const std::vector<T>& v) {
+  if (v.empty()) return nullptr;
+  T* p = nullptr;
+  GDTP_HIP(hipMalloc(&p, v.size() * sizeof(T)));
+  ds.allocations.push_back(p);
+  GDTP_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+static double* Alloc(DeviceSystem& ds, size_t n) {
+  double* p = nullptr;
+  GDTP_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+  ds.allocations.push_back(p);
+  GDTP_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(double)));
+  return p;
+}
+
+static DevCsr UploadCsr(DeviceSystem& ds, const Csr& m) {
+  DevCsr d;
+  d.rows = m.rows, d.nnz = (int)m.col.size();
+  d.rowptr = Upload(ds, m.rowptr);
+  d.col = Upload(ds, m.col);
+  d.val = Upload(ds, m.val);
+  return d;
+}
+
+static std::vector<double> SoA10(const std::vector<double>& aos) {
+  size_t D = aos.size() / 10;
+  std::vector<double> s(aos.size());
+  for (size_t d = 0; d < D; d++)
+    for (int k = 0; k < 10; k++) s[k * D + d] = aos[10 * d + k];
+  return s;
+}
+
+// Partition the elimination tree into subtree teams (at most `smax` rows) plus the top part, and
+// group rows by dependency level inside each team and inside the top.
+static void UploadLdlt(DeviceSystem& ds, const Ldlt& F, DevLdlt& d) {
+  const int n = F.n;
+  d.n = n;
+  d.perm = Upload(ds, F.perm);
+  d.lrRowptr = Upload(ds, F.Lrows.rowptr), d.lrCol = Upload(ds, F.Lrows.col), d.lrVal = Upload(ds, F.Lrows.val);
+  d.lcRowptr = Upload(ds, F.Lcols.rowptr), d.lcCol = Upload(ds, F.Lcols.col), d.lcVal = Upload(ds, F.Lcols.val);
+  std::vector<double> dinv(n);
+  for (int i = 0; i < n; i++) dinv[i] = 1.0 / F.D[i];
+  d.dinv = Upload(ds, dinv);
+  std::vector<int> parent(n, -1), size(n, 1), team(n, -1);
+  for (int i = 0; i < n; i++)
+    if (F.Lcols.rowptr[i + 1] > F.Lcols.rowptr[i]) parent[i] = F.Lcols.col[F.Lcols.rowptr[i]];
+  for (int i = 0; i < n; i++)
+    if (parent[i] >= 0) size[parent[i]] += size[i];
+  const int smax = std::max(64, n / 256);
+  int nTeams = 0;
+  for (int v = n - 1; v >= 0; v--) {
+    if (size[v] > smax) continue;
+    if (parent[v] < 0 || size[parent[v]] > smax) team[v] = nTeams++;
+    else team[v] = team[parent[v]];
+  }
+  // forward levels (dependencies within the same team, or within the top)
+  std::vector<int> lev(n, 0);
+  for (int k = 0; k < n; k++) {
+    int l = 0;
+    for (int p = F.Lrows.rowptr[k]; p < F.Lrows.rowptr[k + 1]; p++) {
+      int i = F.Lrows.col[p];
+      if (team[i] == team[k]) l = std::max(l, lev[i] + 1);
+    }
+    lev[k] = l;
+  }
+  auto build = [&](std::vector<int>& levPtr, std::vector<int>& rows, std::vector<int>& teamLev, int& top0, int& top1) {
+    // group key (team, level); teams first in team order, then the top part
+    int G = nTeams + 1;
+    std::vector<int> maxLev(G, -1);
+    for (int k = 0; k < n; k++) {
+      int g = team[k] < 0 ? nTeams : team[k];
+      maxLev[g] = std::max(maxLev[g], lev[k]);
+    }
+    std::vector<int> groupBase(G + 1, 0);
+    for (int g = 0; g < G; g++) groupBase[g + 1] = groupBase[g] + maxLev[g] + 1;
+    int nGroups = groupBase[G];
+    std::vector<int> cnt(nGroups + 1, 0);
+    for (int k = 0; k < n; k++) {
+      int g = team[k] < 0 ? nTeams : team[k];
+      cnt[groupBase[g] + lev[k] + 1]++;
+    }
+    for (int i = 0; i < nGroups; i++) cnt[i + 1] += cnt[i];
+    levPtr = cnt;
+    rows.assign(n, 0);
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < n; k++) {
+      int g = team[k] < 0 ? nTeams : team[k];
+      rows[fillp[groupBase[g] + lev[k]]++] = k;
+    }
+    teamLev.assign(groupBase.begin(), groupBase.begin() + nTeams + 1);
+    top0 = groupBase[nTeams], top1 = groupBase[G];
+    int mr = 0;
+    for (int i = 0; i < groupBase[nTeams]; i++) mr = std::max(mr, levPtr[i + 1] - levPtr[i]);
+    d.maxTeamRows = std::max(d.maxTeamRows, mr);
+  };
+  std::vector<int> fLevPtr, fRows, fTeamLev, bLevPtr, bRows, bTeamLev;
+  build(fLevPtr, fRows, fTeamLev, d.topFwdLevBegin, d.topFwdLevEnd);
+  // backward levels: dependencies on ancestors in the same team (the top is solved first)
+  std::fill(lev.begin(), lev.end(), 0);
+  for (int i = n - 1; i >= 0; i--) {
+    int l = 0;
+    for (int p = F.Lcols.rowptr[i]; p < F.Lcols.rowptr[i + 1]; p++) {
+      int k = F.Lcols.col[p];
+      if (team[k] == team[i]) l = std::max(l, lev[k] + 1);
+    }
+    lev[i] = l;
+  }
+  build(bLevPtr, bRows, bTeamLev, d.topBwdLevBegin, d.topBwdLevEnd);
+  d.nTeams = nTeams;
+  d.fwdLevPtr = Upload(ds, fLevPtr), d.fwdRows = Upload(ds, fRows), d.teamFwdLev = Upload(ds, fTeamLev);
+  d.bwdLevPtr = Upload(ds, bLevPtr), d.bwdRows = Upload(ds, bRows), d.teamBwdLev = Upload(ds, bTeamLev);
+}
+
+void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds) {
+  const int nl = (int)h.levels.size();
+  ds.levels = nl, ds.kcap = kcap;
+  ds.L.assign(nl, DevLevel());
+  for (int l = 0; l < nl; l++) {
+    const Level& H = h.levels[l];
+    const LevelOperator& O = oh.levels[l];
+    DevLevel& d = ds.L[l];
+    d.N = H.numTexels, d.D = H.numDeep, d.B = H.numBoundary;
+    d.coef = Upload(ds, SoA10(O.deepRecip));
+    d.coefRaw = Upload(ds, SoA10(O.deepRaw));
+    d.bgi = Upload(ds, H.boundaryGlobalIndex);
+    d.bd = UploadCsr(ds, O.bd);
+    d.bb = UploadCsr(ds, O.bb);
+    if (l + 1 < nl) d.brestr = UploadCsr(ds, H.boundaryRestriction);
+    d.nTasks = (int)H.tasks.taskPtr.size() - 1;
+    d.taskPtr = Upload(ds, H.tasks.taskPtr);
+    d.blockBegin = Upload(ds, H.tasks.blockBegin);
+    d.blockDeepEnd = Upload(ds, H.tasks.blockDeepEnd);
+    d.blockPadEnd = Upload(ds, H.tasks.blockPadEnd);
+    d.linePtr = Upload(ds, H.tasks.linePtr);
+    std::vector<int> segs;
+    segs.reserve(H.tasks.segments.size() * 5);
+    for (auto& s : H.tasks.segments) segs.insert(segs.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart});
+    d.segs = Upload(ds, segs);
+    std::vector<int> rl;
+    for (auto& x : H.rasterLines) rl.insert(rl.end(), {x.start, x.end, x.prev, x.next, x.coeff});
+    d.nRaster = (int)H.rasterLines.size();
+    d.raster = Upload(ds, rl);
+    rl.clear();
+    for (auto& x : H.restrictionLines) rl.insert(rl.end(), {x.start, x.end, x.prev, x.next, x.coeff});
+    d.nRestr = (int)H.restrictionLines.size();
+    d.restr = Upload(ds, rl);
+    rl.clear();
+    for (auto& x : H.prolongationLines) rl.insert(rl.end(), {x.start, x.length, x.center, x.prev, x.next, x.aligned});
+    d.nProl = (int)H.prolongationLines.size();
+    d.prol = Upload(ds, rl);
+    if (l + 1 < nl) UploadLdlt(ds, oh.boundaryFactor[l], d.factor);
+    else UploadLdlt(ds, oh.coarseFactor, d.factor);
+    d.x = Alloc(ds, (size_t)d.N * kcap);
+    d.b = Alloc(ds, (size_t)d.N * kcap);
+    d.r = Alloc(ds, (size_t)d.N * kcap);
+    d.vb = Alloc(ds, (size_t)std::max(d.B, 1) * kcap);
+    d.y = Alloc(ds, (size_t)std::max(d.factor.n, 1) * kcap);
+  }
+  for (int which = 0; which < 2; which++) {
+    DevLevel& d = which == 0 ? ds.mass0 : ds.stiff0;
+    const DevLevel& l0 = ds.L[0];
+    d.N = l0.N, d.D = l0.D, d.B = l0.B, d.bgi = l0.bgi, d.nRaster = l0.nRaster, d.raster = l0.raster;
+    d.coefRaw = Upload(ds, SoA10(which == 0 ? op.deepMass : op.deepStiffness));
+    d.bd = UploadCsr(ds, which == 0 ? op.bdMass : op.bdStiffness);
+    d.bb = UploadCsr(ds, which == 0 ? op.bbMass : op.bbStiffness);
+  }
+}
+
+static void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s) {
+  const int n = F.n;
+  if (n == 0) return;
+  const long nk = (long)n * K;
+  KLdltGather<<<Blocks(nk, 256), 256, 0, s>>>(n, F.perm, src, lds, y, K);
+  if (F.nTeams) KLdltTeams<<<F.nTeams, 256, 0, s>>>(F.teamFwdLev, F.fwdLevPtr, F.fwdRows, F.lrRowptr, F.lrCol, F.lrVal, y, n, K);
+  if (F.topFwdLevEnd > F.topFwdLevBegin)
+    KLdltTop<<<1, 1024, 0, s>>>(F.topFwdLevBegin, F.topFwdLevEnd, F.fwdLevPtr, F.fwdRows, F.lrRowptr, F.lrCol, F.lrVal, y, n, K);
+  KLdltDiag<<<Blocks(nk, 256), 256, 0, s>>>(n, F.dinv, y, K);
+  if (F.topBwdLevEnd > F.topBwdLevBegin)
+    KLdltTop<<<1, 1024, 0, s>>>(F.topBwdLevBegin, F.topBwdLevEnd, F.bwdLevPtr, F.bwdRows, F.lcRowptr, F.lcCol, F.lcVal, y, n, K);
+  if (F.nTeams) KLdltTeams<<<F.nTeams, 256, 0, s>>>(F.teamBwdLev, F.bwdLevPtr, F.bwdRows, F.lcRowptr, F.lcCol, F.lcVal, y, n, K);
+  KLdltScatter<<<Blocks(nk, 256), 256, 0, s>>>(n, F.perm, map, y, dst, ldd, K);
+}
+
+static void CheckK(const DeviceSystem& ds, int K) {
+  if (K < 1 || K > ds.kcap) throw std::runtime_error("column count outside [1, capacity]");
+}
+
+void BoundarySolveLevel(DeviceSystem& ds, int l, const double* rhs, double* sol, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& d = ds.L[l];
+  if (l + 1 >= ds.levels) throw std::runtime_error("the coarsest level has no boundary solver");
+  LdltSolve(d.factor, rhs, ld, sol, ld, nullptr, d.y, K, s);
+}
+
+void CoarseSolve(DeviceSystem& ds, const double* rhs, double* sol, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& d = ds.L[ds.levels - 1];
+  LdltSolve(d.factor, rhs, ld, sol, ld, nullptr, d.y, K, s);
+}
+
+void BandSweepLevel(DeviceSystem& ds, int l, double* x, const double* b, double* r, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& d = ds.L[l];
+  if (d.nTasks == 0) return;
+  const int withResidual = r != nullptr ? 1 : 0;
+  KBandGs<<<(d.nTasks + 1) / 2, kWave, 0, s>>>(0, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs, d.coef,
+                                             d.D, x, b, r, K, ld, withResidual);
+  if (d.nTasks > 1)
+    KBandGs<<<d.nTasks / 2, kWave, 0, s>>>(1, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs, d.coef,
+                                         d.D, x, b, r, K, ld, withResidual);
+}
+
+void RelaxLevel(DeviceSystem& ds, int l, double* x, const double* b, double* r, int K, int ld, bool withResidual, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& d = ds.L[l];
+  if (d.B > 0) {
+    const long bk = (long)d.B * K;
+    KBoundaryRhs<<<Blocks(bk, 256), 256, 0, s>>>(d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, x, b, d.vb, K, ld);
+    LdltSolve(d.factor, d.vb, d.B, x, ld, d.bgi, d.y, K, s);
+  }
+  BandSweepLevel(ds, l, x, b, withResidual ? r : nullptr, K, ld, s);
+  if (withResidual && d.B > 0) {
+    const long bk = (long)d.B * K;
+    KBoundaryResidual<<<Blocks(bk, 256), 256, 0, s>>>(d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, d.bb.rowptr, d.bb.col, d.bb.val, x, b, r,
+                                                       K, ld, ld);
+  }
+}
+
+void RestrictLevel(DeviceSystem& ds, int l, const double* rF, double* bC, int K, int ldF, int ldC, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel &f = ds.L[l], &c = ds.L[l + 1];
+  if (c.B > 0)
+    KCsrGatherScatter<<<Blocks((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bgi, f.brestr.rowptr, f.brestr.col, f.brestr.val, rF, ldF, bC, ldC, K);
+  if (c.nRestr > 0) KRestrictDeep<<<c.nRestr, 128, 0, s>>>(c.nRestr, c.restr, rF, ldF, bC, ldC, K);
+}
+
+void ProlongAddLevel(DeviceSystem& ds, int l, const double* xC, double* xF, int K, int ldC, int ldF, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& f = ds.L[l];
+  if (f.nProl > 0) KProlongAdd<<<f.nProl, 128, 0, s>>>(f.nProl, f.prol, xC, ldC, xF, ldF, K);
+}
+
+void ApplyLevel(DeviceSystem& ds, int l, const double* x, double* y, int K, int ld, hipStream_t s) {
+  ApplyOperator(ds, ds.L[l], x, y, K, ld, s);
+}
+
+void ApplyOperator(DeviceSystem& ds, DevLevel& d, const double* x, double* y, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  if (d.B > 0)
+    KApplyBoundary<<<Blocks((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, d.bb.rowptr, d.bb.col, d.bb.val, x,
+                                                              nullptr, y, K, ld);
+  if (d.nRaster > 0) KApplyDeep<<<d.nRaster, 128, 0, s>>>(d.nRaster, d.raster, d.coefRaw, d.D, x, nullptr, y, K, ld);
+}
+
+void ResidualLevel(DeviceSystem& ds, int l, const double* x, const double* b, double* r, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  DevLevel& d = ds.L[l];
+  if (d.B > 0)
+    KApplyBoundary<<<Blocks((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, d.bb.rowptr, d.bb.col, d.bb.val, x, b,
+                                                              r, K, ld);
+  if (d.nRaster > 0) KApplyDeep<<<d.nRaster, 128, 0, s>>>(d.nRaster, d.raster, d.coefRaw, d.D, x, b, r, K, ld);
+}
+
+void VCycleReference(DeviceSystem& ds, double* x, const double* b, int K, int ld, hipStream_t s) {
+  CheckK(ds, K);
+  const int nl = ds.levels;
+  for (int l = 1; l < nl; l++) {
+    long n = (long)ds.L[l].N * K;
+    KZero<<<Blocks(n, 256), 256, 0, s>>>(n, ds.L[l].x);
+  }
+  auto X = [&](int l) { return l == 0 ? x : ds.L[l].x; };
+  auto Bv = [&](int l) { return l == 0 ? const_cast<double*>(b) : ds.L[l].b; };
+  auto LD = [&](int l) { return l == 0 ? ld : ds.L[l].N; };
+  for (int l = 0; l + 1 < nl; l++) {
+    RelaxLevel(ds, l, X(l), Bv(l), ds.L[l].r, K, LD(l), true, s);
+    // the level residual lives in r with the level's own stride
+    RestrictLevel(ds, l, ds.L[l].r, ds.L[l + 1].b, K, LD(l), ds.L[l + 1].N, s);
+  }
+  CoarseSolve(ds, ds.L[nl - 1].b, ds.L[nl - 1].x, K, ds.L[nl - 1].N, s);
+  for (int l = nl - 2; l >= 0; l--) {
+    ProlongAddLevel(ds, l, ds.L[l + 1].x, X(l), K, ds.L[l + 1].N, LD(l), s);
+    RelaxLevel(ds, l, X(l), Bv(l), nullptr, K, LD(l), false, s);
+  }
+}
+
+}  // namespace gdtp

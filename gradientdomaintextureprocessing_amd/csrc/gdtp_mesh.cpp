@@ -58,6 +58,7 @@ int MakeCube(int subdiv, int W, int H, TexturedMesh& m) {
   // quarter texel away from texel centres.
   const double cellW = double(W) / 3.0, cellH = double(H) / 2.0;
   const double s = n * std::floor((std::min(cellW, cellH) - 8.0) / n);
+  if (s < n) return 0;  // fewer than one texel per quad
   for (int f = 0; f < 6; f++) {
     const int* F = faces[f];
     const int col = f % 3, row = f / 3;

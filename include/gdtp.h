@@ -1,0 +1,100 @@
+/* gdtp: texel-domain screened-Poisson multigrid on MI355X (gfx950) — C ABI.
+ *
+ * Conventions
+ *   - Every entry point returns GDTP_OK (0) or a nonzero status; gdtp_last_error() gives the
+ *     message of the last failure on the calling thread.  (The reference returns int 1/0 and
+ *     printf()s; include/Src/IterativeSolvers.inl:1211, UpdateCoefficients.inl:463.)
+ *   - Texel vectors are column-major: `ncols` columns of N_level doubles, contiguous
+ *     ([ncols][N]); device entry points take device pointers and an optional hipStream_t
+ *     passed as void* (NULL = default stream).  Texels are in the reference's global raster order
+ *     (charts in order, rows j ascending, columns i ascending; HierarchyConstruction.inl:259-290).
+ *   - A handle is externally synchronised; distinct handles may be used from distinct threads.
+ */
+#ifndef GDTP_H
+#define GDTP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GDTP_OK = 0,
+  GDTP_ERROR_INVALID = 1,   /* bad argument or handle */
+  GDTP_ERROR_GEOMETRY = 2,  /* mesh / atlas / hierarchy construction failed */
+  GDTP_ERROR_NUMERIC = 3,   /* factorisation or solve failed (e.g. not positive definite) */
+  GDTP_ERROR_DEVICE = 4     /* HIP runtime error or no device */
+};
+
+enum { GDTP_METRIC_EMBEDDING = 0, GDTP_METRIC_UNIFORM = 1 };
+enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS = 2 };
+enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
+
+typedef struct gdtp_mesh gdtp_mesh;
+typedef struct gdtp_hierarchy gdtp_hierarchy;
+typedef struct gdtp_system gdtp_system;
+
+const char* gdtp_last_error(void);
+int gdtp_device_count(int* count);
+
+/* ---- meshes with per-corner texture coordinates (reference: TexturedMesh,
+ *      include/Src/SimpleMesh.h:45-49, read from PLY at :118-160) */
+int gdtp_mesh_create(const double* vertices, int num_vertices, const int* triangles, int num_triangles,
+                     const double* corner_uv, gdtp_mesh** out);
+int gdtp_mesh_create_cube(int subdiv, int width, int height, gdtp_mesh** out);
+int gdtp_mesh_create_icosphere(int levels, int width, int height, double amplitude, unsigned long long seed, gdtp_mesh** out);
+int gdtp_mesh_create_torus(int nu, int nv, int charts_u, int charts_v, double major_radius, double minor_radius,
+                           double amplitude, unsigned long long seed, int width, int height, gdtp_mesh** out);
+int gdtp_mesh_counts(const gdtp_mesh* mesh, int* num_vertices, int* num_triangles);
+int gdtp_mesh_read(const gdtp_mesh* mesh, double* vertices, int* triangles, double* corner_uv);
+void gdtp_mesh_destroy(gdtp_mesh* mesh);
+
+/* ---- hierarchy and finest-level mass / stiffness
+ * Replaces InitializeHierarchy (HierarchyConstruction.inl:799-849), InitializeBoundaryProlongationData
+ * (BoundaryQuadraticElements.inl:775-781), InitializeMetric (Metric.inl:448-466) and
+ * InitializeMassAndStiffness (MassAndStiffness.h:490-549).  Block shape 128x16, padding 2x0. */
+int gdtp_hierarchy_create(const gdtp_mesh* mesh, int width, int height, int levels, int metric_mode, gdtp_hierarchy** out);
+/* info[0..7] = texels, deep, boundary, charts, raster lines, band tasks, blocks, segments */
+int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info);
+/* Named integer / double arrays of the hierarchy (see gdtp_capi.cpp for the list); call with
+ * out == NULL to get the element count. */
+int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* name, int* out, size_t* count);
+int gdtp_hierarchy_array_f64(const gdtp_hierarchy* h, int level, const char* name, double* out, size_t* count);
+void gdtp_hierarchy_destroy(gdtp_hierarchy* h);
+
+/* ---- screened-Poisson operator A = screen*M + stiffness*S and its multigrid hierarchy
+ * Replaces UpdateLinearSystem (UpdateCoefficients.inl:463-513): Galerkin coarsening, reciprocal
+ * deep stencils, exact boundary and coarse LDL^T factorisations.  max_cols = 0 builds the host
+ * operator only (no device); otherwise device buffers for up to max_cols right-hand sides. */
+int gdtp_system_create(const gdtp_hierarchy* h, double screen, double stiffness, int max_cols, gdtp_system** out);
+int gdtp_system_update(gdtp_system* s, double screen, double stiffness);
+int gdtp_system_array_i32(const gdtp_system* s, int level, const char* name, int* out, size_t* count);
+int gdtp_system_array_f64(const gdtp_system* s, int level, const char* name, double* out, size_t* count);
+void gdtp_system_destroy(gdtp_system* s);
+
+/* ---- device operations (FP64, the reference schedule) */
+/* VCycle (IterativeSolvers.inl:1211-1333): x (level 0, in/out), b (level 0). */
+int gdtp_vcycle(gdtp_system* s, double* x, const double* b, int ncols, void* stream);
+/* RelaxationAndResidual (:398-558) when residual != NULL, else Relaxation (:174-395). */
+int gdtp_relax(gdtp_system* s, int level, double* x, const double* b, double* residual, int ncols, void* stream);
+/* Deep-texel part of one relaxation alone (the "Update Interior" pass of :174-558): the
+ * two-coloured band sweep, with the fused deep residual when residual != NULL. */
+int gdtp_band_sweep(gdtp_system* s, int level, double* x, const double* b, double* residual, int ncols, void* stream);
+/* MultiplyByRestriction (:1009-1071): fine residual of `level` -> coarse rhs of level+1. */
+int gdtp_restrict(gdtp_system* s, int level, const double* fine, double* coarse, int ncols, void* stream);
+/* AddProlongation (:1122-1168): fine(level) += P coarse(level+1). */
+int gdtp_prolong_add(gdtp_system* s, int level, const double* coarse, double* fine, int ncols, void* stream);
+/* solve(boundarySolver, ...) (Solver.h:191-203): boundary-indexed vectors of `level`. */
+int gdtp_boundary_solve(gdtp_system* s, int level, const double* rhs, double* x, int ncols, void* stream);
+/* solve(coarseSolver, ...) (IterativeSolvers.inl:1316). */
+int gdtp_coarse_solve(gdtp_system* s, const double* rhs, double* x, int ncols, void* stream);
+/* MultiplyBySystemMatrix_NoReciprocals (:881-950) with A, M or S on level 0. */
+int gdtp_apply(gdtp_system* s, int which, const double* x, double* y, int ncols, void* stream);
+/* r = b - A x on level 0 (ComputeSystemResdiual, :953-1007). */
+int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, int ncols, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDTP_H */
