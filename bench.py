@@ -19,10 +19,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 CONFIGS = {
-    # name: (mesh kind, params, width, levels, stiffness)
-    "c2": ("icosphere", dict(levels=7, amplitude=0.05, seed=1), 4096, 4, 1e-3),
-    "c1": ("cube", dict(subdiv=16), 512, 4, 1e-3),
-    "small": ("icosphere", dict(levels=5, amplitude=0.05, seed=1), 1024, 4, 1e-3),
+    # name: (mesh kind, params, width, reference levels, mixed-solver levels, stiffness)
+    "c2": ("icosphere", dict(levels=7, amplitude=0.05, seed=1), 4096, 4, 8, 1e-3),
+    "c1": ("cube", dict(subdiv=16), 512, 4, 6, 1e-3),
+    "small": ("icosphere", dict(levels=5, amplitude=0.05, seed=1), 1024, 4, 7, 1e-3),
 }
 
 HBM_PEAK = 8.0e12
@@ -32,9 +32,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_system(cfg, K):
+def make_system(cfg, K, mixed):
     import gradientdomaintextureprocessing_amd as g
-    kind, p, W, levels, stiff = CONFIGS[cfg]
+    kind, p, W, ref_levels, mix_levels, stiff = CONFIGS[cfg]
+    levels = mix_levels if mixed else ref_levels
     t0 = time.time()
     if kind == "icosphere":
         mesh = g.TexturedMesh.icosphere(p["levels"], W, W, p["amplitude"], p["seed"])
@@ -44,7 +45,7 @@ def make_system(cfg, K):
     t1 = time.time()
     S = g.UpdateLinearSystem(H, stiff, 1.0, max_cols=K)
     t2 = time.time()
-    log(f"[bench] setup: hierarchy+assembly {t1 - t0:.1f}s, coarsening+factorisation+upload {t2 - t1:.1f}s")
+    log(f"[bench] setup ({levels} levels): hierarchy+assembly {t1 - t0:.1f}s, coarsening+factorisation+upload {t2 - t1:.1f}s")
     return mesh, H, S
 
 
@@ -58,18 +59,35 @@ def texture_rhs(H, S, K, seed=1):
     return f, b
 
 
-def solve_fp64(S, x, b, r, tol, max_cycles):
-    """Plain FP64 reference V-cycles until every column reaches tol."""
-    import torch
+def solve(S, x, b, tol, max_cycles, mode):
+    """One step: from x = 0 until every column reaches tol (reference V-cycles or mixed refinement)."""
     x.zero_()
-    bn = torch.linalg.vector_norm(b, dim=1)
-    for it in range(1, max_cycles + 1):
-        S.vcycle(x, b)
-        S.residual(x, b, r)
-        rel = (torch.linalg.vector_norm(r, dim=1) / bn).max().item()
-        if rel <= tol:
-            return it, rel
-    return max_cycles, rel
+    its, rel = S.solve(x, b, tol=tol, max_iters=max_cycles, mode=mode)
+    return its, float(rel.max())
+
+
+def mixed_roofline(S, H, K, reps=10):
+    """Average duration of one colour-plane launch of the four-colour deep Gauss-Seidel kernel
+    (dominant kernel of the mixed path) on level 0, and its algorithmic bytes: per updated deep
+    texel nine FP32 stencil words, rhs read and e written (K columns); plus one read of the other
+    three colour planes of e at active texels."""
+    import torch
+    info = H.level_info(0)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    S.mixed_deep_sweep(0, K)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        S.mixed_deep_sweep(0, K)
+    e1.record(st)
+    torch.cuda.synchronize()
+    launches = 4 * reps
+    per_launch_s = e0.elapsed_time(e1) / 1e3 / launches
+    bytes_per_launch = info["deep"] / 4 * (9 * 4 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
+    achieved = bytes_per_launch / per_launch_s
+    return dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK, traffic=None,
+                kernel="KMixDeepGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)
 
 
 def band_roofline(S, H, x, b, r, reps=5):
@@ -135,7 +153,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--mode", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
@@ -146,41 +164,52 @@ def main():
         sys.exit(2)
     import torch
     K = 3
-    mesh, H, S = make_system(args.config, K)
-    f, b = texture_rhs(H, S, K)
-    x, r = torch.zeros_like(b), torch.zeros_like(b)
     max_cycles = 40
-
-    def step():
-        return solve_fp64(S, x, b, r, args.tol, max_cycles)
-
-    for _ in range(args.warmup):
-        cycles, rel = step()
-        log(f"[bench] warmup: {cycles} cycles, rel res {rel:.2e}")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    cyc = []
-    for _ in range(args.steps):
-        c, rel = step()
-        cyc.append(c)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms = el / args.steps * 1e3
-    value = K * args.steps / el
-    log(f"[bench] {args.steps} steps in {el:.3f}s, cycles/step {cyc}, final rel res {rel:.2e}")
-    roof = band_roofline(S, H, x, b, r)
-    log(f"[bench] roofline: {roof}")
-    cpu = cpu_baseline(H, S, K, args.tol) if args.cpu_baseline == "on" else None
-    if cpu:
-        log(f"[bench] cpu baseline: {cpu}")
+    results = {}
+    modes = ["mixed", "fp64"] if args.mode == "mixed" else ["fp64"]
+    for mode in modes:
+        mesh, H, S = make_system(args.config, K, mixed=(mode == "mixed"))
+        f, b = texture_rhs(H, S, K)
+        x, r = torch.zeros_like(b), torch.zeros_like(b)
+        for _ in range(args.warmup):
+            cycles, rel = solve(S, x, b, args.tol, max_cycles, mode)
+            log(f"[bench] {mode} warmup: {cycles} iterations, rel res {rel:.2e}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cyc = []
+        for _ in range(args.steps):
+            c, rel = solve(S, x, b, args.tol, max_cycles, mode)
+            cyc.append(c)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res = dict(value=K * args.steps / el, ms_per_step=el / args.steps * 1e3, iterations=cyc, rel_res=rel, levels=H.levels)
+        log(f"[bench] {mode}: {args.steps} steps in {el:.3f}s ({res['value']:.1f} solves/s), iterations/step {cyc}, "
+            f"final rel res {rel:.2e}")
+        if mode == "mixed":
+            res["roofline"] = mixed_roofline(S, H, K)
+        else:
+            res["roofline"] = band_roofline(S, H, x, b, r)
+        log(f"[bench] {mode} roofline: {res['roofline']}")
+        if mode == "fp64" and args.cpu_baseline == "on":
+            res["cpu"] = cpu_baseline(H, S, K, args.tol)
+            log(f"[bench] cpu baseline: {res['cpu']}")
+        results[mode] = (res, H)
+        del S, H, mesh
+    main_mode = modes[0]
+    res, H = results[main_mode]
+    value, ms, cyc, roof = res["value"], res["ms_per_step"], res["iterations"], res["roofline"]
+    cpu = results["fp64"][0].get("cpu")
     info = H.level_info(0)
     out = {
         "metric": "solves/s at rel-residual 1e-6 (4096² atlas) vs FP64 GPU; p50/p99 req latency",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 relax / f64 refine" if main_mode == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {CONFIGS[args.config][0]} {CONFIGS[args.config][2]}^2 atlas, "
                                f"{info['texels']} texels, 3 RHS, (M+1e-3 S)x=Mf from x=0 to rel-res {args.tol:g}",
-                   "mode": args.mode, "levels": H.levels, "cycles_per_step": cyc},
+                   "mode": main_mode, "levels": H.levels, "iterations_per_step": cyc},
+        "fp64_reference_gpu": {k: v for k, v in results["fp64"][0].items() if k in ("value", "ms_per_step", "iterations", "levels")},
+        "speedup_vs_fp64_gpu": value / results["fp64"][0]["value"],
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

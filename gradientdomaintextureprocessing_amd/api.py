@@ -172,6 +172,26 @@ class MultigridSystem:
     def residual(self, x, b, r):
         _lib.check(_lib.load().gdtp_residual(self._h, _ptr(x), _ptr(b), _ptr(r), x.shape[0], _stream()))
 
+    def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed"):
+        """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res)."""
+        rel = np.zeros(x.shape[0])
+        it = ctypes.c_int(0)
+        m = 1 if mode == "mixed" else 0
+        _lib.check(_lib.load().gdtp_solve(self._h, _ptr(x), _ptr(b), x.shape[0], tol, max_iters, m, rel.ctypes.data,
+                                          ctypes.byref(it), _stream()))
+        return it.value, rel
+
+    def configure_mixed(self, nu_pre=2, nu_post=2, boundary_sweeps=4, omega=0.7, use_graph=True):
+        _lib.check(_lib.load().gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
+
+    def mixed_deep_sweep(self, level, ncols):
+        _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
+
+    def memory(self):
+        r, m = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(_lib.load().gdtp_system_memory(self._h, ctypes.byref(r), ctypes.byref(m)))
+        return r.value, m.value
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.gdtp_system_destroy(self._h)

@@ -6,6 +6,7 @@
 
 #include "../../include/gdtp.h"
 #include "gdtp_device.h"
+#include "gdtp_mixed.h"
 #include "gdtp_mesh.h"
 
 using namespace gdtp;
@@ -26,6 +27,15 @@ struct gdtp_system {
   int kcap = 0;
   OperatorHierarchy oh;
   std::unique_ptr<DeviceSystem> ds;
+  std::unique_ptr<MixedSystem> mixed;
+  int nuPre = 2, nuPost = 2, boundarySweeps = 4, useGraph = 1;
+  double omega = 0.7;
+  double* dNorms = nullptr;
+  double* hNorms = nullptr;
+  ~gdtp_system() {
+    if (dNorms) (void)hipFree(dNorms);
+    if (hNorms) (void)hipHostFree(hNorms);
+  }
 };
 
 static thread_local std::string g_error;
@@ -227,6 +237,7 @@ static int BuildSystem(gdtp_system* s, double screen, double stiff) {
     ds.reset(new DeviceSystem());
     UploadSystem(s->H->h, oh, s->H->op, s->kcap, *ds);
   }
+  s->mixed.reset();
   s->oh = std::move(oh);
   s->ds = std::move(ds);
   s->screen = screen, s->stiff = stiff;
@@ -341,3 +352,77 @@ int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, i
   GDTP_DEVICE_OP(x && b && r, ResidualLevel(ds, 0, x, b, r, ncols, ds.L[0].N, st))
 }
 
+static MixedSystem& Mixed(gdtp_system* s) {
+  if (!s->mixed) {
+    std::unique_ptr<MixedSystem> m(new MixedSystem());
+    UploadMixed(s->H->h, s->oh, *s->ds, s->kcap, *m);
+    s->mixed = std::move(m);
+  }
+  MixedSystem& m = *s->mixed;
+  m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
+  return m;
+}
+
+int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySweeps, double omega, int useGraph) {
+  if (!s || nuPre < 0 || nuPost < 0 || nuPre + nuPost == 0 || boundarySweeps < 0 || !(omega > 0 && omega <= 1.5))
+    return Fail(GDTP_ERROR_INVALID, "invalid mixed-solver configuration");
+  s->nuPre = nuPre, s->nuPost = nuPost, s->boundarySweeps = boundarySweeps, s->omega = omega, s->useGraph = useGraph ? 1 : 0;
+  return GDTP_OK;
+}
+
+int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream) {
+  GDTP_DEVICE_OP(level >= 0 && level + 1 < s->ds->levels, MixedDeepSweep(Mixed(s), level, ncols, st))
+}
+
+int gdtp_system_memory(const gdtp_system* s, size_t* refBytes, size_t* mixedBytes) {
+  if (!s) return Fail(GDTP_ERROR_INVALID, "null system");
+  if (refBytes) *refBytes = s->ds ? s->ds->bytes : 0;
+  if (mixedBytes) *mixedBytes = 0;
+  return GDTP_OK;
+}
+
+int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int maxIters, int mode, double* relRes, int* iterations,
+               void* stream) {
+  int rc = Ready(s, ncols);
+  if (rc != GDTP_OK) return rc;
+  if (!x || !b || !(tol > 0) || maxIters < 0 || (mode != GDTP_SOLVE_FP64_REFERENCE && mode != GDTP_SOLVE_MIXED))
+    return Fail(GDTP_ERROR_INVALID, "invalid solve arguments");
+  GDTP_TRY({
+    hipStream_t st = (hipStream_t)stream;
+    DeviceSystem& ds = *s->ds;
+    std::vector<double> rel(ncols, 0.0);
+    int it = 0;
+    bool ok = false;
+    if (mode == GDTP_SOLVE_MIXED) {
+      MixedSystem& m = Mixed(s);
+      it = SolveMixed(ds, m, x, b, ncols, tol, maxIters, rel.data(), st, s->useGraph != 0);
+      ok = true;
+      for (double r : rel) ok = ok && r <= tol;
+    } else {
+      if (!s->dNorms) {
+        GDTP_HIP(hipMalloc(&s->dNorms, 2 * s->kcap * sizeof(double)));
+        GDTP_HIP(hipHostMalloc((void**)&s->hNorms, 2 * s->kcap * sizeof(double), hipHostMallocDefault));
+      }
+      const int N = ds.L[0].N;
+      SquaredNorms(b, N, ncols, N, s->dNorms + s->kcap, st);
+      for (;; it++) {
+        ResidualLevel(ds, 0, x, b, ds.L[0].r, ncols, N, st);
+        SquaredNorms(ds.L[0].r, N, ncols, N, s->dNorms, st);
+        GDTP_HIP(hipMemcpyAsync(s->hNorms, s->dNorms, 2 * s->kcap * sizeof(double), hipMemcpyDeviceToHost, st));
+        GDTP_HIP(hipStreamSynchronize(st));
+        ok = true;
+        for (int k = 0; k < ncols; k++) {
+          rel[k] = std::sqrt(s->hNorms[k]) / std::sqrt(std::max(s->hNorms[s->kcap + k], 1e-300));
+          ok = ok && rel[k] <= tol;
+        }
+        if (ok || it >= maxIters) break;
+        VCycleReference(ds, x, b, ncols, N, st);
+      }
+    }
+    GDTP_HIP(hipGetLastError());
+    if (relRes) for (int k = 0; k < ncols; k++) relRes[k] = rel[k];
+    if (iterations) *iterations = it;
+    if (!ok) return Fail(GDTP_ERROR_NUMERIC, "solve did not reach the tolerance within max_iters");
+    return GDTP_OK;
+  })
+}

@@ -58,9 +58,15 @@ struct DeviceSystem {
   std::vector<DevLevel> L;
   DevLevel mass0, stiff0;      // level-0 mass and stiffness (raw stencils and boundary rows only)
   std::vector<void*> allocations;
+  size_t bytes = 0;
   hipStream_t stream = nullptr;
   ~DeviceSystem();
 };
+
+// Upload an LDL^T factor with its team / top level schedule; LdltSolve solves for K columns,
+// gathering src[perm] and scattering into dst (through `map` when not null).
+void UploadLdlt(std::vector<void*>& allocations, const Ldlt& F, DevLdlt& d);
+void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s);
 
 void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds);
 
@@ -78,6 +84,8 @@ void CoarseSolve(DeviceSystem& ds, const double* rhs, double* sol, int K, int ld
 // y = A x with raw (non-reciprocal) coefficients of `level`; residual r = b - A x.
 void ApplyLevel(DeviceSystem& ds, int level, const double* x, double* y, int K, int ld, hipStream_t s);
 void ApplyOperator(DeviceSystem& ds, DevLevel& op, const double* x, double* y, int K, int ld, hipStream_t s);
+// Squared 2-norms of K columns (stride ld) accumulated into dNorms[0..K) (zeroed first).
+void SquaredNorms(const double* v, int N, int K, int ld, double* dNorms, hipStream_t s);
 void ResidualLevel(DeviceSystem& ds, int level, const double* x, const double* b, double* r, int K, int ld, hipStream_t s);
 
 }  // namespace gdtp

@@ -323,6 +323,18 @@ __global__ void KApplyBoundary(int B, const int* __restrict__ bgi, const int* __
   y[(size_t)c * ld + bgi[i]] = b ? b[(size_t)c * ld + bgi[i]] - v : v;
 }
 
+__global__ __launch_bounds__(256) void KSquaredNorm(int N, const double* __restrict__ v, int ld, double* out) {
+  const int c = blockIdx.y;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = i < N ? v[(long)c * ld + i] : 0.0;
+  a *= a;
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  __shared__ double part[4];
+  if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out + c, part[0] + part[1] + part[2] + part[3]);
+}
+
 __global__ void KZero(long n, double* x) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = 0.0;
@@ -335,19 +347,28 @@ DeviceSystem::~DeviceSystem() {
   for (void* p : allocations) (void)hipFree(p);
 }
 
+static thread_local size_t g_uploadBytes = 0;
+
 template <class T>
-static T* Upload(DeviceSystem& ds, const std::vector<T>& v) {
+static T* UploadTo(std::vector<void*>& allocs, const std::vector<T>& v) {
   if (v.empty()) return nullptr;
   T* p = nullptr;
   GDTP_HIP(hipMalloc(&p, v.size() * sizeof(T)));
-  ds.allocations.push_back(p);
+  allocs.push_back(p);
+  g_uploadBytes += v.size() * sizeof(T);
   GDTP_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return p;
+}
+
+template <class T>
+static T* Upload(DeviceSystem& ds, const std::vector<T>& v) {
+  return UploadTo(ds.allocations, v);
 }
 
 static double* Alloc(DeviceSystem& ds, size_t n) {
   double* p = nullptr;
   GDTP_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+  g_uploadBytes += std::max<size_t>(n, 1) * sizeof(double);
   ds.allocations.push_back(p);
   GDTP_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(double)));
   return p;
@@ -372,15 +393,15 @@ static std::vector<double> SoA10(const std::vector<double>& aos) {
 
 // Partition the elimination tree into subtree teams (at most `smax` rows) plus the top part, and
 // group rows by dependency level inside each team and inside the top.
-static void UploadLdlt(DeviceSystem& ds, const Ldlt& F, DevLdlt& d) {
+void UploadLdlt(std::vector<void*>& ds, const Ldlt& F, DevLdlt& d) {
   const int n = F.n;
   d.n = n;
-  d.perm = Upload(ds, F.perm);
-  d.lrRowptr = Upload(ds, F.Lrows.rowptr), d.lrCol = Upload(ds, F.Lrows.col), d.lrVal = Upload(ds, F.Lrows.val);
-  d.lcRowptr = Upload(ds, F.Lcols.rowptr), d.lcCol = Upload(ds, F.Lcols.col), d.lcVal = Upload(ds, F.Lcols.val);
+  d.perm = UploadTo(ds, F.perm);
+  d.lrRowptr = UploadTo(ds, F.Lrows.rowptr), d.lrCol = UploadTo(ds, F.Lrows.col), d.lrVal = UploadTo(ds, F.Lrows.val);
+  d.lcRowptr = UploadTo(ds, F.Lcols.rowptr), d.lcCol = UploadTo(ds, F.Lcols.col), d.lcVal = UploadTo(ds, F.Lcols.val);
   std::vector<double> dinv(n);
   for (int i = 0; i < n; i++) dinv[i] = 1.0 / F.D[i];
-  d.dinv = Upload(ds, dinv);
+  d.dinv = UploadTo(ds, dinv);
   std::vector<int> parent(n, -1), size(n, 1), team(n, -1);
   for (int i = 0; i < n; i++)
     if (F.Lcols.rowptr[i + 1] > F.Lcols.rowptr[i]) parent[i] = F.Lcols.col[F.Lcols.rowptr[i]];
@@ -447,13 +468,14 @@ static void UploadLdlt(DeviceSystem& ds, const Ldlt& F, DevLdlt& d) {
   }
   build(bLevPtr, bRows, bTeamLev, d.topBwdLevBegin, d.topBwdLevEnd);
   d.nTeams = nTeams;
-  d.fwdLevPtr = Upload(ds, fLevPtr), d.fwdRows = Upload(ds, fRows), d.teamFwdLev = Upload(ds, fTeamLev);
-  d.bwdLevPtr = Upload(ds, bLevPtr), d.bwdRows = Upload(ds, bRows), d.teamBwdLev = Upload(ds, bTeamLev);
+  d.fwdLevPtr = UploadTo(ds, fLevPtr), d.fwdRows = UploadTo(ds, fRows), d.teamFwdLev = UploadTo(ds, fTeamLev);
+  d.bwdLevPtr = UploadTo(ds, bLevPtr), d.bwdRows = UploadTo(ds, bRows), d.teamBwdLev = UploadTo(ds, bTeamLev);
 }
 
 void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds) {
   const int nl = (int)h.levels.size();
   ds.levels = nl, ds.kcap = kcap;
+  g_uploadBytes = 0;
   ds.L.assign(nl, DevLevel());
   for (int l = 0; l < nl; l++) {
     const Level& H = h.levels[l];
@@ -488,8 +510,8 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     for (auto& x : H.prolongationLines) rl.insert(rl.end(), {x.start, x.length, x.center, x.prev, x.next, x.aligned});
     d.nProl = (int)H.prolongationLines.size();
     d.prol = Upload(ds, rl);
-    if (l + 1 < nl) UploadLdlt(ds, oh.boundaryFactor[l], d.factor);
-    else UploadLdlt(ds, oh.coarseFactor, d.factor);
+    if (l + 1 < nl) UploadLdlt(ds.allocations, oh.boundaryFactor[l], d.factor);
+    else UploadLdlt(ds.allocations, oh.coarseFactor, d.factor);
     d.x = Alloc(ds, (size_t)d.N * kcap);
     d.b = Alloc(ds, (size_t)d.N * kcap);
     d.r = Alloc(ds, (size_t)d.N * kcap);
@@ -504,9 +526,10 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     d.bd = UploadCsr(ds, which == 0 ? op.bdMass : op.bdStiffness);
     d.bb = UploadCsr(ds, which == 0 ? op.bbMass : op.bbStiffness);
   }
+  ds.bytes = g_uploadBytes;
 }
 
-static void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s) {
+void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s) {
   const int n = F.n;
   if (n == 0) return;
   const long nk = (long)n * K;
@@ -590,6 +613,11 @@ void ApplyOperator(DeviceSystem& ds, DevLevel& d, const double* x, double* y, in
     KApplyBoundary<<<Blocks((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, d.bb.rowptr, d.bb.col, d.bb.val, x,
                                                               nullptr, y, K, ld);
   if (d.nRaster > 0) KApplyDeep<<<d.nRaster, 128, 0, s>>>(d.nRaster, d.raster, d.coefRaw, d.D, x, nullptr, y, K, ld);
+}
+
+void SquaredNorms(const double* v, int N, int K, int ld, double* dNorms, hipStream_t s) {
+  KZero<<<1, 64, 0, s>>>(K, dNorms);
+  KSquaredNorm<<<dim3(Blocks(N, 256), K), 256, 0, s>>>(N, v, ld, dNorms);
 }
 
 void ResidualLevel(DeviceSystem& ds, int l, const double* x, const double* b, double* r, int K, int ld, hipStream_t s) {

@@ -1,0 +1,64 @@
+// Mixed-precision multi-RHS solver: FP64 residual and correction (iterative refinement) around
+// an FP32 V-cycle on dense per-chart colour planes.
+#pragma once
+#include "gdtp_device.h"
+
+namespace gdtp {
+
+struct MixChart {  // per chart geometry of one level
+  int w, h, w2, h2, base2, cofx, cofy;
+};
+
+struct DevMixLevel {
+  int N = 0, B = 0, nCharts = 0;
+  long plane = 0, Nd = 0;          // dense colour-plane size and total dense size (4 planes)
+  MixChart* charts = nullptr;      // device
+  std::vector<MixChart> hcharts;   // host copy
+  float* stencil = nullptr;        // [10][Nd]: c0..c8 reciprocal form, slot 9 = a4; zero off deep texels
+  int8_t* type = nullptr;          // [Nd]: -1 inactive, 0/1 boundary, 2 deep
+  int* g2d = nullptr;              // [N] compact global index -> dense index
+  int nTiles = 0;
+  int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
+  // boundary rows (Jacobi smoothing and residual): dense index, off-diagonal CSR, 1/diagonal
+  int* bdIdx = nullptr;
+  int *bRowptr = nullptr, *bCol = nullptr;
+  float *bVal = nullptr, *bDiag = nullptr;
+  // restriction into this level (coarse boundary rows: fine dense columns, weights)
+  int *rRowptr = nullptr, *rCol = nullptr;
+  float* rVal = nullptr;
+  // vectors [K][Nd] and boundary scratch [K][B]
+  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr;
+};
+
+struct MixedSystem {
+  int levels = 0, kcap = 0;
+  int nuPre = 2, nuPost = 2, boundarySweeps = 4;
+  float omega = 0.7f;
+  std::vector<DevMixLevel> L;
+  DevLdlt coarse;                  // exact FP64 solve of the coarsest level
+  double *coarseRhs = nullptr, *coarseX = nullptr, *coarseY = nullptr;
+  double* norms = nullptr;         // device [2*kcap]: squared residual norms, squared rhs norms
+  double* hostNorms = nullptr;     // pinned [2*kcap]
+  std::vector<void*> allocations;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  int graphK = 0;
+  const void *graphX = nullptr, *graphB = nullptr;
+  hipStream_t stream = nullptr;
+  ~MixedSystem();
+};
+
+void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& ds, int kcap, MixedSystem& ms);
+
+// Solve A x = b (level-0 compact FP64 vectors, K columns) until every column's relative residual
+// is <= tol or maxIter refinement steps; x is used as the initial guess.  Returns the iterations
+// run; relRes (host, K entries) receives the final relative residuals.
+int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
+               hipStream_t s, bool useGraph);
+
+// One FP32 V-cycle of the correction equation on the current level-0 residual (exposed for profiling).
+void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s);
+// One four-colour deep Gauss-Seidel sweep of `level` on the solver's internal vectors.
+void MixedDeepSweep(MixedSystem& ms, int level, int K, hipStream_t s);
+
+}  // namespace gdtp

@@ -1,0 +1,566 @@
+// Mixed-precision multi-RHS multigrid (the fast path).
+//
+// Outer loop in FP64 on the compact texel vectors (iterative refinement):
+//     r = b - A x  (FP64 stencils, written as FP32 into the level-0 dense planes, norms in FP64)
+//     e = V(r)     (one FP32 V-cycle of the correction equation A e = r, e = 0 initially)
+//     x += e
+// The FP32 V-cycle works on dense per-chart grids split into four colour planes by texel parity
+// (q = (i&1) | (j&1)<<1; texel (i,j) lives at plane q, sub-grid (i>>1, j>>1)).  A four-colour
+// Gauss-Seidel sweep then updates one plane per launch from fully coalesced neighbour planes.
+// Boundary / seam texels are relaxed by damped Jacobi over their sparse rows (A_BB and A_BD
+// merged, dense column indices); the coarsest level is solved exactly (FP64 LDL^T).
+// Transfers are the reference's: 3x3 full weighting / the boundary restriction matrix, and
+// bilinear prolongation (IterativeSolvers.inl:1009-1168), evaluated from chart geometry.
+#include <stdexcept>
+
+#include "gdtp_mixed.h"
+
+namespace gdtp {
+
+static constexpr int kTA = 64, kTB = 4;  // tile of the sub-grid per workgroup: 64 x 4 (256 threads)
+static inline unsigned NBlk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+__device__ __forceinline__ long DenseIdx(const MixChart& c, long plane, int i, int j) {
+  int q = (i & 1) | ((j & 1) << 1);
+  return (long)q * plane + c.base2 + (long)(j >> 1) * c.w2 + (i >> 1);
+}
+
+// ------------------------------------------------------------------ deep four-colour Gauss-Seidel
+__global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                  long Nd, const float* __restrict__ st, float* e, const float* __restrict__ rhs, int K) {
+  const int* T = tiles + 3 * blockIdx.x;
+  const MixChart c = charts[T[0]];
+  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
+  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  if (a >= c.w2 || b >= c.h2 || i >= c.w || j >= c.h) return;
+  const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
+  const float c4 = st[4 * Nd + idx];
+  if (c4 == 0.0f) return;
+  float cf[9];
+  long nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int di = k % 3 - 1, dj = k / 3 - 1;
+    cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+    nb[k] = k == 4 ? idx : DenseIdx(c, plane, i + di, j + dj);
+  }
+  for (int ch = 0; ch < K; ch++) {
+    const float* E = e + (long)ch * Nd;
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (k != 4) s += cf[k] * E[nb[k]];
+    e[(long)ch * Nd + idx] = c4 * rhs[(long)ch * Nd + idx] - s;
+  }
+}
+
+// res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4); blockIdx.y = plane
+__global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                        long Nd, const float* __restrict__ st, const float* __restrict__ e,
+                                                        const float* __restrict__ rhs, float* res, int K) {
+  const int q = blockIdx.y;
+  const int* T = tiles + 3 * blockIdx.x;
+  const MixChart c = charts[T[0]];
+  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
+  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  if (a >= c.w2 || b >= c.h2 || i >= c.w || j >= c.h) return;
+  const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
+  const float a4 = st[9 * Nd + idx];
+  if (a4 == 0.0f) return;
+  float cf[9];
+  long nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int di = k % 3 - 1, dj = k / 3 - 1;
+    cf[k] = k == 4 ? 1.0f : st[k * Nd + idx];
+    nb[k] = k == 4 ? idx : DenseIdx(c, plane, i + di, j + dj);
+  }
+  for (int ch = 0; ch < K; ch++) {
+    const float* E = e + (long)ch * Nd;
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s += cf[k] * E[nb[k]];
+    res[(long)ch * Nd + idx] = rhs[(long)ch * Nd + idx] - s * a4;
+  }
+}
+
+// ------------------------------------------------------------------ boundary rows
+__global__ void KMixBoundaryJacobi(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                   const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
+                                   const float* __restrict__ rhs, float* tmp, long Nd, int K, float omega) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * K) return;
+  const int r = t % B, ch = t / B;
+  const float* E = e + (long)ch * Nd;
+  float s = 0.0f;
+  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
+  const long d = bdIdx[r];
+  const float xo = E[d];
+  tmp[(long)ch * B + r] = xo + omega * ((rhs[(long)ch * Nd + d] - s) / dg[r] - xo);
+}
+
+__global__ void KMixScatterB(int B, const int* __restrict__ bdIdx, const float* __restrict__ tmp, float* e, long Nd, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * K) return;
+  const int r = t % B, ch = t / B;
+  e[(long)ch * Nd + bdIdx[r]] = tmp[(long)ch * B + r];
+}
+
+__global__ void KMixResidualBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                     const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
+                                     const float* __restrict__ rhs, float* res, long Nd, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * K) return;
+  const int r = t % B, ch = t / B;
+  const float* E = e + (long)ch * Nd;
+  const long d = bdIdx[r];
+  float s = dg[r] * E[d];
+  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
+  res[(long)ch * Nd + d] = rhs[(long)ch * Nd + d] - s;
+}
+
+// ------------------------------------------------------------------ transfers
+// Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c.
+__global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ tiles, const MixChart* __restrict__ cch,
+                                                        const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
+                                                        const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, int K) {
+  const int q = blockIdx.y;
+  const int* T = tiles + 3 * blockIdx.x;
+  const MixChart c = cch[T[0]], f = fch[T[0]];
+  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
+  const int ci = 2 * a + (q & 1), cj = 2 * b + (q >> 1);
+  if (a >= c.w2 || b >= c.h2 || ci >= c.w || cj >= c.h) return;
+  const long idx = (long)q * cplane + c.base2 + (long)b * c.w2 + a;
+  if (ctype[idx] != 2) return;
+  const int fi = f.cofx + 2 * (ci - c.cofx), fj = f.cofy + 2 * (cj - c.cofy);
+  long n[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) n[k] = DenseIdx(f, fplane, fi + k % 3 - 1, fj + k / 3 - 1);
+  for (int ch = 0; ch < K; ch++) {
+    const float* R = fres + (long)ch * fNd;
+    crhs[(long)ch * cNd + idx] = R[n[4]] + (R[n[1]] + R[n[3]] + R[n[5]] + R[n[7]]) * 0.5f + (R[n[0]] + R[n[2]] + R[n[6]] + R[n[8]]) * 0.25f;
+  }
+}
+
+// Coarse boundary rhs: the boundary restriction matrix (fine dense columns).
+__global__ void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                     const float* __restrict__ vl, const float* __restrict__ fres, long fNd, float* crhs, long cNd, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * K) return;
+  const int r = t % B, ch = t / B;
+  const float* R = fres + (long)ch * fNd;
+  float s = 0.0f;
+  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * R[cl[k]];
+  crhs[(long)ch * cNd + bdIdx[r]] = s;
+}
+
+__device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+
+// e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid).
+__global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ tiles, const MixChart* __restrict__ fch,
+                                                      const MixChart* __restrict__ cch, long fplane, long fNd, long cplane, long cNd,
+                                                      const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K) {
+  const int q = blockIdx.y;
+  const int* T = tiles + 3 * blockIdx.x;
+  const MixChart f = fch[T[0]], c = cch[T[0]];
+  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
+  const int fi = 2 * a + (q & 1), fj = 2 * b + (q >> 1);
+  if (a >= f.w2 || b >= f.h2 || fi >= f.w || fj >= f.h) return;
+  const long idx = (long)q * fplane + f.base2 + (long)b * f.w2 + a;
+  if (ftype[idx] < 0) return;
+  const int ui = fi - f.cofx, uj = fj - f.cofy;
+  const int ci = FloorHalf(ui) + c.cofx, cj = FloorHalf(uj) + c.cofy;
+  const bool oi = (ui & 1) != 0, oj = (uj & 1) != 0;
+  const long n00 = DenseIdx(c, cplane, ci, cj);
+  const long n10 = oi ? DenseIdx(c, cplane, ci + 1, cj) : n00;
+  const long n01 = oj ? DenseIdx(c, cplane, ci, cj + 1) : n00;
+  const long n11 = (oi && oj) ? DenseIdx(c, cplane, ci + 1, cj + 1) : (oi ? n10 : n01);
+  for (int ch = 0; ch < K; ch++) {
+    const float* E = ce + (long)ch * cNd;
+    float v = (E[n00] + E[n10] + E[n01] + E[n11]) * 0.25f;
+    fe[(long)ch * fNd + idx] += v;
+  }
+}
+
+__global__ void KMixZero(long n, float* p) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0f;
+}
+
+__global__ void KMixZeroD(int n, double* p) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+// coarsest level: dense FP32 <-> compact FP64
+__global__ void KMixGather(int N, const int* __restrict__ g2d, const float* __restrict__ src, long Nd, double* dst, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N * K) return;
+  const int g = t % N, ch = t / N;
+  dst[(long)ch * N + g] = (double)src[(long)ch * Nd + g2d[g]];
+}
+
+__global__ void KMixScatter(int N, const int* __restrict__ g2d, const double* __restrict__ src, float* dst, long Nd, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N * K) return;
+  const int g = t % N, ch = t / N;
+  dst[(long)ch * Nd + g2d[g]] = (float)src[(long)ch * N + g];
+}
+
+// ------------------------------------------------------------------ FP64 outer loop
+__device__ __forceinline__ void BlockAddSquares(double v, double* target) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __shared__ double part[16];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int k = 0; k < (int)(blockDim.x / 64); k++) s += part[k];
+    atomicAdd(target, s);
+  }
+}
+
+// deep rows: r = b - A x with raw FP64 stencils along raster lines; r -> FP32 dense, |r|^2 -> norms
+__global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __restrict__ lines, const double* __restrict__ coef, int D,
+                                                       const double* __restrict__ x, const double* __restrict__ b, int N,
+                                                       const int* __restrict__ g2d, float* rhs0, long Nd, double* norms, int ch) {
+  const int l = blockIdx.x;
+  const int* L = lines + 5 * l;
+  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+  const double* X = x + (long)ch * N;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int base = k < 3 ? prev : (k < 6 ? start : next);
+      s += coef[(long)k * D + d0 + i] * X[base + i + k % 3 - 1];
+    }
+    const double r = b[(long)ch * N + start + i] - s;
+    rhs0[(long)ch * Nd + g2d[start + i]] = (float)r;
+    acc += r * r;
+  }
+  BlockAddSquares(acc, norms + ch);
+}
+
+__global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
+                                                           const int* __restrict__ cold, const double* __restrict__ vald,
+                                                           const int* __restrict__ rpb, const int* __restrict__ colb,
+                                                           const double* __restrict__ valb, const double* __restrict__ x,
+                                                           const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                           long Nd, double* norms, int ch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < B) {
+    const double* X = x + (long)ch * N;
+    double s = 0.0;
+    for (int k = rpb[i]; k < rpb[i + 1]; k++) s += X[bgi[colb[k]]] * valb[k];
+    for (int k = rpd[i]; k < rpd[i + 1]; k++) s += X[cold[k]] * vald[k];
+    const int g = bgi[i];
+    const double r = b[(long)ch * N + g] - s;
+    rhs0[(long)ch * Nd + g2d[g]] = (float)r;
+    acc = r * r;
+  }
+  BlockAddSquares(acc, norms + ch);
+}
+
+__global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out, int ch) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = i < N ? v[(long)ch * N + i] : 0.0;
+  BlockAddSquares(a * a, out + ch);
+}
+
+__global__ void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N * K) return;
+  const int g = t % N, ch = t / N;
+  x[(long)ch * N + g] += (double)e0[(long)ch * Nd + g2d[g]];
+}
+
+// ------------------------------------------------------------------ host
+MixedSystem::~MixedSystem() {
+  if (graphExec) (void)hipGraphExecDestroy(graphExec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (stream) (void)hipStreamDestroy(stream);
+  if (hostNorms) (void)hipHostFree(hostNorms);
+  for (void* p : allocations) (void)hipFree(p);
+}
+
+template <class T>
+static T* Up(MixedSystem& ms, const std::vector<T>& v) {
+  if (v.empty()) return nullptr;
+  T* p = nullptr;
+  GDTP_HIP(hipMalloc(&p, v.size() * sizeof(T)));
+  ms.allocations.push_back(p);
+  GDTP_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <class T>
+static T* Zeros(MixedSystem& ms, size_t n) {
+  T* p = nullptr;
+  GDTP_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  ms.allocations.push_back(p);
+  GDTP_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  return p;
+}
+
+void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& ds, int kcap, MixedSystem& ms) {
+  const int nl = (int)h.levels.size();
+  if (nl < 2) throw std::runtime_error("the mixed-precision solver needs at least two levels");
+  ms.levels = nl, ms.kcap = kcap;
+  ms.L.assign(nl, DevMixLevel());
+  std::vector<std::vector<long>> g2dAll(nl);
+  for (int l = 0; l < nl; l++) {
+    const Level& H = h.levels[l];
+    DevMixLevel& d = ms.L[l];
+    d.N = H.numTexels, d.B = H.numBoundary, d.nCharts = (int)H.charts.size();
+    d.hcharts.resize(d.nCharts);
+    long base = 0;
+    for (int c = 0; c < d.nCharts; c++) {
+      const GridChart& g = H.charts[c];
+      MixChart m{g.width, g.height, (g.width + 1) / 2, (g.height + 1) / 2, (int)base, g.centerOffset[0], g.centerOffset[1]};
+      d.hcharts[c] = m;
+      base += (long)m.w2 * m.h2;
+    }
+    if (base > (1L << 29)) throw std::runtime_error("dense colour plane too large");
+    d.plane = base, d.Nd = 4 * base;
+    auto dense = [&](int c, int i, int j) {
+      const MixChart& m = d.hcharts[c];
+      int q = (i & 1) | ((j & 1) << 1);
+      return (long)q * d.plane + m.base2 + (long)(j >> 1) * m.w2 + (i >> 1);
+    };
+    std::vector<int8_t> type(d.Nd, -1);
+    std::vector<int> g2d(d.N);
+    std::vector<long>& g2dl = g2dAll[l];
+    g2dl.resize(d.N);
+    for (int g = 0; g < d.N; g++) {
+      const NodeInfo& ni = H.nodeInfo[g];
+      long di = dense(ni.chart, ni.ci, ni.cj);
+      g2d[g] = (int)di, g2dl[g] = di;
+      type[di] = (int8_t)ni.type;
+    }
+    d.charts = Up(ms, d.hcharts);
+    d.g2d = Up(ms, g2d);
+    d.type = Up(ms, type);
+    // tiles of the sub-grid that contain at least one active texel (any plane)
+    std::vector<int> tiles;
+    for (int c = 0; c < d.nCharts; c++) {
+      const MixChart& m = d.hcharts[c];
+      for (int b0 = 0; b0 < m.h2; b0 += kTB)
+        for (int a0 = 0; a0 < m.w2; a0 += kTA) {
+          bool any = false;
+          for (int q = 0; q < 4 && !any; q++)
+            for (int bb = b0; bb < std::min(b0 + kTB, m.h2) && !any; bb++)
+              for (int aa = a0; aa < std::min(a0 + kTA, m.w2) && !any; aa++) {
+                int i = 2 * aa + (q & 1), j = 2 * bb + (q >> 1);
+                if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)bb * m.w2 + aa] >= 0) any = true;
+              }
+          if (any) tiles.insert(tiles.end(), {c, a0, b0});
+        }
+    }
+    d.nTiles = (int)tiles.size() / 3;
+    d.tiles = Up(ms, tiles);
+    // deep stencils (reciprocal form) on smoothed levels
+    if (l + 1 < nl) {
+      std::vector<float> st(10 * (size_t)d.Nd, 0.0f);
+      const std::vector<double>& rc = oh.levels[l].deepRecip;
+      for (int dd = 0; dd < H.numDeep; dd++) {
+        long di = g2dl[H.deepGlobalIndex[dd]];
+        for (int k = 0; k < 10; k++) st[(size_t)k * d.Nd + di] = (float)rc[10 * (size_t)dd + k];
+      }
+      d.stencil = Up(ms, st);
+      // boundary rows: merged A_BB (off-diagonal) and A_BD with dense columns
+      const Csr &bb = oh.levels[l].bb, &bd = oh.levels[l].bd;
+      std::vector<int> rp(1, 0), cl, bidx(d.B);
+      std::vector<float> vl, dg(d.B, 0.0f);
+      for (int r = 0; r < d.B; r++) {
+        bidx[r] = g2d[H.boundaryGlobalIndex[r]];
+        for (int k = bb.rowptr[r]; k < bb.rowptr[r + 1]; k++) {
+          if (bb.col[k] == r) { dg[r] = (float)bb.val[k]; continue; }
+          cl.push_back(g2d[H.boundaryGlobalIndex[bb.col[k]]]);
+          vl.push_back((float)bb.val[k]);
+        }
+        for (int k = bd.rowptr[r]; k < bd.rowptr[r + 1]; k++) cl.push_back(g2d[bd.col[k]]), vl.push_back((float)bd.val[k]);
+        rp.push_back((int)cl.size());
+        if (!(dg[r] > 0.0f)) throw std::runtime_error("boundary row without positive diagonal");
+      }
+      d.bdIdx = Up(ms, bidx), d.bRowptr = Up(ms, rp), d.bCol = Up(ms, cl), d.bVal = Up(ms, vl), d.bDiag = Up(ms, dg);
+    } else {
+      std::vector<int> bidx(d.B);
+      for (int r = 0; r < d.B; r++) bidx[r] = g2d[H.boundaryGlobalIndex[r]];
+      d.bdIdx = Up(ms, bidx);
+    }
+    // restriction into this level (rows: this level's boundary texels; columns: fine dense)
+    if (l > 0) {
+      const Csr& R = h.levels[l - 1].boundaryRestriction;
+      std::vector<int> cl(R.col.size());
+      std::vector<float> vl(R.val.size());
+      for (size_t k = 0; k < R.col.size(); k++) cl[k] = (int)g2dAll[l - 1][R.col[k]], vl[k] = (float)R.val[k];
+      d.rRowptr = Up(ms, R.rowptr), d.rCol = Up(ms, cl), d.rVal = Up(ms, vl);
+    }
+    // every active texel's bilinear parents must be active texels of the next level
+    if (l + 1 < nl) {
+      const Level& C = h.levels[l + 1];
+      for (int g = 0; g < d.N; g++) {
+        const NodeInfo& ni = H.nodeInfo[g];
+        const GridChart &fc = H.charts[ni.chart], &cc = C.charts[ni.chart];
+        int ui = ni.ci - fc.centerOffset[0], uj = ni.cj - fc.centerOffset[1];
+        int ci = (ui >= 0 ? ui / 2 : -((1 - ui) / 2)) + cc.centerOffset[0], cj = (uj >= 0 ? uj / 2 : -((1 - uj) / 2)) + cc.centerOffset[1];
+        for (int dj = 0; dj <= (uj & 1); dj++)
+          for (int di = 0; di <= (ui & 1); di++) {
+            int a = ci + di, bq = cj + dj;
+            if (a < 0 || bq < 0 || a >= cc.width || bq >= cc.height || cc.node(a, bq) < 0)
+              throw std::runtime_error("prolongation parent outside the coarse chart");
+          }
+      }
+    }
+    d.e = Zeros<float>(ms, (size_t)d.Nd * kcap);
+    d.rhs = Zeros<float>(ms, (size_t)d.Nd * kcap);
+    d.res = Zeros<float>(ms, (size_t)d.Nd * kcap);
+    d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
+  }
+  UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse);
+  const int Nc = ms.L[nl - 1].N;
+  ms.coarseRhs = Zeros<double>(ms, (size_t)Nc * kcap);
+  ms.coarseX = Zeros<double>(ms, (size_t)Nc * kcap);
+  ms.coarseY = Zeros<double>(ms, (size_t)Nc * kcap);
+  ms.norms = Zeros<double>(ms, 2 * (size_t)kcap);
+  GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * kcap * sizeof(double), hipHostMallocDefault));
+  GDTP_HIP(hipStreamCreateWithFlags(&ms.stream, hipStreamNonBlocking));
+  (void)ds;
+}
+
+static void Smooth(MixedSystem& ms, int l, int K, bool reverse, hipStream_t s) {
+  DevMixLevel& d = ms.L[l];
+  const int nu = reverse ? ms.nuPost : ms.nuPre;
+  for (int it = 0; it < nu; it++) {
+    if (d.B > 0)
+      for (int bs = 0; bs < ms.boundarySweeps; bs++) {
+        const long bk = (long)d.B * K;
+        KMixBoundaryJacobi<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB, d.Nd, K,
+                                                         ms.omega);
+        KMixScatterB<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.tmpB, d.e, d.Nd, K);
+      }
+    if (d.nTiles > 0)
+      for (int qq = 0; qq < 4; qq++) {
+        int q = reverse ? 3 - qq : qq;
+        KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+      }
+  }
+}
+
+void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
+  if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
+  DevMixLevel& d = ms.L[l];
+  for (int q = 0; q < 4; q++) KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+}
+
+void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
+  const int nl = ms.levels;
+  for (int l = 0; l + 1 < nl; l++) {
+    DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
+    KMixZero<<<NBlk(d.Nd * K, 256), 256, 0, s>>>(d.Nd * K, d.e);
+    Smooth(ms, l, K, false, s);
+    if (d.nTiles > 0)
+      KMixResidualDeep<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
+    if (d.B > 0)
+      KMixResidualBoundary<<<NBlk((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
+                                                                     d.Nd, K);
+    if (c.nTiles > 0)
+      KMixRestrictDeep<<<dim3(c.nTiles, 4), 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, K);
+    if (c.B > 0)
+      KMixRestrictBoundary<<<NBlk((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs, c.Nd, K);
+  }
+  {
+    DevMixLevel& c = ms.L[nl - 1];
+    const long nk = (long)c.N * K;
+    KMixGather<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, c.rhs, c.Nd, ms.coarseRhs, K);
+    LdltSolve(ms.coarse, ms.coarseRhs, c.N, ms.coarseX, c.N, nullptr, ms.coarseY, K, s);
+    KMixZero<<<NBlk(c.Nd * K, 256), 256, 0, s>>>(c.Nd * K, c.e);
+    KMixScatter<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, ms.coarseX, c.e, c.Nd, K);
+  }
+  for (int l = nl - 2; l >= 0; l--) {
+    DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
+    if (d.nTiles > 0)
+      KMixProlongAdd<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K);
+    Smooth(ms, l, K, true, s);
+  }
+  (void)ds;
+}
+
+// residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
+static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, hipStream_t s) {
+  DevLevel& L0 = ds.L[0];
+  DevMixLevel& d = ms.L[0];
+  KMixZeroD<<<1, 64, 0, s>>>(K, ms.norms);
+  for (int ch = 0; ch < K; ch++) {
+    if (L0.nRaster > 0)
+      K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.Nd, ms.norms, ch);
+    if (L0.B > 0)
+      K64ResidualBoundary<<<NBlk(L0.B, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr, L0.bb.col,
+                                                          L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.Nd, ms.norms, ch);
+  }
+}
+
+static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
+  DevMixLevel& d = ms.L[0];
+  const long nk = (long)d.N * K;
+  K64Update<<<NBlk(nk, 256), 256, 0, s>>>(d.N, d.g2d, d.e, d.Nd, x, K);
+  (void)ds;
+}
+
+int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
+               hipStream_t s, bool useGraph) {
+  if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
+  const int N = ms.L[0].N;
+  KMixZeroD<<<1, 64, 0, s>>>(2 * K, ms.norms);
+  for (int ch = 0; ch < K; ch++) K64SquaredNorm<<<NBlk(N, 256), 256, 0, s>>>(N, b, ms.norms + K, ch);
+  OuterResidual(ds, ms, x, b, K, s);
+  GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * K * sizeof(double), hipMemcpyDeviceToHost, s));
+  GDTP_HIP(hipStreamSynchronize(s));
+  std::vector<double> bn(K);
+  for (int k = 0; k < K; k++) bn[k] = std::sqrt(std::max(ms.hostNorms[K + k], 1e-300));
+  auto converged = [&]() {
+    bool ok = true;
+    for (int k = 0; k < K; k++) {
+      relRes[k] = std::sqrt(ms.hostNorms[k]) / bn[k];
+      if (!(relRes[k] <= tol)) ok = false;
+    }
+    return ok;
+  };
+  if (converged()) return 0;
+  // one refinement step = V-cycle on the current residual, update, new residual
+  auto step = [&](hipStream_t st) {
+    MixedVCycle(ds, ms, K, st);
+    OuterUpdate(ds, ms, x, K, st);
+    OuterResidual(ds, ms, x, b, K, st);
+    GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, K * sizeof(double), hipMemcpyDeviceToHost, st));
+  };
+  if (useGraph && !(ms.graphExec && ms.graphK == K && ms.graphX == x && ms.graphB == b)) {
+    // capture one refinement step on the solver's own stream (shape-static per K, x, b)
+    if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec), ms.graphExec = nullptr;
+    if (ms.graph) (void)hipGraphDestroy(ms.graph), ms.graph = nullptr;
+    GDTP_HIP(hipStreamBeginCapture(ms.stream, hipStreamCaptureModeThreadLocal));
+    step(ms.stream);
+    GDTP_HIP(hipStreamEndCapture(ms.stream, &ms.graph));
+    GDTP_HIP(hipGraphInstantiate(&ms.graphExec, ms.graph, nullptr, nullptr, 0));
+    ms.graphK = K, ms.graphX = x, ms.graphB = b;
+  }
+  int it = 0;
+  while (it < maxIter) {
+    if (useGraph) {
+      GDTP_HIP(hipGraphLaunch(ms.graphExec, ms.stream));
+      GDTP_HIP(hipStreamSynchronize(ms.stream));
+    } else {
+      step(s);
+      GDTP_HIP(hipStreamSynchronize(s));
+    }
+    it++;
+    if (converged()) break;
+  }
+  return it;
+}
+
+}  // namespace gdtp

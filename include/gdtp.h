@@ -94,6 +94,24 @@ int gdtp_apply(gdtp_system* s, int which, const double* x, double* y, int ncols,
 /* r = b - A x on level 0 (ComputeSystemResdiual, :953-1007). */
 int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, int ncols, void* stream);
 
+/* ---- solves to a relative residual (every column: ||b - A x|| <= tol ||b||)
+ * mode GDTP_SOLVE_FP64_REFERENCE: reference V-cycles with an FP64 residual check per cycle.
+ * mode GDTP_SOLVE_MIXED: FP64 iterative refinement around FP32 V-cycles (four-colour deep
+ *   Gauss-Seidel on dense chart planes, damped-Jacobi boundary rows, exact FP64 coarsest solve),
+ *   on the system's own hierarchy; no counterpart in the reference.
+ * x holds the initial guess and receives the solution; rel_res (host, ncols) and iterations
+ * (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached first. */
+int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int max_iters, int mode, double* rel_res,
+               int* iterations, void* stream);
+/* Mixed-solver smoothing: pre / post smoothing steps, boundary Jacobi sweeps per step and
+ * their damping, hipGraph capture of the refinement step (1) or plain launches (0). */
+int gdtp_mixed_configure(gdtp_system* s, int nu_pre, int nu_post, int boundary_sweeps, double omega, int use_graph);
+/* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
+ * vectors); exposed to time the dominant kernel. */
+int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream);
+/* Device bytes held by the system (reference path, mixed path). */
+int gdtp_system_memory(const gdtp_system* s, size_t* reference_bytes, size_t* mixed_bytes);
+
 #ifdef __cplusplus
 }
 #endif
