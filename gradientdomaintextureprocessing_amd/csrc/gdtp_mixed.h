@@ -21,13 +21,17 @@ struct DevMixLevel {
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
   // boundary rows (Jacobi smoothing and residual): dense index, off-diagonal CSR, 1/diagonal
   int* bdIdx = nullptr;
-  int *bRowptr = nullptr, *bCol = nullptr;
+  int *bRowptr = nullptr, *bCol = nullptr;       // merged off-diagonal row, dense columns (residual)
   float *bVal = nullptr, *bDiag = nullptr;
+  int *bbRowptr = nullptr, *bbCol = nullptr;     // off-diagonal A_BB, boundary-index columns
+  float* bbVal = nullptr;
+  int *bdRowptr = nullptr, *bdCol = nullptr;     // A_BD, dense columns
+  float* bdVal = nullptr;
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
   int *rRowptr = nullptr, *rCol = nullptr;
   float* rVal = nullptr;
   // vectors [K][Nd] and boundary scratch [K][B]
-  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr;
+  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr, *tmpB2 = nullptr;
 };
 
 struct MixedSystem {
@@ -37,8 +41,9 @@ struct MixedSystem {
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level
   double *coarseRhs = nullptr, *coarseX = nullptr, *coarseY = nullptr;
-  double* norms = nullptr;         // device [2*kcap]: squared residual norms, squared rhs norms
-  double* hostNorms = nullptr;     // pinned [2*kcap]
+  double* coarseInv = nullptr;     // dense inverse of a small coarsest level (row-major), else null
+  double* norms = nullptr;         // device [2][kcap][64]: squared residual / rhs norm partial sums
+  double* hostNorms = nullptr;     // pinned copy
   std::vector<void*> allocations;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;

@@ -18,6 +18,7 @@
 namespace gdtp {
 
 static constexpr int kTA = 64, kTB = 4;  // tile of the sub-grid per workgroup: 64 x 4 (256 threads)
+static constexpr int kDenseCoarseMax = 6000;  // coarsest levels up to this size use a dense inverse
 static inline unsigned NBlk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 __device__ __forceinline__ long DenseIdx(const MixChart& c, long plane, int i, int j) {
@@ -85,18 +86,29 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
 }
 
 // ------------------------------------------------------------------ boundary rows
-__global__ void KMixBoundaryJacobi(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                   const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
-                                   const float* __restrict__ rhs, float* tmp, long Nd, int K, float omega) {
+// One damped-Jacobi sweep over the boundary rows.  Boundary neighbours are read from `srcB`
+// (compact [K][B]) or, when srcB is null, from the dense vector; deep neighbours always from the
+// dense vector.  The new values go to `dstB` (compact) or, when null, into the dense vector.
+__global__ void KMixBoundaryJacobi(int B, const int* __restrict__ bdIdx, const int* __restrict__ rpb, const int* __restrict__ clb,
+                                   const float* __restrict__ vlb, const int* __restrict__ rpd, const int* __restrict__ cld,
+                                   const float* __restrict__ vld, const float* __restrict__ dg, const float* srcB, float* e,
+                                   const float* __restrict__ rhs, float* dstB, long Nd, int K, float omega) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)B * K) return;
   const int r = t % B, ch = t / B;
   const float* E = e + (long)ch * Nd;
+  const float* S = srcB ? srcB + (long)ch * B : nullptr;
   float s = 0.0f;
-  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
+  for (int k = rpd[r]; k < rpd[r + 1]; k++) s += vld[k] * E[cld[k]];
+  if (S)
+    for (int k = rpb[r]; k < rpb[r + 1]; k++) s += vlb[k] * S[clb[k]];
+  else
+    for (int k = rpb[r]; k < rpb[r + 1]; k++) s += vlb[k] * E[bdIdx[clb[k]]];
   const long d = bdIdx[r];
-  const float xo = E[d];
-  tmp[(long)ch * B + r] = xo + omega * ((rhs[(long)ch * Nd + d] - s) / dg[r] - xo);
+  const float xo = S ? S[r] : E[d];
+  const float xn = xo + omega * ((rhs[(long)ch * Nd + d] - s) / dg[r] - xo);
+  if (dstB) dstB[(long)ch * B + r] = xn;
+  else e[(long)ch * Nd + d] = xn;
 }
 
 __global__ void KMixScatterB(int B, const int* __restrict__ bdIdx, const float* __restrict__ tmp, float* e, long Nd, int K) {
@@ -123,7 +135,8 @@ __global__ void KMixResidualBoundary(int B, const int* __restrict__ bdIdx, const
 // Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c.
 __global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ tiles, const MixChart* __restrict__ cch,
                                                         const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
-                                                        const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, int K) {
+                                                        const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
+                                                        int K) {
   const int q = blockIdx.y;
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = cch[T[0]], f = fch[T[0]];
@@ -139,12 +152,14 @@ __global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ 
   for (int ch = 0; ch < K; ch++) {
     const float* R = fres + (long)ch * fNd;
     crhs[(long)ch * cNd + idx] = R[n[4]] + (R[n[1]] + R[n[3]] + R[n[5]] + R[n[7]]) * 0.5f + (R[n[0]] + R[n[2]] + R[n[6]] + R[n[8]]) * 0.25f;
+    ce[(long)ch * cNd + idx] = 0.0f;
   }
 }
 
 // Coarse boundary rhs: the boundary restriction matrix (fine dense columns).
 __global__ void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                     const float* __restrict__ vl, const float* __restrict__ fres, long fNd, float* crhs, long cNd, int K) {
+                                     const float* __restrict__ vl, const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
+                                     int K) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)B * K) return;
   const int r = t % B, ch = t / B;
@@ -152,6 +167,7 @@ __global__ void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const
   float s = 0.0f;
   for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * R[cl[k]];
   crhs[(long)ch * cNd + bdIdx[r]] = s;
+  ce[(long)ch * cNd + bdIdx[r]] = 0.0f;
 }
 
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
@@ -182,6 +198,25 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
   }
 }
 
+// Coarsest level: e = A^{-1} rhs with the precomputed dense inverse (FP64), rhs / e in the dense
+// FP32 planes; one workgroup per row.
+__global__ __launch_bounds__(256) void KMixDenseCoarse(int N, const double* __restrict__ Ainv, const int* __restrict__ g2d,
+                                                       const float* __restrict__ rhs, float* e, long Nd, int K) {
+  const int i = blockIdx.x;
+  const double* row = Ainv + (long)i * N;
+  for (int ch = 0; ch < K; ch++) {
+    const float* R = rhs + (long)ch * Nd;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) s += row[j] * (double)R[g2d[j]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ double part[4];
+    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) e[(long)ch * Nd + g2d[i]] = (float)(part[0] + part[1] + part[2] + part[3]);
+    __syncthreads();
+  }
+}
+
 __global__ void KMixZero(long n, float* p) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0.0f;
@@ -208,6 +243,9 @@ __global__ void KMixScatter(int N, const int* __restrict__ g2d, const double* __
 }
 
 // ------------------------------------------------------------------ FP64 outer loop
+static constexpr int kSlots = 64;  // norm accumulators per column (spreads the atomics)
+
+// Block-wide sum of v added to target[blockIdx.x % kSlots]; every thread of the block must call it.
 __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   __shared__ double part[16];
@@ -217,39 +255,56 @@ __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
   if (threadIdx.x == 0) {
     double s = 0;
     for (int k = 0; k < (int)(blockDim.x / 64); k++) s += part[k];
-    atomicAdd(target, s);
+    atomicAdd(target + (blockIdx.x % kSlots), s);
   }
+  __syncthreads();
 }
 
-// deep rows: r = b - A x with raw FP64 stencils along raster lines; r -> FP32 dense, |r|^2 -> norms
+// deep rows: r = b - A x with raw FP64 stencils along raster lines, up to four columns per pass
+// over the stencil; r -> FP32 dense rhs of the correction, e -> 0, |r|^2 -> norms[ch][slot]
 __global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __restrict__ lines, const double* __restrict__ coef, int D,
                                                        const double* __restrict__ x, const double* __restrict__ b, int N,
-                                                       const int* __restrict__ g2d, float* rhs0, long Nd, double* norms, int ch) {
+                                                       const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K) {
   const int l = blockIdx.x;
   const int* L = lines + 5 * l;
   const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
-  const double* X = x + (long)ch * N;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    double s = 0.0;
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      double cf[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int base = k < 3 ? prev : (k < 6 ? start : next);
-      s += coef[(long)k * D + d0 + i] * X[base + i + k % 3 - 1];
+      for (int k = 0; k < 9; k++) cf[k] = coef[(long)k * D + d0 + i];
+      const long di = g2d[start + i];
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) {
+        if (cc >= kc) break;
+        const int ch = c0 + cc;
+        const double* X = x + (long)ch * N;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int base = k < 3 ? prev : (k < 6 ? start : next);
+          s += cf[k] * X[base + i + k % 3 - 1];
+        }
+        const double r = b[(long)ch * N + start + i] - s;
+        rhs0[(long)ch * Nd + di] = (float)r;
+        e0[(long)ch * Nd + di] = 0.0f;
+        acc[cc] += r * r;
+      }
     }
-    const double r = b[(long)ch * N + start + i] - s;
-    rhs0[(long)ch * Nd + g2d[start + i]] = (float)r;
-    acc += r * r;
+    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);
   }
-  BlockAddSquares(acc, norms + ch);
 }
 
+// boundary rows (blockIdx.y = column)
 __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
                                                            const int* __restrict__ cold, const double* __restrict__ vald,
                                                            const int* __restrict__ rpb, const int* __restrict__ colb,
                                                            const double* __restrict__ valb, const double* __restrict__ x,
                                                            const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                           long Nd, double* norms, int ch) {
+                                                           float* e0, long Nd, double* norms) {
+  const int ch = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
   if (i < B) {
@@ -260,15 +315,17 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
     const int g = bgi[i];
     const double r = b[(long)ch * N + g] - s;
     rhs0[(long)ch * Nd + g2d[g]] = (float)r;
+    e0[(long)ch * Nd + g2d[g]] = 0.0f;
     acc = r * r;
   }
-  BlockAddSquares(acc, norms + ch);
+  BlockAddSquares(acc, norms + (long)ch * kSlots);
 }
 
-__global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out, int ch) {
+__global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out) {
+  const int ch = blockIdx.y;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double a = i < N ? v[(long)ch * N + i] : 0.0;
-  BlockAddSquares(a * a, out + ch);
+  BlockAddSquares(a * a, out + (long)ch * kSlots);
 }
 
 __global__ void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
@@ -387,6 +444,16 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         if (!(dg[r] > 0.0f)) throw std::runtime_error("boundary row without positive diagonal");
       }
       d.bdIdx = Up(ms, bidx), d.bRowptr = Up(ms, rp), d.bCol = Up(ms, cl), d.bVal = Up(ms, vl), d.bDiag = Up(ms, dg);
+      std::vector<int> rpb(1, 0), clb, rpd(1, 0), cld;
+      std::vector<float> vlb, vld;
+      for (int r = 0; r < d.B; r++) {
+        for (int k = bb.rowptr[r]; k < bb.rowptr[r + 1]; k++)
+          if (bb.col[k] != r) clb.push_back(bb.col[k]), vlb.push_back((float)bb.val[k]);
+        for (int k = bd.rowptr[r]; k < bd.rowptr[r + 1]; k++) cld.push_back(g2d[bd.col[k]]), vld.push_back((float)bd.val[k]);
+        rpb.push_back((int)clb.size()), rpd.push_back((int)cld.size());
+      }
+      d.bbRowptr = Up(ms, rpb), d.bbCol = Up(ms, clb), d.bbVal = Up(ms, vlb);
+      d.bdRowptr = Up(ms, rpd), d.bdCol = Up(ms, cld), d.bdVal = Up(ms, vld);
     } else {
       std::vector<int> bidx(d.B);
       for (int r = 0; r < d.B; r++) bidx[r] = g2d[H.boundaryGlobalIndex[r]];
@@ -420,29 +487,48 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     d.rhs = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.res = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
+    d.tmpB2 = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
   }
   UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse);
   const int Nc = ms.L[nl - 1].N;
+  if (Nc <= kDenseCoarseMax) {  // dense inverse of the coarsest operator, by columns of the LDL^T solve
+    std::vector<double> inv((size_t)Nc * Nc), unit((size_t)Nc, 0.0), col((size_t)Nc);
+    for (int j = 0; j < Nc; j++) {
+      unit[j] = 1.0;
+      SolveLdlt(oh.coarseFactor, unit.data(), col.data(), 1, Nc);
+      unit[j] = 0.0;
+      for (int i = 0; i < Nc; i++) inv[(size_t)i * Nc + j] = col[i];
+    }
+    ms.coarseInv = Up(ms, inv);
+  }
   ms.coarseRhs = Zeros<double>(ms, (size_t)Nc * kcap);
   ms.coarseX = Zeros<double>(ms, (size_t)Nc * kcap);
   ms.coarseY = Zeros<double>(ms, (size_t)Nc * kcap);
-  ms.norms = Zeros<double>(ms, 2 * (size_t)kcap);
-  GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * kcap * sizeof(double), hipHostMallocDefault));
+  ms.norms = Zeros<double>(ms, 2 * (size_t)kcap * kSlots);
+  GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * (size_t)kcap * kSlots * sizeof(double), hipHostMallocDefault));
   GDTP_HIP(hipStreamCreateWithFlags(&ms.stream, hipStreamNonBlocking));
   (void)ds;
+}
+
+static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s) {
+  const int nb = ms.boundarySweeps;
+  if (d.B == 0 || nb == 0) return;
+  const long bk = (long)d.B * K;
+  float* buf[2] = {d.tmpB, d.tmpB2};
+  for (int sw = 0; sw < nb; sw++) {
+    const float* src = sw == 0 ? nullptr : buf[(sw - 1) % 2];
+    float* dst = (sw == nb - 1 && nb > 1) ? nullptr : buf[sw % 2];
+    KMixBoundaryJacobi<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bbRowptr, d.bbCol, d.bbVal, d.bdRowptr, d.bdCol, d.bdVal, d.bDiag, src,
+                                                     d.e, d.rhs, dst, d.Nd, K, ms.omega);
+  }
+  if (nb == 1) KMixScatterB<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, buf[0], d.e, d.Nd, K);
 }
 
 static void Smooth(MixedSystem& ms, int l, int K, bool reverse, hipStream_t s) {
   DevMixLevel& d = ms.L[l];
   const int nu = reverse ? ms.nuPost : ms.nuPre;
   for (int it = 0; it < nu; it++) {
-    if (d.B > 0)
-      for (int bs = 0; bs < ms.boundarySweeps; bs++) {
-        const long bk = (long)d.B * K;
-        KMixBoundaryJacobi<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB, d.Nd, K,
-                                                         ms.omega);
-        KMixScatterB<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.tmpB, d.e, d.Nd, K);
-      }
+    BoundarySweeps(ms, d, K, s);
     if (d.nTiles > 0)
       for (int qq = 0; qq < 4; qq++) {
         int q = reverse ? 3 - qq : qq;
@@ -461,25 +547,29 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
   const int nl = ms.levels;
   for (int l = 0; l + 1 < nl; l++) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
-    KMixZero<<<NBlk(d.Nd * K, 256), 256, 0, s>>>(d.Nd * K, d.e);
-    Smooth(ms, l, K, false, s);
+    Smooth(ms, l, K, false, s);  // e of this level was zeroed by whoever wrote its rhs
     if (d.nTiles > 0)
       KMixResidualDeep<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
     if (d.B > 0)
       KMixResidualBoundary<<<NBlk((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
                                                                      d.Nd, K);
     if (c.nTiles > 0)
-      KMixRestrictDeep<<<dim3(c.nTiles, 4), 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, K);
+      KMixRestrictDeep<<<dim3(c.nTiles, 4), 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
+                                                         K);
     if (c.B > 0)
-      KMixRestrictBoundary<<<NBlk((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs, c.Nd, K);
+      KMixRestrictBoundary<<<NBlk((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs, c.e, c.Nd,
+                                                                     K);
   }
   {
     DevMixLevel& c = ms.L[nl - 1];
     const long nk = (long)c.N * K;
-    KMixGather<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, c.rhs, c.Nd, ms.coarseRhs, K);
-    LdltSolve(ms.coarse, ms.coarseRhs, c.N, ms.coarseX, c.N, nullptr, ms.coarseY, K, s);
-    KMixZero<<<NBlk(c.Nd * K, 256), 256, 0, s>>>(c.Nd * K, c.e);
-    KMixScatter<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, ms.coarseX, c.e, c.Nd, K);
+    if (ms.coarseInv) {
+      KMixDenseCoarse<<<c.N, 256, 0, s>>>(c.N, ms.coarseInv, c.g2d, c.rhs, c.e, c.Nd, K);
+    } else {
+      KMixGather<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, c.rhs, c.Nd, ms.coarseRhs, K);
+      LdltSolve(ms.coarse, ms.coarseRhs, c.N, ms.coarseX, c.N, nullptr, ms.coarseY, K, s);
+      KMixScatter<<<NBlk(nk, 256), 256, 0, s>>>(c.N, c.g2d, ms.coarseX, c.e, c.Nd, K);
+    }
   }
   for (int l = nl - 2; l >= 0; l--) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
@@ -494,14 +584,12 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
 static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, hipStream_t s) {
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
-  KMixZeroD<<<1, 64, 0, s>>>(K, ms.norms);
-  for (int ch = 0; ch < K; ch++) {
-    if (L0.nRaster > 0)
-      K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.Nd, ms.norms, ch);
-    if (L0.B > 0)
-      K64ResidualBoundary<<<NBlk(L0.B, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr, L0.bb.col,
-                                                          L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.Nd, ms.norms, ch);
-  }
+  KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  if (L0.nRaster > 0)
+    K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+  if (L0.B > 0)
+    K64ResidualBoundary<<<dim3(NBlk(L0.B, 256), K), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr, L0.bb.col,
+                                                                 L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms);
 }
 
 static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
@@ -515,17 +603,23 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
                hipStream_t s, bool useGraph) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
   const int N = ms.L[0].N;
-  KMixZeroD<<<1, 64, 0, s>>>(2 * K, ms.norms);
-  for (int ch = 0; ch < K; ch++) K64SquaredNorm<<<NBlk(N, 256), 256, 0, s>>>(N, b, ms.norms + K, ch);
+  double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
+  KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
+  K64SquaredNorm<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, b, bNorms);
   OuterResidual(ds, ms, x, b, K, s);
-  GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * K * sizeof(double), hipMemcpyDeviceToHost, s));
+  GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
+  auto slotSum = [&](const double* p) {
+    double t = 0;
+    for (int q = 0; q < kSlots; q++) t += p[q];
+    return t;
+  };
   std::vector<double> bn(K);
-  for (int k = 0; k < K; k++) bn[k] = std::sqrt(std::max(ms.hostNorms[K + k], 1e-300));
+  for (int k = 0; k < K; k++) bn[k] = std::sqrt(std::max(slotSum(ms.hostNorms + ((size_t)ms.kcap + k) * kSlots), 1e-300));
   auto converged = [&]() {
     bool ok = true;
     for (int k = 0; k < K; k++) {
-      relRes[k] = std::sqrt(ms.hostNorms[k]) / bn[k];
+      relRes[k] = std::sqrt(slotSum(ms.hostNorms + (size_t)k * kSlots)) / bn[k];
       if (!(relRes[k] <= tol)) ok = false;
     }
     return ok;
@@ -536,7 +630,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     MixedVCycle(ds, ms, K, st);
     OuterUpdate(ds, ms, x, K, st);
     OuterResidual(ds, ms, x, b, K, st);
-    GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, K * sizeof(double), hipMemcpyDeviceToHost, st));
+    GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));
   };
   if (useGraph && !(ms.graphExec && ms.graphK == K && ms.graphX == x && ms.graphB == b)) {
     // capture one refinement step on the solver's own stream (shape-static per K, x, b)
