@@ -5,7 +5,9 @@ Workload (default): noisy icosphere (327,680 triangles, 20 charts) at a 4096x409
 sides.  One step = one solve of all three columns from x = 0 until every column's relative
 residual ||b - A x|| / ||b|| <= 1e-6; one solve = one column (so a step is 3 solves).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode mixed|fp64] [--config c2|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode mixed|mixed-only|fp64] [--config c2|c1|small]
+       python bench.py --profile-kernel mixed|band   (calibration copy + the dominant kernel only, for
+                                                      rocprofv3 --pmc traffic counts)
 """
 import argparse
 import json
@@ -113,6 +115,48 @@ def band_roofline(S, H, x, b, r, reps=5):
                 kernel="KBandGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)
 
 
+COPY_BYTES = 1 << 28
+
+
+def profile_kernel(cfg, K, which, reps=10):
+    """Kernel-only run for hardware-counter passes: four copies of a 256 MiB buffer (the FETCH_SIZE /
+    WRITE_SIZE calibration: each reads and writes exactly COPY_BYTES), then `reps` launches of the
+    dominant kernel of the chosen path on the benchmark workload."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    mesh, H, S = make_system(cfg, K, mixed=(which == "mixed"))
+    if which == "mixed":
+        f, b = texture_rhs(H, S, K)
+        x = torch.zeros_like(b)
+        solve(S, x, b, 1e-6, 40, "mixed")  # leaves meaningful FP32 vectors in the solver's buffers
+        for _ in range(reps):
+            S.mixed_deep_sweep(0, K)
+    else:
+        f, b = texture_rhs(H, S, K)
+        x, r = f.clone(), torch.zeros_like(b)
+        for _ in range(reps):
+            S.band_sweep(0, x, b, r)
+    torch.cuda.synchronize()
+    log(f"[bench] profile-kernel {which}: {reps} launches done")
+
+
+def load_traffic(kernel, workload_key):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE passes, scaled
+    by the calibration copy) and recorded in profiles/traffic.json; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh).get(kernel)
+    if not rec or rec.get("workload") != workload_key:
+        return None
+    return rec["bytes_per_launch"]
+
+
 def cpu_baseline(H, S, K, tol, budget_s=60.0):
     """Oracle (plain-C restatement of the reference V-cycle + exact sparse solves) on one core,
     one solve of the same workload from x = 0, bounded by budget_s."""
@@ -151,12 +195,13 @@ def cpu_baseline(H, S, K, tol, budget_s=60.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="mixed", choices=["fp64", "mixed", "mixed-only"])
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver shards by requests / RHS columns, "
@@ -165,6 +210,9 @@ def main():
     import torch
     K = 3
     max_cycles = 40
+    if args.profile_kernel:
+        profile_kernel(args.config, K, args.profile_kernel)
+        return
     results = {}
     modes = {"mixed": ["mixed", "fp64"], "mixed-only": ["mixed"], "fp64": ["fp64"]}[args.mode]
     for mode in modes:
@@ -176,19 +224,23 @@ def main():
             log(f"[bench] {mode} warmup: {cycles} iterations, rel res {rel:.2e}")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        cyc = []
+        cyc, lat = [], []
         for _ in range(args.steps):
-            c, rel = solve(S, x, b, args.tol, max_cycles, mode)
+            ts = time.perf_counter()
+            c, rel = solve(S, x, b, args.tol, max_cycles, mode)  # returns after the device finished
+            lat.append((time.perf_counter() - ts) * 1e3)
             cyc.append(c)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        res = dict(value=K * args.steps / el, ms_per_step=el / args.steps * 1e3, iterations=cyc, rel_res=rel, levels=H.levels)
+        res = dict(value=K * args.steps / el, ms_per_step=el / args.steps * 1e3, iterations=cyc, rel_res=rel, levels=H.levels,
+                   latency_ms=dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99))))
         log(f"[bench] {mode}: {args.steps} steps in {el:.3f}s ({res['value']:.1f} solves/s), iterations/step {cyc}, "
             f"final rel res {rel:.2e}")
         if mode == "mixed":
             res["roofline"] = mixed_roofline(S, H, K)
         else:
             res["roofline"] = band_roofline(S, H, x, b, r)
+        res["roofline"]["traffic"] = load_traffic(res["roofline"]["kernel"], args.config)
         log(f"[bench] {mode} roofline: {res['roofline']}")
         if mode == "fp64" and args.cpu_baseline == "on":
             res["cpu"] = cpu_baseline(H, S, K, args.tol)
@@ -209,6 +261,7 @@ def main():
         "config": {"workload": f"{args.config}: {CONFIGS[args.config][0]} {CONFIGS[args.config][2]}^2 atlas, "
                                f"{info['texels']} texels, 3 RHS, (M+1e-3 S)x=Mf from x=0 to rel-res {args.tol:g}",
                    "mode": main_mode, "levels": H.levels, "iterations_per_step": cyc},
+        "latency_ms": res["latency_ms"],
         "fp64_reference_gpu": {k: v for k, v in fp.items() if k in ("value", "ms_per_step", "iterations", "levels")} if fp else None,
         "speedup_vs_fp64_gpu": value / fp["value"] if fp else None,
         "roofline": roof, "cpu_baseline": cpu,

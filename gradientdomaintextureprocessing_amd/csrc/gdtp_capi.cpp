@@ -377,7 +377,7 @@ int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream) {
 int gdtp_system_memory(const gdtp_system* s, size_t* refBytes, size_t* mixedBytes) {
   if (!s) return Fail(GDTP_ERROR_INVALID, "null system");
   if (refBytes) *refBytes = s->ds ? s->ds->bytes : 0;
-  if (mixedBytes) *mixedBytes = 0;
+  if (mixedBytes) *mixedBytes = s->mixed ? s->mixed->bytes : 0;
   return GDTP_OK;
 }
 

@@ -66,6 +66,8 @@ struct DeviceSystem {
 // Upload an LDL^T factor with its team / top level schedule; LdltSolve solves for K columns,
 // gathering src[perm] and scattering into dst (through `map` when not null).
 void UploadLdlt(std::vector<void*>& allocations, const Ldlt& F, DevLdlt& d);
+// Bytes uploaded / allocated by this thread's device uploads so far (monotone counter).
+size_t UploadByteCounter();
 void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s);
 
 void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds);

@@ -348,6 +348,7 @@ DeviceSystem::~DeviceSystem() {
 }
 
 static thread_local size_t g_uploadBytes = 0;
+size_t UploadByteCounter() { return g_uploadBytes; }
 
 template <class T>
 static T* UploadTo(std::vector<void*>& allocs, const std::vector<T>& v) {

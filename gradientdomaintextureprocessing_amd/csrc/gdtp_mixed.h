@@ -45,6 +45,7 @@ struct MixedSystem {
   double* norms = nullptr;         // device [2][kcap][64]: squared residual / rhs norm partial sums
   double* hostNorms = nullptr;     // pinned copy
   std::vector<void*> allocations;
+  size_t bytes = 0;                // device bytes held
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphK = 0;

@@ -350,6 +350,7 @@ static T* Up(MixedSystem& ms, const std::vector<T>& v) {
   T* p = nullptr;
   GDTP_HIP(hipMalloc(&p, v.size() * sizeof(T)));
   ms.allocations.push_back(p);
+  ms.bytes += v.size() * sizeof(T);
   GDTP_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return p;
 }
@@ -359,6 +360,7 @@ static T* Zeros(MixedSystem& ms, size_t n) {
   T* p = nullptr;
   GDTP_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
   ms.allocations.push_back(p);
+  ms.bytes += std::max<size_t>(n, 1) * sizeof(T);
   GDTP_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
   return p;
 }
@@ -489,7 +491,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
     d.tmpB2 = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
   }
+  const size_t before = UploadByteCounter();
   UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse);
+  ms.bytes += UploadByteCounter() - before;
   const int Nc = ms.L[nl - 1].N;
   if (Nc <= kDenseCoarseMax) {  // dense inverse of the coarsest operator, by columns of the LDL^T solve
     std::vector<double> inv((size_t)Nc * Nc), unit((size_t)Nc, 0.0), col((size_t)Nc);
