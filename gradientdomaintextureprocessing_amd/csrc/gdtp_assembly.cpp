@@ -146,6 +146,8 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
   op.deepStiffness.assign(10 * (size_t)L0.numDeep, 0.0);
   std::vector<std::vector<Triplet>> bbM(nCharts), bbS(nCharts), bdM(nCharts), bdS(nCharts);
   std::vector<std::string> errs(nCharts);
+  SeamContext seams;
+  if (!BuildSeamContext(h, seams, err)) return 0;
   const int offI[4] = {0, 1, 1, 0}, offJ[4] = {0, 0, 1, 1};
 #pragma omp parallel for schedule(dynamic, 1)
   for (int c = 0; c < nCharts; c++) {
@@ -155,7 +157,7 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     std::vector<std::array<double, 16>> cellM(g.numInteriorCells, std::array<double, 16>{}), cellS(g.numInteriorCells, std::array<double, 16>{});
     std::vector<std::array<double, 16>> bcellM(g.numBoundaryCells, std::array<double, 16>{}), bcellS(g.numBoundaryCells, std::array<double, 16>{});
     std::vector<V2> poly;
-    for (int t = 0; t < nTri; t++) {
+    for (int t = 0; t < nTri && errs[c].empty(); t++) {
       V2 p[3];
       for (int k = 0; k < 3; k++) p[k] = ac.vertices[ac.triangles[3 * t + k]] - g.corner;
       const M2& G = metric[c][t];
@@ -165,7 +167,7 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
       double fmx[2] = {std::max({p[0].x, p[1].x, p[2].x}), std::max({p[0].y, p[1].y, p[2].y})};
       int mn0 = (int)std::floor(std::max(fmn[0], 0.0) / g.cellW), mn1 = (int)std::floor(std::max(fmn[1], 0.0) / g.cellH);
       int mx0 = (int)std::ceil(std::min(fmx[0], 1.0) / g.cellW), mx1 = (int)std::ceil(std::min(fmx[1], 1.0) / g.cellH);
-      for (int j = mn1; j < mx1; j++)
+      for (int j = mn1; j < mx1 && errs[c].empty(); j++)
         for (int i = mn0; i < mx0; i++) {
           if (i < 0 || j < 0 || i >= g.width - 1 || j >= g.height - 1) continue;
           size_t q = (size_t)j * (g.width - 1) + i;
@@ -173,6 +175,11 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
           if (li < 0 && lb < 0) continue;
           poly.assign(p, p + 3);
           if (!ClipToCell(poly, i, j, g.cellW, g.cellH)) continue;
+          if (lb >= 0) {  // boundary cell: pieces touching a seam carry the averaged seam values
+            int rc = AssembleSeamPiece(seams, c, t, i, j, poly, G, bbM[c], bbS[c], errs[c]);
+            if (rc < 0) break;
+            if (rc > 0) continue;
+          }
           double m[4][4] = {}, s[4][4] = {};
           IntegrateCellPolygon(poly, i, j, g.cellW, g.cellH, Ginv, gscale, m, s);
           auto& CM = li >= 0 ? cellM[li] : bcellM[lb];
@@ -181,6 +188,7 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
             for (int b = 0; b < 4; b++) CM[4 * a + b] += m[a][b], CS[4 * a + b] += s[a][b];
         }
     }
+    if (!errs[c].empty()) continue;
     // interior cells: deep rows and the boundary rows they touch (MassAndStiffness.h:277-353)
     for (int ic = 0; ic < g.numInteriorCells; ic++) {
       const auto& gl = g.interiorCellGlobalCorners[ic];
@@ -208,8 +216,8 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
         }
       }
     }
-    // boundary cells and chart seams
-    if (!AssembleBoundaryCells(mesh, h, c, metric[c], bcellM, bcellS, bbM[c], bbS[c], errs[c])) continue;
+    // boundary cells (pieces away from seams)
+    AssembleBoundaryCells(h, c, bcellM, bcellS, bbM[c], bbS[c], errs[c]);
   }
   for (int c = 0; c < nCharts; c++)
     if (!errs[c].empty()) { err = errs[c]; return 0; }
@@ -224,8 +232,6 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     t = merge(bdM); op.bdMass = Csr::FromTriplets(L0.numBoundary, L0.numTexels, t);
     t = merge(bdS); op.bdStiffness = Csr::FromTriplets(L0.numBoundary, L0.numTexels, t);
   }
-  // Seam terms couple boundary texels of different charts; AssembleSeams adds them.
-  if (!AssembleSeams(mesh, h, metric, op, err)) return 0;
   return 1;
 }
 

@@ -28,8 +28,8 @@ struct gdtp_system {
   OperatorHierarchy oh;
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
-  int nuPre = 2, nuPost = 2, boundarySweeps = 4, useGraph = 1;
-  double omega = 0.7;
+  int nuPre = 2, nuPost = 2, boundarySweeps = 1, useGraph = 1;
+  double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
   ~gdtp_system() {

@@ -19,25 +19,29 @@ struct DevMixLevel {
   int* g2d = nullptr;              // [N] compact global index -> dense index
   int nTiles = 0;
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
-  // boundary rows (Jacobi smoothing and residual): dense index, off-diagonal CSR, 1/diagonal
+  // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;
-  int *bRowptr = nullptr, *bCol = nullptr;       // merged off-diagonal row, dense columns (residual)
+  int *bRowptr = nullptr, *bCol = nullptr;
   float *bVal = nullptr, *bDiag = nullptr;
-  int *bbRowptr = nullptr, *bbCol = nullptr;     // off-diagonal A_BB, boundary-index columns
-  float* bbVal = nullptr;
-  int *bdRowptr = nullptr, *bdCol = nullptr;     // A_BD, dense columns
-  float* bdVal = nullptr;
+  // overlapping Schwarz blocks of boundary rows: block b updates its core rows (dense indices
+  // blkCore[blkCorePtr[b]..]) from the residual at its extended rows (boundary indices
+  // blkExt[blkExtPtr[b]..]) through the core rows of the inverse of A restricted to the extended
+  // rows, stored FP32 column-major at blkMat + blkMatPtr[b] ([ext][core])
+  int nBlocks = 0;
+  int *blkCorePtr = nullptr, *blkCore = nullptr, *blkExtPtr = nullptr, *blkExt = nullptr;
+  long* blkMatPtr = nullptr;
+  float* blkMat = nullptr;
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
   int *rRowptr = nullptr, *rCol = nullptr;
   float* rVal = nullptr;
-  // vectors [K][Nd] and boundary scratch [K][B]
-  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr, *tmpB2 = nullptr;
+  // vectors [K][Nd] and boundary residual scratch [K][B]
+  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr;
 };
 
 struct MixedSystem {
   int levels = 0, kcap = 0;
-  int nuPre = 2, nuPost = 2, boundarySweeps = 4;
-  float omega = 0.7f;
+  int nuPre = 2, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per smoothing step
+  float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level
   double *coarseRhs = nullptr, *coarseX = nullptr, *coarseY = nullptr;

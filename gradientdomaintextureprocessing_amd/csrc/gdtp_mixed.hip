@@ -7,13 +7,18 @@
 // The FP32 V-cycle works on dense per-chart grids split into four colour planes by texel parity
 // (q = (i&1) | (j&1)<<1; texel (i,j) lives at plane q, sub-grid (i>>1, j>>1)).  A four-colour
 // Gauss-Seidel sweep then updates one plane per launch from fully coalesced neighbour planes.
-// Boundary / seam texels are relaxed by damped Jacobi over their sparse rows (A_BB and A_BD
-// merged, dense column indices); the coarsest level is solved exactly (FP64 LDL^T).
+// Boundary / seam texels are relaxed by restricted additive Schwarz: overlapping blocks of up to
+// 64 boundary rows, each solved exactly through a precomputed inverse (seam coupling makes the
+// boundary rows far from diagonally dominant, so pointwise smoothers stall); the coarsest level
+// is solved exactly in FP64 (dense inverse when small, else LDL^T).
 // Transfers are the reference's: 3x3 full weighting / the boundary restriction matrix, and
 // bilinear prolongation (IterativeSolvers.inl:1009-1168), evaluated from chart geometry.
-#include <stdexcept>
-
 #include "gdtp_mixed.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <unordered_map>
 
 namespace gdtp {
 
@@ -86,36 +91,50 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
 }
 
 // ------------------------------------------------------------------ boundary rows
-// One damped-Jacobi sweep over the boundary rows.  Boundary neighbours are read from `srcB`
-// (compact [K][B]) or, when srcB is null, from the dense vector; deep neighbours always from the
-// dense vector.  The new values go to `dstB` (compact) or, when null, into the dense vector.
-__global__ void KMixBoundaryJacobi(int B, const int* __restrict__ bdIdx, const int* __restrict__ rpb, const int* __restrict__ clb,
-                                   const float* __restrict__ vlb, const int* __restrict__ rpd, const int* __restrict__ cld,
-                                   const float* __restrict__ vld, const float* __restrict__ dg, const float* srcB, float* e,
-                                   const float* __restrict__ rhs, float* dstB, long Nd, int K, float omega) {
+// Boundary-row residual into the compact [K][B] layout.
+__global__ void KMixBoundaryResidualCompact(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                            const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
+                                            const float* __restrict__ rhs, float* resB, long Nd, int K) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)B * K) return;
   const int r = t % B, ch = t / B;
   const float* E = e + (long)ch * Nd;
-  const float* S = srcB ? srcB + (long)ch * B : nullptr;
-  float s = 0.0f;
-  for (int k = rpd[r]; k < rpd[r + 1]; k++) s += vld[k] * E[cld[k]];
-  if (S)
-    for (int k = rpb[r]; k < rpb[r + 1]; k++) s += vlb[k] * S[clb[k]];
-  else
-    for (int k = rpb[r]; k < rpb[r + 1]; k++) s += vlb[k] * E[bdIdx[clb[k]]];
   const long d = bdIdx[r];
-  const float xo = S ? S[r] : E[d];
-  const float xn = xo + omega * ((rhs[(long)ch * Nd + d] - s) / dg[r] - xo);
-  if (dstB) dstB[(long)ch * B + r] = xn;
-  else e[(long)ch * Nd + d] = xn;
+  float s = dg[r] * E[d];
+  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
+  resB[(long)ch * B + r] = rhs[(long)ch * Nd + d] - s;
 }
 
-__global__ void KMixScatterB(int B, const int* __restrict__ bdIdx, const float* __restrict__ tmp, float* e, long Nd, int K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)B * K) return;
-  const int r = t % B, ch = t / B;
-  e[(long)ch * Nd + bdIdx[r]] = tmp[(long)ch * B + r];
+// Restricted additive Schwarz over the boundary rows: one wave per block; thread i owns core
+// row i (< 64) and accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time,
+// the block's residual staged in LDS.  Cores of different blocks are disjoint.
+static constexpr int kBlockCore = 64, kBlockExtMax = 384;
+__global__ __launch_bounds__(64) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
+                                                  const int* __restrict__ extPtr, const int* __restrict__ ext,
+                                                  const long* __restrict__ matPtr, const float* __restrict__ mat,
+                                                  const float* __restrict__ resB, int B, float* e, long Nd, int K, float omega) {
+  __shared__ float R[4 * kBlockExtMax];
+  const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
+  const float* M = mat + matPtr[b];
+  const int i = threadIdx.x;
+  for (int ch0 = 0; ch0 < K; ch0 += 4) {
+    const int kc = min(4, K - ch0);
+    for (int j = i; j < nx; j += 64)
+      for (int q = 0; q < kc; q++) R[q * kBlockExtMax + j] = resB[(long)(ch0 + q) * B + ext[x0 + j]];
+    __syncthreads();
+    if (i < nc) {
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int j = 0; j < nx; j++) {
+        const float m = M[(long)j * nc + i];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
+      }
+      const long d = core[c0 + i];
+      for (int q = 0; q < kc; q++) e[(long)(ch0 + q) * Nd + d] += omega * acc[q];
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void KMixResidualBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
@@ -365,6 +384,115 @@ static T* Zeros(MixedSystem& ms, size_t n) {
   return p;
 }
 
+struct SchwarzBlocks {
+  std::vector<int> corePtr{0}, core, extPtr{0}, ext;
+  std::vector<long> matPtr{0};
+  std::vector<float> mat;
+};
+
+// Blocks of up to kBlockCore boundary rows grown breadth-first over the A_BB graph, each extended
+// by its A_BB neighbours (the most strongly coupled first, up to kBlockExtMax rows); per block the
+// core rows of the inverse of A restricted to the extended rows (FP64 Cholesky, stored FP32).
+static void BuildSchwarzBlocks(const Csr& bb, const std::vector<int>& bidx, SchwarzBlocks& out) {
+  const int B = bb.rows;
+  std::vector<int> owner(B, -1), diagPos(B, -1);
+  for (int r = 0; r < B; r++)
+    for (int k = bb.rowptr[r]; k < bb.rowptr[r + 1]; k++)
+      if (bb.col[k] == r) diagPos[r] = k;
+  std::vector<std::vector<int>> cores;
+  for (int seed = 0; seed < B; seed++) {
+    if (owner[seed] >= 0) continue;
+    const int id = (int)cores.size();
+    std::vector<int> c{seed};
+    owner[seed] = id;
+    for (size_t q = 0; q < c.size() && (int)c.size() < kBlockCore; q++)
+      for (int k = bb.rowptr[c[q]]; k < bb.rowptr[c[q] + 1] && (int)c.size() < kBlockCore; k++) {
+        const int v = bb.col[k];
+        if (owner[v] < 0) owner[v] = id, c.push_back(v);
+      }
+    cores.push_back(std::move(c));
+  }
+  const int nb = (int)cores.size();
+  std::vector<std::vector<int>> exts(nb);
+  std::vector<std::vector<float>> mats(nb);
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int b = 0; b < nb; b++) {
+    const std::vector<int>& c = cores[b];
+    const int nc = (int)c.size();
+    std::vector<std::pair<double, int>> nbr;  // (-strength, row)
+    for (int u : c)
+      for (int k = bb.rowptr[u]; k < bb.rowptr[u + 1]; k++) {
+        const int v = bb.col[k];
+        if (owner[v] == b) continue;
+        const double st = std::fabs(bb.val[k]) / std::sqrt(bb.val[diagPos[u]] * bb.val[diagPos[v]]);
+        nbr.push_back({-st, v});
+      }
+    std::sort(nbr.begin(), nbr.end());
+    std::vector<int>& x = exts[b];
+    x = c;
+    for (const auto& p : nbr) {
+      if ((int)x.size() >= kBlockExtMax) break;
+      if (std::find(x.begin() + nc, x.end(), p.second) == x.end()) x.push_back(p.second);
+    }
+    const int n = (int)x.size();
+    std::unordered_map<int, int> pos;
+    for (int i = 0; i < n; i++) pos[x[i]] = i;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+      for (int k = bb.rowptr[x[i]]; k < bb.rowptr[x[i] + 1]; k++) {
+        auto it = pos.find(bb.col[k]);
+        if (it != pos.end()) A[(size_t)i * n + it->second] = bb.val[k];
+      }
+    // Cholesky A = L L^T (lower triangle in place)
+    bool ok = true;
+    for (int j = 0; j < n && ok; j++) {
+      double d = A[(size_t)j * n + j];
+      for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+      if (!(d > 0)) { ok = false; break; }
+      d = std::sqrt(d);
+      A[(size_t)j * n + j] = d;
+      for (int i = j + 1; i < n; i++) {
+        double v = A[(size_t)i * n + j];
+        for (int k = 0; k < j; k++) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+        A[(size_t)i * n + j] = v / d;
+      }
+    }
+    if (!ok) {
+#pragma omp atomic write
+      failed = 1;
+      continue;
+    }
+    std::vector<float>& M = mats[b];
+    M.assign((size_t)n * nc, 0.0f);
+    std::vector<double> z(n);
+    for (int i = 0; i < nc; i++) {  // column i of the inverse (= row i: the inverse is symmetric)
+      std::fill(z.begin(), z.end(), 0.0);
+      z[i] = 1.0;
+      for (int r = 0; r < n; r++) {
+        double v = z[r];
+        for (int k = 0; k < r; k++) v -= A[(size_t)r * n + k] * z[k];
+        z[r] = v / A[(size_t)r * n + r];
+      }
+      for (int r = n - 1; r >= 0; r--) {
+        double v = z[r];
+        for (int k = r + 1; k < n; k++) v -= A[(size_t)k * n + r] * z[k];
+        z[r] = v / A[(size_t)r * n + r];
+      }
+      for (int j = 0; j < n; j++) M[(size_t)j * nc + i] = (float)z[j];
+    }
+  }
+  if (failed) throw std::runtime_error("boundary Schwarz block is not positive definite");
+  for (int b = 0; b < nb; b++) {
+    for (int u : cores[b]) out.core.push_back(bidx[u]);
+    out.corePtr.push_back((int)out.core.size());
+    out.ext.insert(out.ext.end(), exts[b].begin(), exts[b].end());
+    out.extPtr.push_back((int)out.ext.size());
+    out.mat.insert(out.mat.end(), mats[b].begin(), mats[b].end());
+    out.matPtr.push_back((long)out.mat.size());
+  }
+}
+
 void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& ds, int kcap, MixedSystem& ms) {
   const int nl = (int)h.levels.size();
   if (nl < 2) throw std::runtime_error("the mixed-precision solver needs at least two levels");
@@ -446,16 +574,11 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         if (!(dg[r] > 0.0f)) throw std::runtime_error("boundary row without positive diagonal");
       }
       d.bdIdx = Up(ms, bidx), d.bRowptr = Up(ms, rp), d.bCol = Up(ms, cl), d.bVal = Up(ms, vl), d.bDiag = Up(ms, dg);
-      std::vector<int> rpb(1, 0), clb, rpd(1, 0), cld;
-      std::vector<float> vlb, vld;
-      for (int r = 0; r < d.B; r++) {
-        for (int k = bb.rowptr[r]; k < bb.rowptr[r + 1]; k++)
-          if (bb.col[k] != r) clb.push_back(bb.col[k]), vlb.push_back((float)bb.val[k]);
-        for (int k = bd.rowptr[r]; k < bd.rowptr[r + 1]; k++) cld.push_back(g2d[bd.col[k]]), vld.push_back((float)bd.val[k]);
-        rpb.push_back((int)clb.size()), rpd.push_back((int)cld.size());
-      }
-      d.bbRowptr = Up(ms, rpb), d.bbCol = Up(ms, clb), d.bbVal = Up(ms, vlb);
-      d.bdRowptr = Up(ms, rpd), d.bdCol = Up(ms, cld), d.bdVal = Up(ms, vld);
+      SchwarzBlocks blk;
+      BuildSchwarzBlocks(bb, bidx, blk);
+      d.nBlocks = (int)blk.corePtr.size() - 1;
+      d.blkCorePtr = Up(ms, blk.corePtr), d.blkCore = Up(ms, blk.core), d.blkExtPtr = Up(ms, blk.extPtr), d.blkExt = Up(ms, blk.ext);
+      d.blkMatPtr = Up(ms, blk.matPtr), d.blkMat = Up(ms, blk.mat);
     } else {
       std::vector<int> bidx(d.B);
       for (int r = 0; r < d.B; r++) bidx[r] = g2d[H.boundaryGlobalIndex[r]];
@@ -489,7 +612,6 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     d.rhs = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.res = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
-    d.tmpB2 = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
   }
   const size_t before = UploadByteCounter();
   UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse);
@@ -515,29 +637,28 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 }
 
 static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s) {
-  const int nb = ms.boundarySweeps;
-  if (d.B == 0 || nb == 0) return;
-  const long bk = (long)d.B * K;
-  float* buf[2] = {d.tmpB, d.tmpB2};
-  for (int sw = 0; sw < nb; sw++) {
-    const float* src = sw == 0 ? nullptr : buf[(sw - 1) % 2];
-    float* dst = (sw == nb - 1 && nb > 1) ? nullptr : buf[sw % 2];
-    KMixBoundaryJacobi<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bbRowptr, d.bbCol, d.bbVal, d.bdRowptr, d.bdCol, d.bdVal, d.bDiag, src,
-                                                     d.e, d.rhs, dst, d.Nd, K, ms.omega);
+  if (d.B == 0 || d.nBlocks == 0) return;
+  for (int sw = 0; sw < ms.boundarySweeps; sw++) {
+    KMixBoundaryResidualCompact<<<NBlk((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB,
+                                                                         d.Nd, K);
+    KMixSchwarz<<<d.nBlocks, 64, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
+                                         ms.omega);
   }
-  if (nb == 1) KMixScatterB<<<NBlk(bk, 256), 256, 0, s>>>(d.B, d.bdIdx, buf[0], d.e, d.Nd, K);
 }
 
-static void Smooth(MixedSystem& ms, int l, int K, bool reverse, hipStream_t s) {
+// Pre-smoothing: (boundary Schwarz, deep colours 0..3) x nuPre; post-smoothing is its adjoint
+// order: (deep colours 3..0, boundary Schwarz) x nuPost.
+static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
   DevMixLevel& d = ms.L[l];
-  const int nu = reverse ? ms.nuPost : ms.nuPre;
+  const int nu = post ? ms.nuPost : ms.nuPre;
   for (int it = 0; it < nu; it++) {
-    BoundarySweeps(ms, d, K, s);
+    if (!post) BoundarySweeps(ms, d, K, s);
     if (d.nTiles > 0)
       for (int qq = 0; qq < 4; qq++) {
-        int q = reverse ? 3 - qq : qq;
+        int q = post ? 3 - qq : qq;
         KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
       }
+    if (post) BoundarySweeps(ms, d, K, s);
   }
 }
 
