@@ -4,11 +4,11 @@ The product is libgdtp.so (HIP kernels for gfx950 + native host set-up behind th
 include/gdtp.h); this package is its Python binding and host-side call surface.
 """
 from .api import (METRIC_EMBEDDING, METRIC_UNIFORM, OPERATOR_MASS, OPERATOR_STIFFNESS, OPERATOR_SYSTEM, AddProlongation,
-                  HierarchicalSystem, InitializeHierarchy, MultigridSystem, MultiplyByRestriction,
-                  MultiplyBySystemMatrix_NoReciprocals, Relaxation, RelaxationAndResidual, TexturedMesh, UpdateLinearSystem,
-                  VCycle)
+                  Geodesics, HierarchicalSystem, InitializeHierarchy, MultigridSystem, MultiplyByRestriction,
+                  MultiplyBySystemMatrix_NoReciprocals, Relaxation, RelaxationAndResidual, TextureFilter, TexturedMesh,
+                  UpdateLinearSystem, VCycle)
 
 __all__ = ["TexturedMesh", "HierarchicalSystem", "MultigridSystem", "InitializeHierarchy", "UpdateLinearSystem", "VCycle",
            "Relaxation", "RelaxationAndResidual", "MultiplyByRestriction", "AddProlongation",
            "MultiplyBySystemMatrix_NoReciprocals", "OPERATOR_SYSTEM", "OPERATOR_MASS", "OPERATOR_STIFFNESS",
-           "METRIC_EMBEDDING", "METRIC_UNIFORM"]
+           "METRIC_EMBEDDING", "METRIC_UNIFORM", "TextureFilter", "Geodesics"]

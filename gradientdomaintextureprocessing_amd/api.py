@@ -172,6 +172,10 @@ class MultigridSystem:
     def residual(self, x, b, r):
         _lib.check(_lib.load().gdtp_residual(self._h, _ptr(x), _ptr(b), _ptr(r), x.shape[0], _stream()))
 
+    def geodesic_divergence(self, potential, rhs):
+        """rhs = integral < -grad u / |grad u|_g , grad phi_t >_g per column of `potential`."""
+        _lib.check(_lib.load().gdtp_geodesic_divergence(self._h, _ptr(potential), _ptr(rhs), potential.shape[0], _stream()))
+
     def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed"):
         """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res)."""
         rel = np.zeros(x.shape[0])
@@ -228,3 +232,76 @@ def AddProlongation(system, level, coarse, fine):
 
 def MultiplyBySystemMatrix_NoReciprocals(system, which, x, y):
     system.apply(which, x, y)
+
+
+# ---------------------------------------------------------------------------- application recipes
+def _solve_mode(mode):
+    if mode not in ("mixed", "fp64"):
+        raise ValueError("mode must be 'mixed' or 'fp64'")
+    return mode
+
+
+class TextureFilter:
+    """Screened-Poisson texture filtering (TextureFiltering.cpp:581-718, 562-579):
+        (M + a S) x = M x0 + s a S x0,   x starts at x0,
+    with a the stiffness (smoothing) weight and s the sharpness weight (s = 0: pure smoothing,
+    s > 1: sharpening).  Columns are channels."""
+
+    def __init__(self, hierarchy, stiffness_weight=1e-3, sharpness_weight=1.0, max_cols=3):
+        self.hierarchy, self.stiffness, self.sharpness = hierarchy, stiffness_weight, sharpness_weight
+        self.system = UpdateLinearSystem(hierarchy, stiffness_weight, 1.0, max_cols=max_cols)
+
+    def set_stiffness(self, stiffness_weight):
+        """UpdateLinearSystem on a new weight (the app's slider, TextureFiltering.cpp:515-545)."""
+        self.stiffness = stiffness_weight
+        self.system.update(stiffness_weight, 1.0)
+
+    def rhs(self, x0):
+        import torch
+        m, s = torch.empty_like(x0), torch.empty_like(x0)
+        self.system.apply(OPERATOR_MASS, x0, m)
+        if self.sharpness == 0:
+            return m
+        self.system.apply(OPERATOR_STIFFNESS, x0, s)
+        return m + (self.sharpness * self.stiffness) * s
+
+    def filter(self, x0, tol=1e-6, mode="mixed", max_iters=60):
+        """Filtered texels for input texels x0 ([K, N] FP64 on the GPU); returns (x, iterations)."""
+        b = self.rhs(x0)
+        x = x0.clone()
+        its, _ = self.system.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        return x, its
+
+
+class Geodesics:
+    """Heat-method geodesic distances from source texels (Geodesics.cpp:508-550, 600-716):
+        (M + t S) u = delta_source,   rhs = div(-grad u / |grad u|),   (1e-6 M + S) phi = rhs,
+    and phi -= phi[source].  One column per source.  t defaults to the reference's 1e-3
+    (Geodesics.cpp:19), in the area-normalised metric."""
+
+    def __init__(self, hierarchy, heat_time=1e-3, max_sources=8):
+        self.hierarchy, self.t = hierarchy, heat_time
+        self.heat = UpdateLinearSystem(hierarchy, heat_time, 1.0, max_cols=max_sources)
+        self.distance = UpdateLinearSystem(hierarchy, 1.0, 1e-6, max_cols=max_sources)
+        self.max_sources = max_sources
+
+    def distances(self, sources, tol=1e-6, mode="mixed", max_iters=80):
+        """Distances ([len(sources), N] FP64 on the GPU) from each source texel (global index)."""
+        import torch
+        K = len(sources)
+        if not 1 <= K <= self.max_sources:
+            raise ValueError(f"between 1 and {self.max_sources} sources")
+        N = self.hierarchy.level_info(0)["texels"]
+        src = torch.as_tensor(np.asarray(sources, dtype=np.int64), device="cuda")
+        if int(src.min()) < 0 or int(src.max()) >= N:
+            raise ValueError("source texel out of range")
+        delta = torch.zeros(K, N, dtype=torch.float64, device="cuda")
+        delta[torch.arange(K, device="cuda"), src] = 1.0
+        u = torch.zeros_like(delta)
+        self.heat.solve(u, delta, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        div = torch.empty_like(u)
+        self.heat.geodesic_divergence(u, div)
+        phi = torch.zeros_like(u)
+        self.distance.solve(phi, div, tol=tol, max_iters=max_iters, mode=mode)
+        phi -= phi[torch.arange(K, device="cuda"), src][:, None]
+        return phi

@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "libgdtp.so")
 BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HOST_SOURCES = ["gdtp_mesh.cpp", "gdtp_hierarchy.cpp", "gdtp_assembly.cpp", "gdtp_boundary.cpp", "gdtp_ldlt.cpp",
+HOST_SOURCES = ["gdtp_mesh.cpp", "gdtp_hierarchy.cpp", "gdtp_assembly.cpp", "gdtp_boundary.cpp", "gdtp_vectorfield.cpp", "gdtp_ldlt.cpp",
                 "gdtp_galerkin.cpp"]
 HIP_SOURCES = ["gdtp_device.hip", "gdtp_mixed.hip", "gdtp_capi.cpp"]
 

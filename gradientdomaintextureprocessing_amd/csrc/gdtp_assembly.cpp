@@ -97,7 +97,7 @@ void ClipConvex(std::vector<V2>& poly, V2 n, double offset) {
 }
 
 // Clip to the primal cell [i cw,(i+1) cw] x [j ch,(j+1) ch] (ClipTriangleToPrimalCell, PolygonClipping.h:85-118).
-static bool ClipToCell(std::vector<V2>& poly, int i, int j, double cw, double ch) {
+bool ClipToCell(std::vector<V2>& poly, int i, int j, double cw, double ch) {
   ClipConvex(poly, V2(0, 1), j * ch);
   if (poly.empty()) return false;
   ClipConvex(poly, V2(0, -1), -(j + 1) * ch);

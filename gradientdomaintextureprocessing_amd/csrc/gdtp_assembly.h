@@ -26,6 +26,10 @@ void ComputeParameterMetric(const TexturedMesh& mesh, const Atlas& atlas, int mo
 void ComputeParameterMetricFromSurface(const TexturedMesh& mesh, const Atlas& atlas, const std::vector<M2>& surfaceMetric,
                                        std::vector<std::vector<M2>>& metric);
 
+// Clip a convex polygon (chart-local coordinates) to the primal cell [i cw,(i+1) cw] x [j ch,(j+1) ch];
+// false if nothing is left (ClipTriangleToPrimalCell, PolygonClipping.h:85-118).
+bool ClipToCell(std::vector<V2>& poly, int i, int j, double cw, double ch);
+
 // Assemble mass and stiffness over interior and boundary cells (MassAndStiffness.h:5-549).
 int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric,
                          FineOperator& op, std::string& err);

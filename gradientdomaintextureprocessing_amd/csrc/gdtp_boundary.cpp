@@ -25,7 +25,7 @@ const double kQP[6][2] = {{0.816847572980459, 0.0915762135097705}, {0.0915762135
 const double kQW[6] = {0.109951743655322, 0.109951743655322, 0.109951743655322,
                        0.223381589678011, 0.223381589678011, 0.223381589678011};
 
-using Weights = std::vector<std::pair<int, double>>;  // (global boundary index, weight)
+using Weights = NodeWeights;
 
 V2 ChartVertex(const Hierarchy& h, int c, int lv) { return h.atlas.charts[c].vertices[lv] - h.levels[0].charts[c].corner; }
 
@@ -185,8 +185,8 @@ int BuildSeamContext(const Hierarchy& h, SeamContext& ctx, std::string& err) {
   return 1;
 }
 
-int AssembleSeamPiece(const SeamContext& ctx, int c, int t, int i, int j, const std::vector<V2>& piece, const M2& metric,
-                      std::vector<Triplet>& bbM, std::vector<Triplet>& bbS, std::string& err) {
+int SeamPieceElements(const SeamContext& ctx, int c, int t, int i, int j, const std::vector<V2>& piece, std::vector<SeamElement>& els,
+                      std::string& err) {
   const Hierarchy& h = *ctx.h;
   const AtlasChart& ac = h.atlas.charts[c];
   const GridChart& g = h.levels[0].charts[c];
@@ -276,6 +276,28 @@ int AssembleSeamPiece(const SeamContext& ctx, int c, int t, int i, int j, const 
   cen = cen / (double)n;
   Weights wc;
   if (!Bilinear(h, c, cen, i, j, 1.0, wc, err)) return -1;
+  for (int q = 0; q < m; q++) {
+    const int r = (q + 1) % m;
+    SeamElement el;
+    el.p[0] = cen, el.p[1] = ring[q].p, el.p[2] = ring[r].p;
+    if (std::fabs(cross(el.p[1] - el.p[0], el.p[2] - el.p[0])) < 1e-24 * g.cellW * g.cellH) continue;
+    el.w[0] = wc, el.w[1] = rw[q], el.w[2] = rw[r];
+    if (!Bilinear(h, c, (cen + ring[q].p) * 0.5, i, j, 1.0, el.w[3], err)) return -1;
+    if (ringEdge[q].k >= 0) {
+      const Node mid{(ring[q].p + ring[r].p) * 0.5, 3 + ringEdge[q].k, 0.5 * (ringEdge[q].s0 + ringEdge[q].s1)};
+      if (!weightsOf(mid, el.w[4])) return -1;
+    } else if (!Bilinear(h, c, (ring[q].p + ring[r].p) * 0.5, i, j, 1.0, el.w[4], err)) return -1;
+    if (!Bilinear(h, c, (ring[r].p + cen) * 0.5, i, j, 1.0, el.w[5], err)) return -1;
+    els.push_back(std::move(el));
+  }
+  return 1;
+}
+
+int AssembleSeamPiece(const SeamContext& ctx, int c, int t, int i, int j, const std::vector<V2>& piece, const M2& metric,
+                      std::vector<Triplet>& bbM, std::vector<Triplet>& bbS, std::string& err) {
+  std::vector<SeamElement> els;
+  const int rc = SeamPieceElements(ctx, c, t, i, j, piece, els, err);
+  if (rc <= 0) return rc;
   // local dense accumulation over the boundary texels this piece touches
   std::vector<int> ids;
   auto local = [&](int b) {
@@ -296,23 +318,12 @@ int AssembleSeamPiece(const SeamContext& ctx, int c, int t, int i, int j, const 
   };
   const M2 Ginv = metric.inv();
   const double gscale = std::sqrt(metric.det());
-  for (int q = 0; q < m; q++) {
-    const int r = (q + 1) % m;
-    const V2 tri[3] = {cen, ring[q].p, ring[r].p};
-    if (std::fabs(cross(tri[1] - tri[0], tri[2] - tri[0])) < 1e-24 * g.cellW * g.cellH) continue;
-    Weights nw[6];
-    nw[0] = wc, nw[1] = rw[q], nw[2] = rw[r];
-    if (!Bilinear(h, c, (cen + ring[q].p) * 0.5, i, j, 1.0, nw[3], err)) return -1;
-    if (ringEdge[q].k >= 0) {
-      const Node mid{(ring[q].p + ring[r].p) * 0.5, 3 + ringEdge[q].k, 0.5 * (ringEdge[q].s0 + ringEdge[q].s1)};
-      if (!weightsOf(mid, nw[4])) return -1;
-    } else if (!Bilinear(h, c, (ring[q].p + ring[r].p) * 0.5, i, j, 1.0, nw[4], err)) return -1;
-    if (!Bilinear(h, c, (ring[r].p + cen) * 0.5, i, j, 1.0, nw[5], err)) return -1;
+  for (const SeamElement& el : els) {
     double Me[6][6], Se[6][6];
-    P2Element(tri, Ginv, gscale, Me, Se);
+    P2Element(el.p, Ginv, gscale, Me, Se);
     std::vector<std::pair<int, double>> lw[6];
     for (int a = 0; a < 6; a++)
-      for (const auto& x : nw[a]) lw[a].push_back({local(x.first), x.second});
+      for (const auto& x : el.w[a]) lw[a].push_back({local(x.first), x.second});
     grow();
     const size_t nl = ids.size();
     for (int a = 0; a < 6; a++)

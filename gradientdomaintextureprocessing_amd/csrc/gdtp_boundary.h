@@ -28,6 +28,20 @@ struct SeamContext {
 
 int BuildSeamContext(const Hierarchy& h, SeamContext& ctx, std::string& err);
 
+// (global boundary index, weight) pairs: the value of a node as a combination of texel values.
+using NodeWeights = std::vector<std::pair<int, double>>;
+
+// One quadratic triangle of a seam piece (chart-local coordinates); nodes: corners 0,1,2, then the
+// midpoints of (0,1), (1,2), (2,0).
+struct SeamElement {
+  V2 p[3];
+  NodeWeights w[6];
+};
+
+// The quadratic elements of a piece that touches a seam (1), none if it does not (0), -1 on error.
+int SeamPieceElements(const SeamContext& ctx, int c, int t, int i, int j, const std::vector<V2>& piece, std::vector<SeamElement>& els,
+                      std::string& err);
+
 // Piece = (chart triangle t of chart c) ∩ (boundary cell (i,j)), as a convex polygon in chart-local
 // coordinates.  If the piece touches a seam, integrate its quadratic elements and append the
 // boundary-boundary triplets (global boundary indices) to bbM / bbS and return 1; return 0 if it

@@ -1,6 +1,7 @@
 // C ABI of libgdtp (declared in include/gdtp.h).
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -19,7 +20,21 @@ struct gdtp_hierarchy {
   TexturedMesh mesh;
   Hierarchy h;
   FineOperator op;
+  std::vector<std::vector<M2>> metric;                 // per chart triangle, UV coordinates
+  mutable std::unique_ptr<VectorFieldSamples> vf;      // built on first use
+  mutable std::mutex vfMutex;
 };
+
+static const VectorFieldSamples& Samples(const gdtp_hierarchy* h) {
+  std::lock_guard<std::mutex> lock(h->vfMutex);
+  if (!h->vf) {
+    auto vs = std::make_unique<VectorFieldSamples>();
+    std::string err;
+    if (!BuildVectorFieldSamples(h->h, h->metric, *vs, err)) throw std::runtime_error(err);
+    h->vf = std::move(vs);
+  }
+  return *h->vf;
+}
 
 struct gdtp_system {
   const gdtp_hierarchy* H = nullptr;
@@ -28,6 +43,7 @@ struct gdtp_system {
   OperatorHierarchy oh;
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
+  std::unique_ptr<DevVectorField> vf;
   int nuPre = 2, nuPost = 2, boundarySweeps = 1, useGraph = 1;
   double omega = 1.0;
   double* dNorms = nullptr;
@@ -134,9 +150,8 @@ int gdtp_hierarchy_create(const gdtp_mesh* mesh, int W, int H, int levels, int m
     std::string err;
     BlockInfo bi;
     if (!BuildHierarchy(h->mesh, W, H, levels, bi, h->h, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
-    std::vector<std::vector<M2>> metric;
-    ComputeParameterMetric(h->mesh, h->h.atlas, metricMode == GDTP_METRIC_UNIFORM ? METRIC_UNIFORM : METRIC_EMBEDDING, metric);
-    if (!AssembleFineOperator(h->mesh, h->h, metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    ComputeParameterMetric(h->mesh, h->h.atlas, metricMode == GDTP_METRIC_UNIFORM ? METRIC_UNIFORM : METRIC_EMBEDDING, h->metric);
+    if (!AssembleFineOperator(h->mesh, h->h, h->metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
     *out = h;
     return GDTP_OK;
   })
@@ -170,6 +185,13 @@ static bool CsrI32(const Csr& m, const std::string& name, const std::string& bas
 
 int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cname, int* out, size_t* count) {
   if (!h || !cname || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  if (std::strncmp(cname, "vf.", 3) == 0 && level == 0) {  // samples are built on first use
+    try {
+      Samples(h);
+    } catch (const std::exception& e) {
+      return Fail(GDTP_ERROR_GEOMETRY, e.what());
+    }
+  }
   const Level& L = h->h.levels[level];
   const std::string name(cname);
   std::vector<int> v;
@@ -197,7 +219,10 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
   else if (name == "chartTexelIndex") { for (auto& c : L.charts) v.insert(v.end(), c.globalTexel.begin(), c.globalTexel.end()); }
   else if (CsrI32(L.boundaryRestriction, name, "boundaryRestriction", v) || CsrI32(L.boundaryCoarseFineProlongation, name, "boundaryCoarseFineProlongation", v) ||
            CsrI32(L.boundaryFineCoarseRestriction, name, "boundaryFineCoarseRestriction", v)) {
-  } else if (level == 0 && (CsrI32(h->op.bbMass, name, "mass.bb", v) || CsrI32(h->op.bbStiffness, name, "stiffness.bb", v) ||
+  } else if (level == 0 && name == "vf.cellCorners") v = Samples(h).cellCorners;
+  else if (level == 0 && name == "vf.genPtr") v = Samples(h).genPtr;
+  else if (level == 0 && name == "vf.genTexel") v = Samples(h).genTexel;
+  else if (level == 0 && (CsrI32(h->op.bbMass, name, "mass.bb", v) || CsrI32(h->op.bbStiffness, name, "stiffness.bb", v) ||
                             CsrI32(h->op.bdMass, name, "mass.bd", v) || CsrI32(h->op.bdStiffness, name, "stiffness.bd", v))) {
   } else return Fail(GDTP_ERROR_INVALID, "unknown hierarchy array " + name);
   return Emit(v, out, count);
@@ -205,6 +230,13 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
 
 int gdtp_hierarchy_array_f64(const gdtp_hierarchy* h, int level, const char* cname, double* out, size_t* count) {
   if (!h || !cname || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid arguments");
+  if (std::strncmp(cname, "vf.", 3) == 0 && level == 0) {  // samples are built on first use
+    try {
+      Samples(h);
+    } catch (const std::exception& e) {
+      return Fail(GDTP_ERROR_GEOMETRY, e.what());
+    }
+  }
   const Level& L = h->h.levels[level];
   const std::string name(cname);
   std::vector<double> v;
@@ -213,6 +245,14 @@ int gdtp_hierarchy_array_f64(const gdtp_hierarchy* h, int level, const char* cna
   else if (name == "boundaryFineCoarseRestriction.val") v = L.boundaryFineCoarseRestriction.val;
   else if (name == "boundaryBoundaryWeights") { for (auto& x : L.boundaryBoundaryIndices) v.push_back(x.weight); }
   else if (name == "chartsGeom") { for (auto& c : L.charts) v.insert(v.end(), {c.corner.x, c.corner.y, c.cellW, c.cellH}); }
+  else if (level == 0 && name.rfind("vf.", 0) == 0) {
+    const VectorFieldSamples& vs = Samples(h);
+    const std::vector<float>* f = name == "vf.cellData" ? &vs.cellData : name == "vf.genTensor" ? &vs.genTensor
+                                : name == "vf.genG" ? &vs.genG : name == "vf.genV" ? &vs.genV : nullptr;
+    if (f) v.assign(f->begin(), f->end());
+    else if (name == "vf.invCell") v = {vs.invCellW, vs.invCellH};
+    else return Fail(GDTP_ERROR_INVALID, "unknown hierarchy array " + name);
+  }
   else if (level == 0 && name == "mass.deep") v = h->op.deepMass;
   else if (level == 0 && name == "stiffness.deep") v = h->op.deepStiffness;
   else if (level == 0 && name == "mass.bb.val") v = h->op.bbMass.val;
@@ -379,6 +419,17 @@ int gdtp_system_memory(const gdtp_system* s, size_t* refBytes, size_t* mixedByte
   if (refBytes) *refBytes = s->ds ? s->ds->bytes : 0;
   if (mixedBytes) *mixedBytes = s->mixed ? s->mixed->bytes : 0;
   return GDTP_OK;
+}
+
+int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rhs, int ncols, void* stream) {
+  GDTP_DEVICE_OP(potential && rhs, {
+    if (!s->vf) {
+      auto d = std::make_unique<DevVectorField>();
+      UploadVectorField(Samples(s->H), *d);
+      s->vf = std::move(d);
+    }
+    VectorFieldDivergence(*s->vf, potential, ds.L[0].N, rhs, ds.L[0].N, ds.L[0].N, ncols, st);
+  })
 }
 
 int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int maxIters, int mode, double* relRes, int* iterations,

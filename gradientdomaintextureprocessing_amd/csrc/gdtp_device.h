@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "gdtp_galerkin.h"
+#include "gdtp_vectorfield.h"
 
 namespace gdtp {
 
@@ -89,5 +90,23 @@ void ApplyOperator(DeviceSystem& ds, DevLevel& op, const double* x, double* y, i
 // Squared 2-norms of K columns (stride ld) accumulated into dNorms[0..K) (zeroed first).
 void SquaredNorms(const double* v, int N, int K, int ld, double* dNorms, hipStream_t s);
 void ResidualLevel(DeviceSystem& ds, int level, const double* x, const double* b, double* r, int K, int ld, hipStream_t s);
+
+// Normalised-gradient integration samples on the device (cell samples SoA [13][n]).
+struct DevVectorField {
+  long nCell = 0;
+  int nGen = 0;
+  int* cellCorners = nullptr;  // 4 per sample
+  float* cellData = nullptr;   // SoA [13][nCell]
+  double invCellW = 0, invCellH = 0;
+  int *genPtr = nullptr, *genTexel = nullptr;
+  float *genTensor = nullptr, *genG = nullptr, *genV = nullptr;
+  std::vector<void*> allocations;
+  ~DevVectorField();
+};
+void UploadVectorField(const VectorFieldSamples& vs, DevVectorField& d);
+// rhs[:, t] = sum over samples of < -grad u / |grad u|_g , v(t) > for K columns of level-0 vectors
+// (IntergrateNormalizedVectorFieldFromSamples, VectorFieldIntergration.inl:452-582); rhs is
+// overwritten.
+void VectorFieldDivergence(const DevVectorField& d, const double* u, int ldu, double* rhs, int ldr, int N, int K, hipStream_t s);
 
 }  // namespace gdtp

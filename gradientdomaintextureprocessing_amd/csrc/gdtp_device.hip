@@ -347,6 +347,65 @@ DeviceSystem::~DeviceSystem() {
   for (void* p : allocations) (void)hipFree(p);
 }
 
+// ------------------------------------------------------------------ normalised-gradient integration
+// One thread per cell sample: the bilinear gradient at the sample from its 4 corners, normalised in
+// the metric, dotted with the 4 integrated basis gradients and added to the corners' rhs.
+__global__ void KVfCell(long n, const int* __restrict__ corners, const float* __restrict__ data, double invW, double invH,
+                        const double* __restrict__ u, int ldu, double* rhs, int ldr, int K) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int cn[4];
+  for (int k = 0; k < 4; k++) cn[k] = corners[4 * s + k];
+  const double x = data[s], y = data[n + s], t00 = data[2 * n + s], t01 = data[3 * n + s], t11 = data[4 * n + s];
+  double vx[4], vy[4];
+  for (int k = 0; k < 4; k++) vx[k] = data[(5 + 2 * k) * n + s], vy[k] = data[(6 + 2 * k) * n + s];
+  for (int ch = 0; ch < K; ch++) {
+    const double* U = u + (long)ch * ldu;
+    const double c0 = U[cn[0]], c1 = U[cn[1]], c2 = U[cn[2]], c3 = U[cn[3]];
+    // the bilinear gradient in difference form: exactly zero for a constant potential
+    const double nx = -((1 - y) * (c1 - c0) + y * (c2 - c3)) * invW;
+    const double ny = -((1 - x) * (c3 - c0) + x * (c2 - c1)) * invH;
+    const double ax = t00 * nx + t01 * ny, ay = t01 * nx + t11 * ny;
+    const double len2 = nx * ax + ny * ay;
+    if (!(len2 > 0)) continue;
+    const double il = 1.0 / sqrt(len2);
+    double* R = rhs + (long)ch * ldr;
+    for (int k = 0; k < 4; k++) atomicAdd(&R[cn[k]], (ax * vx[k] + ay * vy[k]) * il);
+  }
+}
+
+// One thread per general sample (quadratic seam elements): gradient from its texel entries.
+__global__ void KVfGeneral(int n, const int* __restrict__ ptr, const int* __restrict__ texel, const float* __restrict__ G,
+                           const float* __restrict__ V, const float* __restrict__ T, const double* __restrict__ u, int ldu, double* rhs,
+                           int ldr, int K) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int e0 = ptr[s], e1 = ptr[s + 1];
+  const double t00 = T[3 * s], t01 = T[3 * s + 1], t11 = T[3 * s + 2];
+  for (int ch = 0; ch < K; ch++) {
+    const double* U = u + (long)ch * ldu;
+    // the entries' G sum to zero: differences to the first entry's value keep a constant exact
+    const double u0 = U[texel[e0]];
+    double nx = 0, ny = 0;
+    for (int e = e0 + 1; e < e1; e++) {
+      const double du = U[texel[e]] - u0;
+      nx -= G[2 * e] * du, ny -= G[2 * e + 1] * du;
+    }
+    const double ax = t00 * nx + t01 * ny, ay = t01 * nx + t11 * ny;
+    const double len2 = nx * ax + ny * ay;
+    if (!(len2 > 0)) continue;
+    const double il = 1.0 / sqrt(len2);
+    double* R = rhs + (long)ch * ldr;
+    for (int e = e0; e < e1; e++) atomicAdd(&R[texel[e]], (ax * V[2 * e] + ay * V[2 * e + 1]) * il);
+  }
+}
+
+__global__ void KZeroStrided(int N, double* x, int ld, int K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N * K) return;
+  x[(t / N) * ld + t % N] = 0.0;
+}
+
 static thread_local size_t g_uploadBytes = 0;
 size_t UploadByteCounter() { return g_uploadBytes; }
 
@@ -650,6 +709,33 @@ void VCycleReference(DeviceSystem& ds, double* x, const double* b, int K, int ld
     ProlongAddLevel(ds, l, ds.L[l + 1].x, X(l), K, ds.L[l + 1].N, LD(l), s);
     RelaxLevel(ds, l, X(l), Bv(l), nullptr, K, LD(l), false, s);
   }
+}
+
+DevVectorField::~DevVectorField() {
+  for (void* p : allocations) (void)hipFree(p);
+}
+
+void UploadVectorField(const VectorFieldSamples& vs, DevVectorField& d) {
+  d.nCell = (long)vs.numCellSamples();
+  d.nGen = (int)vs.numGeneralSamples();
+  d.invCellW = vs.invCellW, d.invCellH = vs.invCellH;
+  std::vector<float> soa(13 * (size_t)d.nCell);
+  for (long s = 0; s < d.nCell; s++)
+    for (int f = 0; f < 13; f++) soa[(size_t)f * d.nCell + s] = vs.cellData[13 * (size_t)s + f];
+  d.cellCorners = UploadTo(d.allocations, vs.cellCorners);
+  d.cellData = UploadTo(d.allocations, soa);
+  d.genPtr = UploadTo(d.allocations, vs.genPtr), d.genTexel = UploadTo(d.allocations, vs.genTexel);
+  d.genTensor = UploadTo(d.allocations, vs.genTensor), d.genG = UploadTo(d.allocations, vs.genG), d.genV = UploadTo(d.allocations, vs.genV);
+}
+
+void VectorFieldDivergence(const DevVectorField& d, const double* u, int ldu, double* rhs, int ldr, int N, int K, hipStream_t s) {
+  KZeroStrided<<<(unsigned)(((long)N * K + 255) / 256), 256, 0, s>>>(N, rhs, ldr, K);
+  if (d.nCell > 0)
+    KVfCell<<<(unsigned)((d.nCell + 255) / 256), 256, 0, s>>>(d.nCell, d.cellCorners, d.cellData, d.invCellW, d.invCellH, u, ldu, rhs, ldr,
+                                                              K);
+  if (d.nGen > 0)
+    KVfGeneral<<<(unsigned)((d.nGen + 255) / 256), 256, 0, s>>>(d.nGen, d.genPtr, d.genTexel, d.genG, d.genV, d.genTensor, u, ldu, rhs,
+                                                                ldr, K);
 }
 
 }  // namespace gdtp

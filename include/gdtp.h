@@ -94,6 +94,13 @@ int gdtp_apply(gdtp_system* s, int which, const double* x, double* y, int ncols,
 /* r = b - A x on level 0 (ComputeSystemResdiual, :953-1007). */
 int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, int ncols, void* stream);
 
+/* Normalised-gradient divergence of the heat-method geodesics: for K potentials u (level-0
+ * columns), rhs[:, t] = integral < -grad u / |grad u|_g , grad phi_t >_g over the surface, one
+ * sample per (mesh triangle ∩ cell) piece (quadratic elements on pieces touching a seam).
+ * Replaces IntergrateNormalizedVectorFieldFromSamples (VectorFieldIntergration.inl:452-582) with
+ * the boundary prolongation / restriction around it (Geodesics.cpp:522-535).  rhs is overwritten. */
+int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rhs, int ncols, void* stream);
+
 /* ---- solves to a relative residual (every column: ||b - A x|| <= tol ||b||)
  * mode GDTP_SOLVE_FP64_REFERENCE: reference V-cycles with an FP64 residual check per cycle.
  * mode GDTP_SOLVE_MIXED: FP64 iterative refinement around FP32 V-cycles (four-colour deep

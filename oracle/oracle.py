@@ -237,3 +237,58 @@ def vcycle(levels, solvers, x, b):
         prolong_add(levels[l], levels[l + 1], X[l + 1], X[l])
         relaxation(levels[l], solvers.boundary[l], X[l], Bv[l], None)
     return x
+
+
+# ---------------------------------------------------------------------------- normalised gradient
+def vf_samples(H):
+    """The library's normalised-gradient integration samples (exported arrays, FP32 values)."""
+    cell = H.array(0, "vf.cellData", np.float64).reshape(-1, 13)
+    return dict(corners=H.array(0, "vf.cellCorners").reshape(-1, 4), pos=cell[:, 0:2], tensor=cell[:, 2:5],
+                v=cell[:, 5:13].reshape(-1, 4, 2), inv=H.array(0, "vf.invCell", np.float64),
+                gptr=H.array(0, "vf.genPtr"), gtex=H.array(0, "vf.genTexel"),
+                gten=H.array(0, "vf.genTensor", np.float64).reshape(-1, 3),
+                gG=H.array(0, "vf.genG", np.float64).reshape(-1, 2), gV=H.array(0, "vf.genV", np.float64).reshape(-1, 2))
+
+
+def _normalise(n, t):
+    """X = g^-1 n / sqrt(n^T g^-1 n), zero where the gradient vanishes
+    (VectorFieldIntergration.inl:508-518)."""
+    ax = t[:, 0] * n[:, 0] + t[:, 1] * n[:, 1]
+    ay = t[:, 1] * n[:, 0] + t[:, 2] * n[:, 1]
+    l2 = n[:, 0] * ax + n[:, 1] * ay
+    il = np.where(l2 > 0, 1.0 / np.sqrt(np.where(l2 > 0, l2, 1.0)), 0.0)
+    return ax * il, ay * il
+
+
+def vf_cell_gradients(S, u):
+    """Gradient (UV units) of the bilinear interpolant of the 4 cell corners at each cell sample
+    (VectorFieldIntergration.inl:500-503, corner order (0,0),(1,0),(1,1),(0,1)), written in the
+    algebraically identical difference form that is exactly zero for a constant."""
+    cv, (x, y) = u[S["corners"]], S["pos"].T
+    gx = ((1 - y) * (cv[:, 1] - cv[:, 0]) + y * (cv[:, 2] - cv[:, 3])) * S["inv"][0]
+    gy = ((1 - x) * (cv[:, 3] - cv[:, 0]) + x * (cv[:, 2] - cv[:, 1])) * S["inv"][1]
+    return np.stack([gx, gy], 1)
+
+
+def vf_divergence(S, u, N):
+    """rhs_t = sum over samples of < X, v(t) >, X the metric-normalised negative gradient of the
+    potential u at the sample (IntergrateNormalizedVectorFieldFromSamples,
+    VectorFieldIntergration.inl:452-582: bilinear cell gradient at the sample position, scaled by
+    the inverse cell size; the quadratic seam samples expand their node values into texel
+    weights).  u: [N] or [K, N] FP64."""
+    u = np.atleast_2d(u)
+    out = np.zeros_like(u)
+    c = S["corners"]
+    for k in range(u.shape[0]):
+        X, Y = _normalise(-vf_cell_gradients(S, u[k]), S["tensor"])
+        for j in range(4):
+            np.add.at(out[k], c[:, j], X * S["v"][:, j, 0] + Y * S["v"][:, j, 1])
+        # general samples
+        ns = len(S["gptr"]) - 1
+        if ns:
+            seg = np.repeat(np.arange(ns), np.diff(S["gptr"]))
+            ut = u[k][S["gtex"]] - u[k][S["gtex"][S["gptr"][:-1]]][seg]  # differences to the first entry
+            n = -np.stack([np.bincount(seg, S["gG"][:, 0] * ut, ns), np.bincount(seg, S["gG"][:, 1] * ut, ns)], 1)
+            X, Y = _normalise(n, S["gten"])
+            np.add.at(out[k], S["gtex"], X[seg] * S["gV"][:, 0] + Y[seg] * S["gV"][:, 1])
+    return out

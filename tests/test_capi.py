@@ -56,3 +56,9 @@ def test_geometry_errors_surface():
     with pytest.raises(_lib.GdtpError) as e:
         g.InitializeHierarchy(mesh, 64, 64, 2)
     assert e.value.code == 2
+
+
+def test_recipe_classes_are_exported():
+    import gradientdomaintextureprocessing_amd as g
+    for name in ["TextureFilter", "Geodesics", "InitializeHierarchy", "UpdateLinearSystem", "VCycle"]:
+        assert hasattr(g, name), name
