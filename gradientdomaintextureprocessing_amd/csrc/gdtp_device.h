@@ -29,6 +29,12 @@ struct DevLdlt {
   int *fwdLevPtr = nullptr, *fwdRows = nullptr, *bwdLevPtr = nullptr, *bwdRows = nullptr;
   int topFwdLevBegin = 0, topFwdLevEnd = 0, topBwdLevBegin = 0, topBwdLevEnd = 0;
   int maxTeamRows = 0;
+  // dense top (when the top has at most kDenseTopMax rows): the top rows (permuted indices), their
+  // coupling to team rows (CSR over the top rows), the inverse of the unit-lower top block and its
+  // transpose (row-major s x s), and a [kcap][s] scratch vector
+  int nTop = 0, kcap = 0;
+  int *topRows = nullptr, *topCplRowptr = nullptr, *topCplCol = nullptr;
+  double *topCplVal = nullptr, *topInv = nullptr, *topInvT = nullptr, *topBuf = nullptr;
 };
 
 struct DevCsr {
@@ -66,7 +72,7 @@ struct DeviceSystem {
 
 // Upload an LDL^T factor with its team / top level schedule; LdltSolve solves for K columns,
 // gathering src[perm] and scattering into dst (through `map` when not null).
-void UploadLdlt(std::vector<void*>& allocations, const Ldlt& F, DevLdlt& d);
+void UploadLdlt(std::vector<void*>& allocations, const Ldlt& F, DevLdlt& d, int kcap);
 // Bytes uploaded / allocated by this thread's device uploads so far (monotone counter).
 size_t UploadByteCounter();
 void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, const int* map, double* y, int K, hipStream_t s);

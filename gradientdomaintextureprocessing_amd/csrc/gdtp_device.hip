@@ -10,6 +10,7 @@ namespace gdtp {
 
 static constexpr int kWave = 64;
 static constexpr int kKc = 4;  // channels processed together in registers
+static constexpr int kDenseTopMax = 4096;  // largest LDL^T top solved through a dense inverse
 
 // ------------------------------------------------------------------ band Gauss-Seidel
 // The reference smoother (IterativeSolvers.inl:398-558 with residual, :174-395 without):
@@ -323,11 +324,15 @@ __global__ void KApplyBoundary(int B, const int* __restrict__ bgi, const int* __
   y[(size_t)c * ld + bgi[i]] = b ? b[(size_t)c * ld + bgi[i]] - v : v;
 }
 
+// Squared 2-norm of column blockIdx.y: a bounded grid strides over the column so that only a
+// few hundred blocks add into the column's accumulator.
 __global__ __launch_bounds__(256) void KSquaredNorm(int N, const double* __restrict__ v, int ld, double* out) {
   const int c = blockIdx.y;
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double a = i < N ? v[(long)c * ld + i] : 0.0;
-  a *= a;
+  double a = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const double t = v[(long)c * ld + i];
+    a += t * t;
+  }
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
   __shared__ double part[4];
   if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = a;
@@ -406,6 +411,42 @@ __global__ void KZeroStrided(int N, double* x, int ld, int K) {
   x[(t / N) * ld + t % N] = 0.0;
 }
 
+// ------------------------------------------------------------------ dense top of an LDL^T solve
+// buf[c][a] = y[c][top[a]] - sum over team columns of L[top[a]][i] y[c][i]   (one wave per top row)
+__global__ __launch_bounds__(256) void KTopCouple(int st, const int* __restrict__ top, const int* __restrict__ rp,
+                                                  const int* __restrict__ cl, const double* __restrict__ vl, const double* __restrict__ y,
+                                                  int n, double* buf, int K) {
+  const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (a >= st) return;
+  for (int c = 0; c < K; c++) {
+    const double* Y = y + (long)c * n;
+    double s = 0.0;
+    for (int p = rp[a] + lane; p < rp[a + 1]; p += kWave) s += vl[p] * Y[cl[p]];
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if (lane == 0) buf[(long)c * st + a] = Y[top[a]] - s;
+  }
+}
+// buf[c][a] = y[c][top[a]]
+__global__ void KTopGather(int st, const int* __restrict__ top, const double* __restrict__ y, int n, double* buf, int K) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)st * K) return;
+  buf[t] = y[(t / st) * n + top[t % st]];
+}
+// y[c][top[a]] = sum_b M[a][b] buf[c][b]   (one wave per row; M row-major st x st)
+__global__ __launch_bounds__(256) void KTopDense(int st, const int* __restrict__ top, const double* __restrict__ M,
+                                                 const double* __restrict__ buf, double* y, int n, int K) {
+  const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (a >= st) return;
+  const double* row = M + (long)a * st;
+  for (int c = 0; c < K; c++) {
+    const double* in = buf + (long)c * st;
+    double s = 0.0;
+    for (int b = lane; b < st; b += kWave) s += row[b] * in[b];
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if (lane == 0) y[(long)c * n + top[a]] = s;
+  }
+}
+
 static thread_local size_t g_uploadBytes = 0;
 size_t UploadByteCounter() { return g_uploadBytes; }
 
@@ -453,7 +494,7 @@ static std::vector<double> SoA10(const std::vector<double>& aos) {
 
 // Partition the elimination tree into subtree teams (at most `smax` rows) plus the top part, and
 // group rows by dependency level inside each team and inside the top.
-void UploadLdlt(std::vector<void*>& ds, const Ldlt& F, DevLdlt& d) {
+void UploadLdlt(std::vector<void*>& ds, const Ldlt& F, DevLdlt& d, int kcap) {
   const int n = F.n;
   d.n = n;
   d.perm = UploadTo(ds, F.perm);
@@ -530,6 +571,43 @@ void UploadLdlt(std::vector<void*>& ds, const Ldlt& F, DevLdlt& d) {
   d.nTeams = nTeams;
   d.fwdLevPtr = UploadTo(ds, fLevPtr), d.fwdRows = UploadTo(ds, fRows), d.teamFwdLev = UploadTo(ds, fTeamLev);
   d.bwdLevPtr = UploadTo(ds, bLevPtr), d.bwdRows = UploadTo(ds, bRows), d.teamBwdLev = UploadTo(ds, bTeamLev);
+  // dense top: the top rows form a unit-lower block L_TT (separators after fill, nearly dense);
+  // its inverse turns the sequential top solve into two dense matrix-vector products
+  std::vector<int> top, local(n, -1);
+  for (int k = 0; k < n; k++)
+    if (team[k] < 0) local[k] = (int)top.size(), top.push_back(k);
+  const int st = (int)top.size();
+  if (st == 0 || st > kDenseTopMax || kcap < 1) return;
+  std::vector<int> crp(1, 0), ccl;
+  std::vector<double> cvl, Ltt((size_t)st * st, 0.0);
+  for (int a = 0; a < st; a++) {
+    const int k = top[a];
+    Ltt[(size_t)a * st + a] = 1.0;
+    for (int p = F.Lrows.rowptr[k]; p < F.Lrows.rowptr[k + 1]; p++) {
+      const int i = F.Lrows.col[p];
+      if (local[i] >= 0) Ltt[(size_t)a * st + local[i]] = F.Lrows.val[p];
+      else ccl.push_back(i), cvl.push_back(F.Lrows.val[p]);
+    }
+    crp.push_back((int)ccl.size());
+  }
+  std::vector<double> inv((size_t)st * st, 0.0), invT((size_t)st * st, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < st; j++) {  // column j of the inverse: forward substitution from e_j
+    std::vector<double> z(st, 0.0);
+    z[j] = 1.0;
+    for (int a = j + 1; a < st; a++) {
+      double v = 0.0;
+      const double* row = &Ltt[(size_t)a * st];
+      for (int b = j; b < a; b++) v -= row[b] * z[b];
+      z[a] = v;
+    }
+    for (int a = j; a < st; a++) inv[(size_t)a * st + j] = z[a], invT[(size_t)j * st + a] = z[a];
+  }
+  d.nTop = st, d.kcap = kcap;
+  d.topRows = UploadTo(ds, top);
+  d.topCplRowptr = UploadTo(ds, crp), d.topCplCol = UploadTo(ds, ccl), d.topCplVal = UploadTo(ds, cvl);
+  d.topInv = UploadTo(ds, inv), d.topInvT = UploadTo(ds, invT);
+  d.topBuf = UploadTo(ds, std::vector<double>((size_t)st * kcap, 0.0));
 }
 
 void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOperator& op, int kcap, DeviceSystem& ds) {
@@ -570,8 +648,8 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     for (auto& x : H.prolongationLines) rl.insert(rl.end(), {x.start, x.length, x.center, x.prev, x.next, x.aligned});
     d.nProl = (int)H.prolongationLines.size();
     d.prol = Upload(ds, rl);
-    if (l + 1 < nl) UploadLdlt(ds.allocations, oh.boundaryFactor[l], d.factor);
-    else UploadLdlt(ds.allocations, oh.coarseFactor, d.factor);
+    if (l + 1 < nl) UploadLdlt(ds.allocations, oh.boundaryFactor[l], d.factor, kcap);
+    else UploadLdlt(ds.allocations, oh.coarseFactor, d.factor, kcap);
     d.x = Alloc(ds, (size_t)d.N * kcap);
     d.b = Alloc(ds, (size_t)d.N * kcap);
     d.r = Alloc(ds, (size_t)d.N * kcap);
@@ -595,10 +673,18 @@ void LdltSolve(DevLdlt& F, const double* src, int lds, double* dst, int ldd, con
   const long nk = (long)n * K;
   KLdltGather<<<Blocks(nk, 256), 256, 0, s>>>(n, F.perm, src, lds, y, K);
   if (F.nTeams) KLdltTeams<<<F.nTeams, 256, 0, s>>>(F.teamFwdLev, F.fwdLevPtr, F.fwdRows, F.lrRowptr, F.lrCol, F.lrVal, y, n, K);
-  if (F.topFwdLevEnd > F.topFwdLevBegin)
+  const bool dense = F.nTop > 0 && K <= F.kcap;
+  const unsigned topBlocks = Blocks(F.nTop, 256 / kWave);
+  if (dense) {
+    KTopCouple<<<topBlocks, 256, 0, s>>>(F.nTop, F.topRows, F.topCplRowptr, F.topCplCol, F.topCplVal, y, n, F.topBuf, K);
+    KTopDense<<<topBlocks, 256, 0, s>>>(F.nTop, F.topRows, F.topInv, F.topBuf, y, n, K);
+  } else if (F.topFwdLevEnd > F.topFwdLevBegin)
     KLdltTop<<<1, 1024, 0, s>>>(F.topFwdLevBegin, F.topFwdLevEnd, F.fwdLevPtr, F.fwdRows, F.lrRowptr, F.lrCol, F.lrVal, y, n, K);
   KLdltDiag<<<Blocks(nk, 256), 256, 0, s>>>(n, F.dinv, y, K);
-  if (F.topBwdLevEnd > F.topBwdLevBegin)
+  if (dense) {
+    KTopGather<<<Blocks((long)F.nTop * K, 256), 256, 0, s>>>(F.nTop, F.topRows, y, n, F.topBuf, K);
+    KTopDense<<<topBlocks, 256, 0, s>>>(F.nTop, F.topRows, F.topInvT, F.topBuf, y, n, K);
+  } else if (F.topBwdLevEnd > F.topBwdLevBegin)
     KLdltTop<<<1, 1024, 0, s>>>(F.topBwdLevBegin, F.topBwdLevEnd, F.bwdLevPtr, F.bwdRows, F.lcRowptr, F.lcCol, F.lcVal, y, n, K);
   if (F.nTeams) KLdltTeams<<<F.nTeams, 256, 0, s>>>(F.teamBwdLev, F.bwdLevPtr, F.bwdRows, F.lcRowptr, F.lcCol, F.lcVal, y, n, K);
   KLdltScatter<<<Blocks(nk, 256), 256, 0, s>>>(n, F.perm, map, y, dst, ldd, K);
@@ -677,7 +763,7 @@ void ApplyOperator(DeviceSystem& ds, DevLevel& d, const double* x, double* y, in
 
 void SquaredNorms(const double* v, int N, int K, int ld, double* dNorms, hipStream_t s) {
   KZero<<<1, 64, 0, s>>>(K, dNorms);
-  KSquaredNorm<<<dim3(Blocks(N, 256), K), 256, 0, s>>>(N, v, ld, dNorms);
+  KSquaredNorm<<<dim3(std::min(Blocks(N, 256), 512u), K), 256, 0, s>>>(N, v, ld, dNorms);
 }
 
 void ResidualLevel(DeviceSystem& ds, int l, const double* x, const double* b, double* r, int K, int ld, hipStream_t s) {

@@ -91,63 +91,86 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
 }
 
 // ------------------------------------------------------------------ boundary rows
-// Boundary-row residual into the compact [K][B] layout.
-__global__ void KMixBoundaryResidualCompact(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                            const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
-                                            const float* __restrict__ rhs, float* resB, long Nd, int K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)B * K) return;
-  const int r = t % B, ch = t / B;
-  const float* E = e + (long)ch * Nd;
-  const long d = bdIdx[r];
-  float s = dg[r] * E[d];
-  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
-  resB[(long)ch * B + r] = rhs[(long)ch * Nd + d] - s;
+// Boundary rows are sparse (about 16 entries); a group of kRowLanes lanes works on one row so that
+// the row's entries are read coalesced, once for up to four columns, and summed by shuffles.
+static constexpr int kRowLanes = 16;
+template <class T>
+__device__ __forceinline__ T RowSum(T v) {
+  for (int o = kRowLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kRowLanes);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T Pick(const T (&a)[4], int q) {
+  return q == 0 ? a[0] : q == 1 ? a[1] : q == 2 ? a[2] : a[3];
 }
 
-// Restricted additive Schwarz over the boundary rows: one wave per block; thread i owns core
-// row i (< 64) and accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time,
-// the block's residual staged in LDS.  Cores of different blocks are disjoint.
+// Boundary-row residual rhs - A e (FP32).  out: compact [K][B] when `compact`, else the dense
+// vector [K][Nd] at the row's texel.
+__global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                            const int* __restrict__ cl, const float* __restrict__ vl,
+                                                            const float* __restrict__ dg, const float* __restrict__ e,
+                                                            const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  if (r >= B) return;  // whole row groups leave together
+  const int k0 = rp[r], k1 = rp[r + 1];
+  const long d = bdIdx[r];
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = k0 + lane; k < k1; k += kRowLanes) {
+      const float v = vl[k];
+      const long c = cl[k];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < kc) acc[q] += v * e[(long)(c0 + q) * Nd + c];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = RowSum(acc[q]);
+    if (lane < kc) {
+      const long o = (long)(c0 + lane) * Nd + d;
+      const float rv = rhs[o] - (dg[r] * e[o] + Pick(acc, lane));
+      if (compact) out[(long)(c0 + lane) * B + r] = rv;
+      else out[o] = rv;
+    }
+  }
+}
+
+// Restricted additive Schwarz over the boundary rows: one workgroup per block, its four waves
+// splitting the extended rows; lane i of every wave owns core row i (< 64) and accumulates
+// sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual staged in
+// LDS.  Cores of different blocks are disjoint.
 static constexpr int kBlockCore = 64, kBlockExtMax = 384;
-__global__ __launch_bounds__(64) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
-                                                  const int* __restrict__ extPtr, const int* __restrict__ ext,
-                                                  const long* __restrict__ matPtr, const float* __restrict__ mat,
-                                                  const float* __restrict__ resB, int B, float* e, long Nd, int K, float omega) {
+__global__ __launch_bounds__(256) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
+                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
+                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
+                                                   const float* __restrict__ resB, int B, float* e, long Nd, int K, float omega) {
   __shared__ float R[4 * kBlockExtMax];
+  __shared__ float P[4][4][kBlockCore];
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
   const float* M = mat + matPtr[b];
-  const int i = threadIdx.x;
+  const int i = threadIdx.x % 64, part = threadIdx.x / 64;
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
-    for (int j = i; j < nx; j += 64)
+    for (int j = threadIdx.x; j < nx; j += 256)
       for (int q = 0; q < kc; q++) R[q * kBlockExtMax + j] = resB[(long)(ch0 + q) * B + ext[x0 + j]];
     __syncthreads();
-    if (i < nc) {
-      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int j = 0; j < nx; j++) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (i < nc)
+      for (int j = part; j < nx; j += 4) {
         const float m = M[(long)j * nc + i];
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
       }
+#pragma unroll
+    for (int q = 0; q < 4; q++) P[part][q][i] = acc[q];
+    __syncthreads();
+    if (part == 0 && i < nc) {
       const long d = core[c0 + i];
-      for (int q = 0; q < kc; q++) e[(long)(ch0 + q) * Nd + d] += omega * acc[q];
+      for (int q = 0; q < kc; q++) e[(long)(ch0 + q) * Nd + d] += omega * (P[0][q][i] + P[1][q][i] + P[2][q][i] + P[3][q][i]);
     }
     __syncthreads();
   }
-}
-
-__global__ void KMixResidualBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                     const float* __restrict__ vl, const float* __restrict__ dg, const float* __restrict__ e,
-                                     const float* __restrict__ rhs, float* res, long Nd, int K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)B * K) return;
-  const int r = t % B, ch = t / B;
-  const float* E = e + (long)ch * Nd;
-  const long d = bdIdx[r];
-  float s = dg[r] * E[d];
-  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * E[cl[k]];
-  res[(long)ch * Nd + d] = rhs[(long)ch * Nd + d] - s;
 }
 
 // ------------------------------------------------------------------ transfers
@@ -322,22 +345,41 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
                                                            const int* __restrict__ rpb, const int* __restrict__ colb,
                                                            const double* __restrict__ valb, const double* __restrict__ x,
                                                            const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                           float* e0, long Nd, double* norms) {
-  const int ch = blockIdx.y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (i < B) {
-    const double* X = x + (long)ch * N;
-    double s = 0.0;
-    for (int k = rpb[i]; k < rpb[i + 1]; k++) s += X[bgi[colb[k]]] * valb[k];
-    for (int k = rpd[i]; k < rpd[i + 1]; k++) s += X[cold[k]] * vald[k];
-    const int g = bgi[i];
-    const double r = b[(long)ch * N + g] - s;
-    rhs0[(long)ch * Nd + g2d[g]] = (float)r;
-    e0[(long)ch * Nd + g2d[g]] = 0.0f;
-    acc = r * r;
+                                                           float* e0, long Nd, double* norms, int K) {
+  // kRowLanes lanes per boundary row (no early exit: every thread takes part in the block sums)
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  const bool active = r < B;
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+      for (int k = rpb[r] + lane; k < rpb[r + 1]; k += kRowLanes) {
+        const double v = valb[k];
+        const int g = bgi[colb[k]];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q < kc) acc[q] += v * x[(long)(c0 + q) * N + g];
+      }
+      for (int k = rpd[r] + lane; k < rpd[r + 1]; k += kRowLanes) {
+        const double v = vald[k];
+        const int g = cold[k];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q < kc) acc[q] += v * x[(long)(c0 + q) * N + g];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = RowSum(acc[q]);
+    double sq = 0.0;
+    if (active && lane < kc) {
+      const int g = bgi[r];
+      const double rv = b[(long)(c0 + lane) * N + g] - Pick(acc, lane);
+      rhs0[(long)(c0 + lane) * Nd + g2d[g]] = (float)rv;
+      e0[(long)(c0 + lane) * Nd + g2d[g]] = 0.0f;
+      sq = rv * rv;
+    }
+    for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
   }
-  BlockAddSquares(acc, norms + (long)ch * kSlots);
 }
 
 __global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out) {
@@ -614,7 +656,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
   }
   const size_t before = UploadByteCounter();
-  UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse);
+  UploadLdlt(ms.allocations, oh.coarseFactor, ms.coarse, kcap);
   ms.bytes += UploadByteCounter() - before;
   const int Nc = ms.L[nl - 1].N;
   if (Nc <= kDenseCoarseMax) {  // dense inverse of the coarsest operator, by columns of the LDL^T solve
@@ -639,9 +681,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s) {
   if (d.B == 0 || d.nBlocks == 0) return;
   for (int sw = 0; sw < ms.boundarySweeps; sw++) {
-    KMixBoundaryResidualCompact<<<NBlk((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB,
-                                                                         d.Nd, K);
-    KMixSchwarz<<<d.nBlocks, 64, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
+    KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB,
+                                                                          1, d.Nd, K);
+    KMixSchwarz<<<d.nBlocks, 256, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
                                          ms.omega);
   }
 }
@@ -676,8 +718,8 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
     if (d.nTiles > 0)
       KMixResidualDeep<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
     if (d.B > 0)
-      KMixResidualBoundary<<<NBlk((long)d.B * K, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
-                                                                     d.Nd, K);
+      KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
+                                                                            0, d.Nd, K);
     if (c.nTiles > 0)
       KMixRestrictDeep<<<dim3(c.nTiles, 4), 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
                                                          K);
@@ -713,8 +755,8 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   if (L0.nRaster > 0)
     K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
   if (L0.B > 0)
-    K64ResidualBoundary<<<dim3(NBlk(L0.B, 256), K), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr, L0.bb.col,
-                                                                 L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms);
+    K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
+                                                                          L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
 }
 
 static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
