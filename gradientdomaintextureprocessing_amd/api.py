@@ -185,7 +185,7 @@ class MultigridSystem:
                                           ctypes.byref(it), _stream()))
         return it.value, rel
 
-    def configure_mixed(self, nu_pre=2, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True):
+    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True):
         _lib.check(_lib.load().gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
 
     def mixed_deep_sweep(self, level, ncols):

@@ -44,7 +44,7 @@ struct gdtp_system {
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
-  int nuPre = 2, nuPost = 2, boundarySweeps = 1, useGraph = 1;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
@@ -407,6 +407,7 @@ int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySwee
   if (!s || nuPre < 0 || nuPost < 0 || nuPre + nuPost == 0 || boundarySweeps < 0 || !(omega > 0 && omega <= 1.5))
     return Fail(GDTP_ERROR_INVALID, "invalid mixed-solver configuration");
   s->nuPre = nuPre, s->nuPost = nuPost, s->boundarySweeps = boundarySweeps, s->omega = omega, s->useGraph = useGraph ? 1 : 0;
+  if (s->mixed) InvalidateGraph(*s->mixed);  // the captured step holds the previous schedule
   return GDTP_OK;
 }
 

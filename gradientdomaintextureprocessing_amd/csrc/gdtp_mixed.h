@@ -40,7 +40,7 @@ struct DevMixLevel {
 
 struct MixedSystem {
   int levels = 0, kcap = 0;
-  int nuPre = 2, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per smoothing step
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per smoothing step
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level
@@ -65,6 +65,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 // run; relRes (host, K entries) receives the final relative residuals.
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
                hipStream_t s, bool useGraph);
+
+// Drop the captured refinement step (after the smoothing configuration changes).
+void InvalidateGraph(MixedSystem& ms);
 
 // One FP32 V-cycle of the correction equation on the current level-0 residual (exposed for profiling).
 void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s);

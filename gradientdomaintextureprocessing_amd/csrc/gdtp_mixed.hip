@@ -135,28 +135,29 @@ __global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __
   }
 }
 
-// Restricted additive Schwarz over the boundary rows: one workgroup per block, its four waves
-// splitting the extended rows; lane i of every wave owns core row i (< 64) and accumulates
-// sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual staged in
-// LDS.  Cores of different blocks are disjoint.
-static constexpr int kBlockCore = 64, kBlockExtMax = 384;
-__global__ __launch_bounds__(256) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
-                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
-                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
-                                                   const float* __restrict__ resB, int B, float* e, long Nd, int K, float omega) {
+// Restricted additive Schwarz over the boundary rows: one workgroup of kSchwarzWaves waves per
+// block, the waves splitting the extended rows; lane i of every wave owns core row i (< 64) and
+// accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual
+// staged in LDS.  Cores of different blocks are disjoint.
+static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
+__global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
+                                                                  const int* __restrict__ extPtr, const int* __restrict__ ext,
+                                                                  const long* __restrict__ matPtr, const float* __restrict__ mat,
+                                                                  const float* __restrict__ resB, int B, float* e, long Nd, int K,
+                                                                  float omega) {
   __shared__ float R[4 * kBlockExtMax];
-  __shared__ float P[4][4][kBlockCore];
+  __shared__ float P[kSchwarzWaves][4][kBlockCore];
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
   const float* M = mat + matPtr[b];
   const int i = threadIdx.x % 64, part = threadIdx.x / 64;
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
-    for (int j = threadIdx.x; j < nx; j += 256)
+    for (int j = threadIdx.x; j < nx; j += 64 * kSchwarzWaves)
       for (int q = 0; q < kc; q++) R[q * kBlockExtMax + j] = resB[(long)(ch0 + q) * B + ext[x0 + j]];
     __syncthreads();
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (i < nc)
-      for (int j = part; j < nx; j += 4) {
+      for (int j = part; j < nx; j += kSchwarzWaves) {
         const float m = M[(long)j * nc + i];
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -165,9 +166,13 @@ __global__ __launch_bounds__(256) void KMixSchwarz(const int* __restrict__ coreP
 #pragma unroll
     for (int q = 0; q < 4; q++) P[part][q][i] = acc[q];
     __syncthreads();
-    if (part == 0 && i < nc) {
-      const long d = core[c0 + i];
-      for (int q = 0; q < kc; q++) e[(long)(ch0 + q) * Nd + d] += omega * (P[0][q][i] + P[1][q][i] + P[2][q][i] + P[3][q][i]);
+    if (threadIdx.x < 4 * 64) {  // wave q sums column q over the parts
+      const int q = threadIdx.x / 64;
+      if (q < kc && i < nc) {
+        float v = 0.0f;
+        for (int p = 0; p < kSchwarzWaves; p++) v += P[p][q][i];
+        e[(long)(ch0 + q) * Nd + core[c0 + i]] += omega * v;
+      }
     }
     __syncthreads();
   }
@@ -549,7 +554,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     long base = 0;
     for (int c = 0; c < d.nCharts; c++) {
       const GridChart& g = H.charts[c];
-      MixChart m{g.width, g.height, (g.width + 1) / 2, (g.height + 1) / 2, (int)base, g.centerOffset[0], g.centerOffset[1]};
+      // plane rows padded to 32 floats (128 B): every 64-wide tile row is two whole cache lines
+      const int w2 = ((g.width + 1) / 2 + 31) / 32 * 32;
+      MixChart m{g.width, g.height, w2, (g.height + 1) / 2, (int)base, g.centerOffset[0], g.centerOffset[1]};
       d.hcharts[c] = m;
       base += (long)m.w2 * m.h2;
     }
@@ -683,7 +690,7 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
   for (int sw = 0; sw < ms.boundarySweeps; sw++) {
     KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB,
                                                                           1, d.Nd, K);
-    KMixSchwarz<<<d.nBlocks, 256, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
+    KMixSchwarz<<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
                                          ms.omega);
   }
 }
@@ -702,6 +709,12 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
       }
     if (post) BoundarySweeps(ms, d, K, s);
   }
+}
+
+void InvalidateGraph(MixedSystem& ms) {
+  if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec);
+  if (ms.graph) (void)hipGraphDestroy(ms.graph);
+  ms.graphExec = nullptr, ms.graph = nullptr, ms.graphK = 0, ms.graphX = nullptr, ms.graphB = nullptr;
 }
 
 void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
