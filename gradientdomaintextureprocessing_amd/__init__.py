@@ -5,10 +5,11 @@ include/gdtp.h); this package is its Python binding and host-side call surface.
 """
 from .api import (METRIC_EMBEDDING, METRIC_UNIFORM, OPERATOR_MASS, OPERATOR_STIFFNESS, OPERATOR_SYSTEM, AddProlongation,
                   Geodesics, HierarchicalSystem, InitializeHierarchy, MultigridSystem, MultiplyByRestriction,
-                  MultiplyBySystemMatrix_NoReciprocals, Relaxation, RelaxationAndResidual, TextureFilter, TexturedMesh,
-                  UpdateLinearSystem, VCycle)
+                  LineIntegralConvolution, MultiplyBySystemMatrix_NoReciprocals, Relaxation, RelaxationAndResidual, TextureFilter,
+                  TexturedMesh, UpdateLinearSystem, VCycle, embedding_metric, surface_gradient_field, vector_field_metric)
 
 __all__ = ["TexturedMesh", "HierarchicalSystem", "MultigridSystem", "InitializeHierarchy", "UpdateLinearSystem", "VCycle",
            "Relaxation", "RelaxationAndResidual", "MultiplyByRestriction", "AddProlongation",
            "MultiplyBySystemMatrix_NoReciprocals", "OPERATOR_SYSTEM", "OPERATOR_MASS", "OPERATOR_STIFFNESS",
-           "METRIC_EMBEDDING", "METRIC_UNIFORM", "TextureFilter", "Geodesics"]
+           "METRIC_EMBEDDING", "METRIC_UNIFORM", "TextureFilter", "Geodesics",
+           "LineIntegralConvolution", "embedding_metric", "vector_field_metric", "surface_gradient_field"]

@@ -21,6 +21,7 @@ SIGNATURES = {
     "gdtp_mesh_read": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "gdtp_mesh_destroy": (None, [c_void_p]),
     "gdtp_hierarchy_create": (c_int, [c_void_p, c_int, c_int, c_int, c_int, P(c_void_p)]),
+    "gdtp_hierarchy_create_with_metric": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, P(c_void_p)]),
     "gdtp_hierarchy_level_info": (c_int, [c_void_p, c_int, c_void_p]),
     "gdtp_hierarchy_array_i32": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),
     "gdtp_hierarchy_array_f64": (c_int, [c_void_p, c_int, c_char_p, c_void_p, P(c_size_t)]),

@@ -117,12 +117,54 @@ class HierarchicalSystem:
             self._h = None
 
 
-def InitializeHierarchy(mesh, width, height, levels=4, metric=METRIC_EMBEDDING):
+def InitializeHierarchy(mesh, width, height, levels=4, metric=METRIC_EMBEDDING, triangle_metric=None):
     """InitializeHierarchy + InitializeMetric + InitializeMassAndStiffness
-    (HierarchyConstruction.inl:799-849, Metric.inl:448-466, MassAndStiffness.h:490-549)."""
+    (HierarchyConstruction.inl:799-849, Metric.inl:448-466, MassAndStiffness.h:490-549).
+    triangle_metric ([T, 2, 2], the surface metric in each triangle's edge basis) replaces the
+    embedding / uniform metric (Metric.inl:468-486)."""
     h = ctypes.c_void_p()
-    _lib.check(_lib.load().gdtp_hierarchy_create(mesh._h, width, height, levels, metric, ctypes.byref(h)))
+    if triangle_metric is None:
+        _lib.check(_lib.load().gdtp_hierarchy_create(mesh._h, width, height, levels, metric, ctypes.byref(h)))
+    else:
+        g = np.asarray(triangle_metric, dtype=np.float64)
+        m = np.ascontiguousarray(np.stack([g[:, 0, 0], 0.5 * (g[:, 0, 1] + g[:, 1, 0]), g[:, 1, 1]], axis=1))
+        _lib.check(_lib.load().gdtp_hierarchy_create_with_metric(mesh._h, width, height, levels, m.ctypes.data, ctypes.byref(h)))
     return HierarchicalSystem(h, levels, width, height)
+
+
+def embedding_metric(vertices, triangles, normalize_area=True):
+    """Per-triangle Gram matrix of the edges (p1 - p0, p2 - p0), scaled to unit total area
+    (InitializeEmbeddingMetric, Metric.inl:248-276).  Returns [T, 2, 2]."""
+    P = np.asarray(vertices, np.float64)[np.asarray(triangles)]
+    d = np.stack([P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]], axis=1)
+    g = np.einsum("tkc,tlc->tkl", d, d)
+    if normalize_area:
+        g /= np.sum(np.sqrt(np.linalg.det(g)) / 2)
+    return g
+
+
+def vector_field_metric(g, vf, normalize_area=True, along=0.0, across=1e-1, floor=1e-5):
+    """Anisotropic metric of a tangent field (InitializeVectorFieldMetric, Metric.inl:12-67):
+    with v the field (per triangle, edge-basis coordinates) and w its g-orthogonal complement,
+    both g-normalised and B = [v w], the metric is  g B diag(along, across) B^-1 + floor * g,
+    zero-field triangles get floor * g; scaled to unit total area."""
+    g = np.asarray(g, np.float64)
+    v = np.asarray(vf, np.float64)
+    out = floor * g.copy()
+    gv = np.einsum("tkl,tl->tk", g, v)
+    nrm = np.einsum("tk,tk->t", v, gv)
+    ok = nrm > 0
+    w = np.stack([-gv[:, 1], gv[:, 0]], axis=1)
+    v0 = v / np.sqrt(np.where(ok, nrm, 1.0))[:, None]
+    nw = np.einsum("tk,tkl,tl->t", w, g, w)
+    v1 = w / np.sqrt(np.where(nw > 0, nw, 1.0))[:, None]
+    B = np.stack([v0, v1], axis=2)  # columns v0, v1
+    D = np.diag([along, across])
+    u = np.einsum("tkl,tlm,mn,tnp->tkp", g, B, D, np.linalg.inv(np.where(ok[:, None, None], B, np.eye(2))))
+    out[ok] += u[ok]
+    if normalize_area:
+        out /= np.sum(np.sqrt(np.linalg.det(out)) / 2)
+    return out
 
 
 class MultigridSystem:
@@ -305,3 +347,46 @@ class Geodesics:
         self.distance.solve(phi, div, tol=tol, max_iters=max_iters, mode=mode)
         phi -= phi[torch.arange(K, device="cuda"), src][:, None]
         return phi
+
+
+def surface_gradient_field(vertices, triangles, f):
+    """Per-triangle gradient of the per-vertex scalar f, as contravariant components in the edge
+    basis (p1 - p0, p2 - p0): g^-1 (f1 - f0, f2 - f0) with g the edge Gram matrix."""
+    t = np.asarray(triangles)
+    g = embedding_metric(vertices, t, normalize_area=False)
+    f = np.asarray(f, np.float64)[t]
+    return np.linalg.solve(g, np.stack([f[:, 1] - f[:, 0], f[:, 2] - f[:, 0]], axis=1)[..., None])[..., 0]
+
+
+class LineIntegralConvolution:
+    """Line integral convolution as anisotropic diffusion (LineIntegralConvolution.cpp:503-527,
+    560-710), on two hierarchies over the same atlas:
+        (M_a + l S_a) x = M_a noise        with the vector-field metric (vector_field_metric),
+        (M + a S) y = M x + s a S x        with the embedding metric (the sharpening pass),
+    defaults as the reference's: l = 1e-4 (licStiffness), a = 1e-4 (stiffnessWeight),
+    s = 100 (sharpnessWeight).  vector_field: per mesh triangle, edge-basis components."""
+
+    def __init__(self, mesh, width, height, levels, vector_field, convolution_weight=1e-4, stiffness_weight=1e-4,
+                 sharpening_weight=100.0, max_cols=3):
+        v, t, _ = mesh.arrays()
+        metric = vector_field_metric(embedding_metric(v, t), vector_field)
+        self.anisotropic = InitializeHierarchy(mesh, width, height, levels, triangle_metric=metric)
+        self.embedding = InitializeHierarchy(mesh, width, height, levels)
+        self.convolution_weight, self.stiffness, self.sharpening = convolution_weight, stiffness_weight, sharpening_weight
+        self.lic = UpdateLinearSystem(self.anisotropic, convolution_weight, 1.0, max_cols=max_cols)
+        self.sharpen = UpdateLinearSystem(self.embedding, stiffness_weight, 1.0, max_cols=max_cols)
+
+    def convolve(self, noise, tol=1e-6, mode="mixed", max_iters=80):
+        """(x, y): the anisotropically diffused noise and its sharpened version ([K, N] FP64)."""
+        import torch
+        b = torch.empty_like(noise)
+        self.lic.apply(OPERATOR_MASS, noise, b)
+        x = torch.zeros_like(noise)
+        self.lic.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        m, s = torch.empty_like(x), torch.empty_like(x)
+        self.sharpen.apply(OPERATOR_MASS, x, m)
+        self.sharpen.apply(OPERATOR_STIFFNESS, x, s)
+        rhs = m + (self.sharpening * self.stiffness) * s
+        y = x.clone()
+        self.sharpen.solve(y, rhs, tol=tol, max_iters=max_iters, mode=mode)
+        return x, y

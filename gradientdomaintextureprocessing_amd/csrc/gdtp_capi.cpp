@@ -157,6 +157,28 @@ int gdtp_hierarchy_create(const gdtp_mesh* mesh, int W, int H, int levels, int m
   })
 }
 
+int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int W, int H, int levels, const double* triangleMetric, gdtp_hierarchy** out) {
+  if (!mesh || !out || !triangleMetric || W < 8 || H < 8 || levels < 1 || levels > 16)
+    return Fail(GDTP_ERROR_INVALID, "invalid hierarchy arguments");
+  const int nT = mesh->m.numTriangles();
+  std::vector<M2> g(nT);
+  for (int t = 0; t < nT; t++) {
+    g[t](0, 0) = triangleMetric[3 * t], g[t](0, 1) = g[t](1, 0) = triangleMetric[3 * t + 1], g[t](1, 1) = triangleMetric[3 * t + 2];
+    if (!(g[t](0, 0) > 0) || !(g[t].det() > 0)) return Fail(GDTP_ERROR_INVALID, "triangle metric is not positive definite");
+  }
+  GDTP_TRY({
+    auto* h = new gdtp_hierarchy();
+    h->mesh = mesh->m;
+    std::string err;
+    BlockInfo bi;
+    if (!BuildHierarchy(h->mesh, W, H, levels, bi, h->h, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    ComputeParameterMetricFromSurface(h->mesh, h->h.atlas, g, h->metric);
+    if (!AssembleFineOperator(h->mesh, h->h, h->metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    *out = h;
+    return GDTP_OK;
+  })
+}
+
 int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info) {
   if (!h || !info || level < 0 || level >= (int)h->h.levels.size()) return Fail(GDTP_ERROR_INVALID, "invalid level");
   const Level& L = h->h.levels[level];

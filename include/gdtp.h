@@ -55,6 +55,12 @@ void gdtp_mesh_destroy(gdtp_mesh* mesh);
  * (BoundaryQuadraticElements.inl:775-781), InitializeMetric (Metric.inl:448-466) and
  * InitializeMassAndStiffness (MassAndStiffness.h:490-549).  Block shape 128x16, padding 2x0. */
 int gdtp_hierarchy_create(const gdtp_mesh* mesh, int width, int height, int levels, int metric_mode, gdtp_hierarchy** out);
+/* The same with a caller-supplied surface metric per mesh triangle: 3 doubles (g00, g01, g11), the
+ * metric in the triangle's edge basis (p1 - p0, p2 - p0), used as given (no area normalisation).
+ * Replaces InitializeAnisotropicMetric / InitializeVectorFieldMetric + InitializeParameterMetric
+ * (Metric.inl:12-67, 415-486) for line integral convolution. */
+int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int width, int height, int levels, const double* triangle_metric,
+                                      gdtp_hierarchy** out);
 /* info[0..7] = texels, deep, boundary, charts, raster lines, band tasks, blocks, segments */
 int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info);
 /* Named integer / double arrays of the hierarchy (see gdtp_capi.cpp for the list); call with
