@@ -14,6 +14,7 @@ SIGNATURES = {
     "gdtp_last_error": (c_char_p, []),
     "gdtp_device_count": (c_int, [P(c_int)]),
     "gdtp_mesh_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, P(c_void_p)]),
+    "gdtp_mesh_read_ply": (c_int, [c_char_p, P(c_void_p)]),
     "gdtp_mesh_create_cube": (c_int, [c_int, c_int, c_int, P(c_void_p)]),
     "gdtp_mesh_create_icosphere": (c_int, [c_int, c_int, c_int, c_double, c_ull, P(c_void_p)]),
     "gdtp_mesh_create_torus": (c_int, [c_int, c_int, c_int, c_int, c_double, c_double, c_double, c_ull, c_int, c_int, P(c_void_p)]),

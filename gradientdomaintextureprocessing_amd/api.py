@@ -45,6 +45,14 @@ class TexturedMesh:
         return cls(h)
 
     @classmethod
+    def read_ply(cls, path):
+        """PLY with per-face texcoord lists (ReadTexturedMesh, SimpleMesh.h:118-160)."""
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(lib.gdtp_mesh_read_ply(str(path).encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
     def cube(cls, subdiv, width, height):
         lib = _lib.load()
         h = ctypes.c_void_p()
@@ -390,3 +398,108 @@ class LineIntegralConvolution:
         y = x.clone()
         self.sharpen.solve(y, rhs, tol=tol, max_iters=max_iters, mode=mode)
         return x, y
+
+
+# ---------------------------------------------------------------------------- file formats and the apps' preprocessing
+class Padding:
+    """Texels added on each side of the atlas so that every texture coordinate lies inside the
+    texel-centre box and both sides are multiples of 8 (Padding.h:4-45)."""
+
+    def __init__(self, left=0, bottom=0, right=0, top=0):
+        self.left, self.bottom, self.right, self.top = left, bottom, right, top
+
+    @property
+    def non_trivial(self):
+        return bool(self.left or self.bottom or self.right or self.top)
+
+
+def compute_padding(width, height, uv):
+    """ComputePadding (Padding.h:16-45); uv: [..., 2] texture coordinates."""
+    import math
+    uv = np.asarray(uv, np.float64).reshape(-1, 2)
+    lo, hi = uv.min(axis=0), uv.max(axis=0)
+    pmin = (0.5 / width, 0.5 / height)
+    pmax = ((width - 0.5) / width, (height - 0.5) / height)
+    left = math.ceil((pmin[0] - lo[0]) * width) if lo[0] < pmin[0] else 0
+    bottom = math.ceil((pmin[1] - lo[1]) * height) if lo[1] < pmin[1] else 0
+    right = math.ceil((hi[0] - pmax[0]) * width) if hi[0] > pmax[0] else 0
+    top = math.ceil((hi[1] - pmax[1]) * height) if hi[1] > pmax[1] else 0
+    nw, nh = width + left + right, height + bottom + top
+    left += 8 * ((nw - 1) // 8 + 1) - nw
+    bottom += 8 * ((nh - 1) // 8 + 1) - nh
+    return Padding(left, bottom, right, top)
+
+
+def pad_texture_coordinates(padding, width, height, uv):
+    """PaddTextureCoordinates (Padding.h:70-77): returns the new coordinates."""
+    uv = np.array(uv, np.float64)
+    nw, nh = width + padding.left + padding.right, height + padding.bottom + padding.top
+    uv[..., 0] = (uv[..., 0] * width + padding.left) / nw
+    uv[..., 1] = (uv[..., 1] * height + padding.bottom) / nh
+    return uv
+
+
+def pad_image(padding, image):
+    """PaddImage (Padding.h:46-57): image [H, W, C] (row j = texture row j) with clamped borders."""
+    h, w = image.shape[:2]
+    jj = np.clip(np.arange(h + padding.bottom + padding.top) - padding.bottom, 0, h - 1)
+    ii = np.clip(np.arange(w + padding.left + padding.right) - padding.left, 0, w - 1)
+    return image[jj][:, ii]
+
+
+def unpad_image(padding, image):
+    """UnpaddImage (Padding.h:58-66)."""
+    h, w = image.shape[:2]
+    return image[padding.bottom:h - padding.top, padding.left:w - padding.right]
+
+
+def load_textured_mesh(path, width, height, flip_v=True, jitter=False, seed=0):
+    """The apps' mesh preparation (TextureFiltering.cpp:807-837): read the PLY, centre the
+    vertices and scale them to unit radius, flip v, optionally jitter texture coordinates by
+    1e-3 texel (seeded here; the reference draws from rand(), AtlasMesh.inl:36-55), and pad the
+    atlas.  Returns (mesh, width, height, padding) with the padded resolution."""
+    m = TexturedMesh.read_ply(path)
+    v, t, uv = m.arrays()
+    v = v - v.mean(axis=0)
+    v /= np.linalg.norm(v, axis=1).max()
+    if flip_v:
+        uv[..., 1] = 1.0 - uv[..., 1]
+    if jitter:
+        uv = uv + (np.random.default_rng(seed).random(uv.shape) * 2 - 1) * (1e-3 / max(width, height))
+    pad = compute_padding(width, height, uv)
+    if pad.non_trivial:
+        uv = pad_texture_coordinates(pad, width, height, uv)
+        width, height = width + pad.left + pad.right, height + pad.bottom + pad.top
+    return TexturedMesh.from_arrays(v, t, uv), width, height, pad
+
+
+def read_vector(path, dtype):
+    """ReadVector (VectorIO.h:8-18): int32 count, then count records of `dtype` (e.g. the .vf
+    files: np.dtype((np.float64, 2)))."""
+    with open(path, "rb") as fh:
+        n = int(np.frombuffer(fh.read(4), np.int32)[0])
+        return np.frombuffer(fh.read(), dtype, count=n).copy()
+
+
+def write_vector(path, vec):
+    """WriteVector (VectorIO.h:19-27)."""
+    vec = np.ascontiguousarray(vec)
+    with open(path, "wb") as fh:
+        fh.write(np.int32(len(vec)).tobytes())
+        fh.write(vec.tobytes())
+
+
+def read_binary_image(path, channels=3, dtype=np.float32):
+    """ReadBinaryImage (VectorIO.h:39-50), e.g. .normap: int32 width, height, then width*height
+    records of `channels` values; returns [height, width, channels]."""
+    with open(path, "rb") as fh:
+        w, h = (int(x) for x in np.frombuffer(fh.read(8), np.int32))
+        return np.frombuffer(fh.read(), dtype, count=w * h * channels).reshape(h, w, channels).copy()
+
+
+def write_binary_image(path, image):
+    """WriteBinaryImage (VectorIO.h:28-38)."""
+    image = np.ascontiguousarray(image)
+    with open(path, "wb") as fh:
+        fh.write(np.array([image.shape[1], image.shape[0]], np.int32).tobytes())
+        fh.write(image.tobytes())

@@ -99,6 +99,17 @@ int gdtp_mesh_create(const double* V, int nv, const int* T, int nt, const double
   })
 }
 
+int gdtp_mesh_read_ply(const char* path, gdtp_mesh** out) {
+  if (!path || !out) return Fail(GDTP_ERROR_INVALID, "null argument");
+  GDTP_TRY({
+    auto* m = new gdtp_mesh();
+    std::string err;
+    if (!ReadPly(path, m->m, err)) { delete m; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    *out = m;
+    return GDTP_OK;
+  })
+}
+
 int gdtp_mesh_create_cube(int subdiv, int W, int H, gdtp_mesh** out) {
   if (!out) return Fail(GDTP_ERROR_INVALID, "null output");
   auto* m = new gdtp_mesh();

@@ -1,9 +1,8 @@
-// Seeded synthetic textured meshes for the solver's workloads, plus a compact binary mesh file.
+// Seeded synthetic textured meshes for the solver's workloads, a compact binary mesh file, and
+// PLY input with per-corner texture coordinates (include/Src/SimpleMesh.h:118-160).
 //
-// The reference reads PLY meshes with per-corner texture coordinates
-// (include/Src/SimpleMesh.h:118-160).  The workloads here are synthetic, so the meshes are
-// generated: a unit cube (6 charts packed 3x2), a noisy icosphere (20 icosahedron-face charts)
-// and a noisy torus (rectangular patches).  Every UV corner is kept at least 1e-3 texel away
+// The generated meshes: a unit cube (6 charts packed 3x2), a noisy icosphere (20 icosahedron-face
+// charts) and a noisy torus (rectangular patches).  Every UV corner is kept at least 1e-3 texel away
 // from texel-centre lines so that the reference's automatic grid-line jitter
 // (include/Src/AtlasMesh.inl:36-55) never triggers.
 #include "gdtp_mesh.h"
@@ -12,6 +11,7 @@
 #include <array>
 #include <cstring>
 #include <map>
+#include <sstream>
 
 namespace gdtp {
 
@@ -269,6 +269,150 @@ int ReadMesh(const char* path, TexturedMesh& m) {
   }
   fclose(f);
   return ok ? 1 : 0;
+}
+
+
+
+// ---------------------------------------------------------------------------- PLY input
+// Triangle meshes with per-corner texture coordinates in the layout the reference reads
+// (SimpleMesh.h:118-160): element vertex with x, y, z; element face with a vertex_indices list
+// and a texcoord list (u, v per corner).  ASCII and binary little-endian; other properties skipped.
+namespace {
+
+struct PlyProp {
+  std::string name, type, countType;
+  bool list = false;
+};
+struct PlyElement {
+  std::string name;
+  long count = 0;
+  std::vector<PlyProp> props;
+};
+
+int TypeSize(const std::string& t) {
+  if (t == "char" || t == "uchar" || t == "int8" || t == "uint8") return 1;
+  if (t == "short" || t == "ushort" || t == "int16" || t == "uint16") return 2;
+  if (t == "int" || t == "uint" || t == "float" || t == "int32" || t == "uint32" || t == "float32") return 4;
+  if (t == "double" || t == "float64") return 8;
+  return 0;
+}
+
+double ReadBinary(const unsigned char* p, const std::string& t) {
+  if (t == "char" || t == "int8") return (double)*(const int8_t*)p;
+  if (t == "uchar" || t == "uint8") return (double)*p;
+  int16_t s16;
+  uint16_t u16;
+  int32_t s32;
+  uint32_t u32;
+  float f;
+  double d;
+  if (t == "short" || t == "int16") return memcpy(&s16, p, 2), (double)s16;
+  if (t == "ushort" || t == "uint16") return memcpy(&u16, p, 2), (double)u16;
+  if (t == "int" || t == "int32") return memcpy(&s32, p, 4), (double)s32;
+  if (t == "uint" || t == "uint32") return memcpy(&u32, p, 4), (double)u32;
+  if (t == "float" || t == "float32") return memcpy(&f, p, 4), (double)f;
+  return memcpy(&d, p, 8), d;
+}
+
+}  // namespace
+
+int ReadPly(const std::string& path, TexturedMesh& m, std::string& err) {
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) return err = "cannot open " + path, 0;
+  std::vector<unsigned char> data;
+  {
+    unsigned char buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), fp)) > 0) data.insert(data.end(), buf, buf + n);
+    fclose(fp);
+  }
+  // header
+  size_t pos = 0;
+  auto line = [&](std::string& out) {
+    if (pos >= data.size()) return false;
+    size_t e = pos;
+    while (e < data.size() && data[e] != '\n') e++;
+    out.assign((const char*)&data[pos], e - pos);
+    if (!out.empty() && out.back() == '\r') out.pop_back();
+    pos = e + 1;
+    return true;
+  };
+  std::string ln;
+  if (!line(ln) || ln != "ply") return err = path + ": not a PLY file", 0;
+  bool binary = false;
+  std::vector<PlyElement> elems;
+  for (;;) {
+    if (!line(ln)) return err = path + ": truncated header", 0;
+    std::istringstream is(ln);
+    std::string kw;
+    is >> kw;
+    if (kw == "format") {
+      std::string f;
+      is >> f;
+      if (f == "binary_little_endian") binary = true;
+      else if (f != "ascii") return err = path + ": unsupported PLY format " + f, 0;
+    } else if (kw == "element") {
+      PlyElement e;
+      is >> e.name >> e.count;
+      elems.push_back(e);
+    } else if (kw == "property") {
+      if (elems.empty()) return err = path + ": property before element", 0;
+      PlyProp p;
+      std::string t;
+      is >> t;
+      if (t == "list") p.list = true, is >> p.countType >> p.type >> p.name;
+      else p.type = t, is >> p.name;
+      if (!TypeSize(p.type) || (p.list && !TypeSize(p.countType))) return err = path + ": unknown property type", 0;
+      elems.back().props.push_back(p);
+    } else if (kw == "end_header") break;
+  }
+  std::vector<V3> verts;
+  std::vector<int> tris;
+  std::vector<V2> uv;
+  std::istringstream ascii;
+  if (!binary) ascii.str(std::string((const char*)data.data() + pos, data.size() - pos));
+  auto scalar = [&](const std::string& t, double& v) -> bool {
+    if (!binary) return (bool)(ascii >> v);
+    const int sz = TypeSize(t);
+    if (pos + sz > data.size()) return false;
+    v = ReadBinary(&data[pos], t);
+    pos += sz;
+    return true;
+  };
+  for (const PlyElement& e : elems) {
+    const bool isVertex = e.name == "vertex", isFace = e.name == "face";
+    for (long r = 0; r < e.count; r++) {
+      V3 p;
+      std::vector<double> idx, tc;
+      for (const PlyProp& pr : e.props) {
+        if (pr.list) {
+          double cnt;
+          if (!scalar(pr.countType, cnt)) return err = path + ": truncated data", 0;
+          std::vector<double> vals((size_t)cnt);
+          for (auto& v : vals)
+            if (!scalar(pr.type, v)) return err = path + ": truncated data", 0;
+          if (isFace && (pr.name == "vertex_indices" || pr.name == "vertex_index")) idx = vals;
+          if (isFace && pr.name == "texcoord") tc = vals;
+        } else {
+          double v;
+          if (!scalar(pr.type, v)) return err = path + ": truncated data", 0;
+          if (isVertex && pr.name == "x") p.x = v;
+          if (isVertex && pr.name == "y") p.y = v;
+          if (isVertex && pr.name == "z") p.z = v;
+        }
+      }
+      if (isVertex) verts.push_back(p);
+      if (isFace) {
+        if (idx.size() != 3 || tc.size() != 6) return err = path + ": faces must be triangles with 6 texcoords", 0;
+        for (int k = 0; k < 3; k++) tris.push_back((int)idx[k]), uv.push_back(V2(tc[2 * k], tc[2 * k + 1]));
+      }
+    }
+  }
+  for (int v : tris)
+    if (v < 0 || v >= (int)verts.size()) return err = path + ": vertex index out of range", 0;
+  if (tris.empty()) return err = path + ": no faces", 0;
+  m.vertices = std::move(verts), m.triangles = std::move(tris), m.uv = std::move(uv);
+  return 1;
 }
 
 }  // namespace gdtp

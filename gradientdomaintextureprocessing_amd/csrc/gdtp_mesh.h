@@ -23,4 +23,7 @@ void MakeNoiseTexture(int W, int H, int channels, uint64_t seed, std::vector<flo
 int WriteMesh(const char* path, const TexturedMesh& m);
 int ReadMesh(const char* path, TexturedMesh& m);
 
+// PLY with per-face texcoord lists (SimpleMesh.h:118-160), ASCII or binary little-endian.
+int ReadPly(const std::string& path, TexturedMesh& m, std::string& err);
+
 }  // namespace gdtp

@@ -42,6 +42,9 @@ int gdtp_device_count(int* count);
  *      include/Src/SimpleMesh.h:45-49, read from PLY at :118-160) */
 int gdtp_mesh_create(const double* vertices, int num_vertices, const int* triangles, int num_triangles,
                      const double* corner_uv, gdtp_mesh** out);
+/* Triangle mesh with per-face texcoord lists from a PLY file (ASCII or binary little-endian); the
+ * reference's ReadTexturedMesh (SimpleMesh.h:118-160) without the texture image. */
+int gdtp_mesh_read_ply(const char* path, gdtp_mesh** out);
 int gdtp_mesh_create_cube(int subdiv, int width, int height, gdtp_mesh** out);
 int gdtp_mesh_create_icosphere(int levels, int width, int height, double amplitude, unsigned long long seed, gdtp_mesh** out);
 int gdtp_mesh_create_torus(int nu, int nv, int charts_u, int charts_v, double major_radius, double minor_radius,
