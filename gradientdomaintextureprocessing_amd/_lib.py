@@ -41,6 +41,7 @@ SIGNATURES = {
     "gdtp_coarse_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_solve_weighted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_geodesic_divergence": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_mixed_configure": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int]),

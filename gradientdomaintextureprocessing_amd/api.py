@@ -235,6 +235,16 @@ class MultigridSystem:
                                           ctypes.byref(it), _stream()))
         return it.value, rel
 
+    def solve_weighted(self, x, b, weights, tol=1e-6, max_iters=60):
+        """Column c solves (weights[c][0] M + weights[c][1] S) x = b (mixed precision; this system's
+        V-cycle preconditions every column).  Returns (iterations, rel_res)."""
+        w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(x.shape[0], 2))
+        rel = np.zeros(x.shape[0])
+        it = ctypes.c_int(0)
+        _lib.check(_lib.load().gdtp_solve_weighted(self._h, _ptr(x), _ptr(b), x.shape[0], w.ctypes.data, tol, max_iters, rel.ctypes.data,
+                                                   ctypes.byref(it), _stream()))
+        return it.value, rel
+
     def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True):
         _lib.check(_lib.load().gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
 

@@ -455,6 +455,24 @@ int gdtp_system_memory(const gdtp_system* s, size_t* refBytes, size_t* mixedByte
   return GDTP_OK;
 }
 
+int gdtp_solve_weighted(gdtp_system* s, double* x, const double* b, int ncols, const double* weights, double tol, int maxIters,
+                        double* relRes, int* iterations, void* stream) {
+  int rc = Ready(s, ncols);
+  if (rc != GDTP_OK) return rc;
+  if (!x || !b || !weights || !(tol > 0) || maxIters < 0) return Fail(GDTP_ERROR_INVALID, "invalid solve arguments");
+  GDTP_TRY({
+    std::vector<double> rel(ncols, 0.0);
+    const int it = SolveMixed(*s->ds, Mixed(s), x, b, ncols, tol, maxIters, rel.data(), (hipStream_t)stream, s->useGraph != 0, weights);
+    GDTP_HIP(hipGetLastError());
+    bool ok = true;
+    for (double r : rel) ok = ok && r <= tol;
+    if (relRes) for (int k = 0; k < ncols; k++) relRes[k] = rel[k];
+    if (iterations) *iterations = it;
+    if (!ok) return Fail(GDTP_ERROR_NUMERIC, "solve did not reach the tolerance within max_iters");
+    return GDTP_OK;
+  })
+}
+
 int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rhs, int ncols, void* stream) {
   GDTP_DEVICE_OP(potential && rhs, {
     if (!s->vf) {

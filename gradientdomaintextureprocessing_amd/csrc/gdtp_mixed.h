@@ -48,12 +48,14 @@ struct MixedSystem {
   double* coarseInv = nullptr;     // dense inverse of a small coarsest level (row-major), else null
   double* norms = nullptr;         // device [2][kcap][64]: squared residual / rhs norm partial sums
   double* hostNorms = nullptr;     // pinned copy
+  double* weights = nullptr;       // device [kcap][2]: per-column (mass, stiffness) weights of weighted solves
   std::vector<void*> allocations;
   size_t bytes = 0;                // device bytes held
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphK = 0;
   const void *graphX = nullptr, *graphB = nullptr;
+  bool graphWeighted = false;
   hipStream_t stream = nullptr;
   ~MixedSystem();
 };
@@ -62,9 +64,11 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 
 // Solve A x = b (level-0 compact FP64 vectors, K columns) until every column's relative residual
 // is <= tol or maxIter refinement steps; x is used as the initial guess.  Returns the iterations
-// run; relRes (host, K entries) receives the final relative residuals.
+// run; relRes (host, K entries) receives the final relative residuals.  With `weights` (host,
+// K pairs), column c solves (w[c][0] M + w[c][1] S) x = b instead, the system's own V-cycle
+// serving as the preconditioner of every column.
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph);
+               hipStream_t s, bool useGraph, const double* weights = nullptr);
 
 // Drop the captured refinement step (after the smoothing configuration changes).
 void InvalidateGraph(MixedSystem& ms);

@@ -345,6 +345,90 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __
 }
 
 // boundary rows (blockIdx.y = column)
+// The same residual for per-column operators A_c = w[c][0] M + w[c][1] S: both level-0 stencils are
+// read once per texel for up to four columns (parametric solves: one refinement loop serves
+// columns with different weights, its FP32 V-cycle built for one nearby pair of weights).
+__global__ __launch_bounds__(256) void K64ResidualDeepW(int nLines, const int* __restrict__ lines, const double* __restrict__ coefM,
+                                                        const double* __restrict__ coefS, int D, const double* __restrict__ x,
+                                                        const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                        float* e0, long Nd, double* norms, int K, const double* __restrict__ w) {
+  const int l = blockIdx.x;
+  const int* L = lines + 5 * l;
+  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      double cm[9], cs[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d0 + i], cs[k] = coefS[(long)k * D + d0 + i];
+      const long di = g2d[start + i];
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) {
+        if (cc >= kc) break;
+        const int ch = c0 + cc;
+        const double* X = x + (long)ch * N;
+        double sm = 0.0, ss = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int base = k < 3 ? prev : (k < 6 ? start : next);
+          const double xv = X[base + i + k % 3 - 1];
+          sm += cm[k] * xv, ss += cs[k] * xv;
+        }
+        const double r = b[(long)ch * N + start + i] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
+        rhs0[(long)ch * Nd + di] = (float)r;
+        e0[(long)ch * Nd + di] = 0.0f;
+        acc[cc] += r * r;
+      }
+    }
+    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);
+  }
+}
+
+// Boundary rows of the per-column-weights residual: rows of M (A_BB, A_BD) then of S.
+__global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
+                                                            const int* __restrict__ mcold, const double* __restrict__ mvald,
+                                                            const int* __restrict__ mrpb, const int* __restrict__ mcolb,
+                                                            const double* __restrict__ mvalb, const int* __restrict__ srpd,
+                                                            const int* __restrict__ scold, const double* __restrict__ svald,
+                                                            const int* __restrict__ srpb, const int* __restrict__ scolb,
+                                                            const double* __restrict__ svalb, const double* __restrict__ x,
+                                                            const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                            float* e0, long Nd, double* norms, int K, const double* __restrict__ w) {
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  const bool active = r < B;
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double am[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+      auto rows = [&](const int* rp, const int* cl, const double* vl, bool viaBoundary, double (&acc)[4]) {
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes) {
+          const double v = vl[k];
+          const int g = viaBoundary ? bgi[cl[k]] : cl[k];
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (q < kc) acc[q] += v * x[(long)(c0 + q) * N + g];
+        }
+      };
+      rows(mrpb, mcolb, mvalb, true, am);
+      rows(mrpd, mcold, mvald, false, am);
+      rows(srpb, scolb, svalb, true, as);
+      rows(srpd, scold, svald, false, as);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) am[q] = RowSum(am[q]), as[q] = RowSum(as[q]);
+    double sq = 0.0;
+    if (active && lane < kc) {
+      const int ch = c0 + lane, g = bgi[r];
+      const double rv = b[(long)ch * N + g] - (w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane));
+      rhs0[(long)ch * Nd + g2d[g]] = (float)rv;
+      e0[(long)ch * Nd + g2d[g]] = 0.0f;
+      sq = rv * rv;
+    }
+    for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
+  }
+}
+
 __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
                                                            const int* __restrict__ cold, const double* __restrict__ vald,
                                                            const int* __restrict__ rpb, const int* __restrict__ colb,
@@ -680,6 +764,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
   ms.coarseX = Zeros<double>(ms, (size_t)Nc * kcap);
   ms.coarseY = Zeros<double>(ms, (size_t)Nc * kcap);
   ms.norms = Zeros<double>(ms, 2 * (size_t)kcap * kSlots);
+  ms.weights = Zeros<double>(ms, 2 * (size_t)kcap);
   GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * (size_t)kcap * kSlots * sizeof(double), hipHostMallocDefault));
   GDTP_HIP(hipStreamCreateWithFlags(&ms.stream, hipStreamNonBlocking));
   (void)ds;
@@ -714,7 +799,7 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
 void InvalidateGraph(MixedSystem& ms) {
   if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec);
   if (ms.graph) (void)hipGraphDestroy(ms.graph);
-  ms.graphExec = nullptr, ms.graph = nullptr, ms.graphK = 0, ms.graphX = nullptr, ms.graphB = nullptr;
+  ms.graphExec = nullptr, ms.graph = nullptr, ms.graphK = 0, ms.graphX = nullptr, ms.graphB = nullptr, ms.graphWeighted = false;
 }
 
 void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
@@ -761,10 +846,21 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
 }
 
 // residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
-static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, hipStream_t s) {
+static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s) {
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  if (weighted) {
+    const DevLevel &M = ds.mass0, &S = ds.stiff0;
+    if (L0.nRaster > 0)
+      K64ResidualDeepW<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, M.coefRaw, S.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
+                                                  ms.norms, K, ms.weights);
+    if (L0.B > 0)
+      K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(
+          L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr,
+          S.bb.col, S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights);
+    return;
+  }
   if (L0.nRaster > 0)
     K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
   if (L0.B > 0)
@@ -780,13 +876,21 @@ static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hip
 }
 
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph) {
+               hipStream_t s, bool useGraph, const double* weights) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
+  const bool weighted = weights != nullptr;
+  if (weighted) {
+    for (int k = 0; k < 2 * K; k++)
+      if (!(weights[k] >= 0) || !std::isfinite(weights[k])) throw std::runtime_error("operator weights must be finite and non-negative");
+    for (int k = 0; k < K; k++)
+      if (!(weights[2 * k] > 0)) throw std::runtime_error("the mass weight of every column must be positive");
+    GDTP_HIP(hipMemcpyAsync(ms.weights, weights, 2 * (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
+  }
   const int N = ms.L[0].N;
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
   K64SquaredNorm<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, b, bNorms);
-  OuterResidual(ds, ms, x, b, K, s);
+  OuterResidual(ds, ms, x, b, K, weighted, s);
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
   auto slotSum = [&](const double* p) {
@@ -809,10 +913,10 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   auto step = [&](hipStream_t st) {
     MixedVCycle(ds, ms, K, st);
     OuterUpdate(ds, ms, x, K, st);
-    OuterResidual(ds, ms, x, b, K, st);
+    OuterResidual(ds, ms, x, b, K, weighted, st);
     GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));
   };
-  if (useGraph && !(ms.graphExec && ms.graphK == K && ms.graphX == x && ms.graphB == b)) {
+  if (useGraph && !(ms.graphExec && ms.graphK == K && ms.graphX == x && ms.graphB == b && ms.graphWeighted == weighted)) {
     // capture one refinement step on the solver's own stream (shape-static per K, x, b)
     if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec), ms.graphExec = nullptr;
     if (ms.graph) (void)hipGraphDestroy(ms.graph), ms.graph = nullptr;
@@ -820,7 +924,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     step(ms.stream);
     GDTP_HIP(hipStreamEndCapture(ms.stream, &ms.graph));
     GDTP_HIP(hipGraphInstantiate(&ms.graphExec, ms.graph, nullptr, nullptr, 0));
-    ms.graphK = K, ms.graphX = x, ms.graphB = b;
+    ms.graphK = K, ms.graphX = x, ms.graphB = b, ms.graphWeighted = weighted;
   }
   int it = 0;
   while (it < maxIter) {

@@ -10,7 +10,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 import oracle
-from helpers import noise_texture_rhs, oracle_levels
+from helpers import cube_system, noise_texture_rhs, oracle_levels
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -94,3 +94,25 @@ def test_columns_alone_and_together(case):
         e = alone - together[k]
         xd = together[k]
         assert np.sqrt(e @ (A @ e) / (xd @ (A @ xd))) < 1e-5
+
+
+def test_weighted_columns_match_direct_solves():
+    """Parametric solve: one system (built for alpha = 1e-3) serves columns with other weights."""
+    import scipy.sparse.linalg as spla
+    from helpers import mass_matrix, stiffness_matrix
+    _, H, S = cube_system(256, 5, max_cols=4)
+    L0 = oracle_levels(H, S)[0]
+    M, Kst = mass_matrix(H, L0), stiffness_matrix(H, L0)
+    weights = np.array([[1.0, 1e-4], [1.0, 1e-3], [1.0, 1e-2], [0.5, 3e-3]])
+    f = np.random.default_rng(8).random((4, L0.N))
+    b = (M @ f.T).T
+    x = torch.zeros((4, L0.N), dtype=torch.float64, device="cuda")
+    its, rel = S.solve_weighted(x, torch.from_numpy(b).cuda(), weights, tol=1e-10, max_iters=60)
+    print(f"[weighted] {its} iterations, rel res {rel}", flush=True)
+    assert np.all(rel <= 1e-10)
+    xs = x.cpu().numpy()
+    for k, (sw, aw) in enumerate(weights):
+        A = (sw * M + aw * Kst).tocsc()
+        want = spla.splu(A).solve(b[k])
+        e = xs[k] - want
+        assert e @ (A @ e) <= 1e-14 * (want @ (A @ want))
