@@ -441,7 +441,11 @@ int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySwee
     return Fail(GDTP_ERROR_INVALID, "invalid mixed-solver configuration");
   s->nuPre = nuPre, s->nuPost = nuPost, s->boundarySweeps = boundarySweeps, s->omega = omega, s->useGraph = useGraph ? 1 : 0;
   if (s->mixed) InvalidateGraph(*s->mixed);  // the captured step holds the previous schedule
-  return GDTP_OK;
+  if (!s->ds) return GDTP_OK;
+  GDTP_TRY({
+    Mixed(s);  // build the mixed solver now (device set-up off the first solve's path)
+    return GDTP_OK;
+  })
 }
 
 int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream) {

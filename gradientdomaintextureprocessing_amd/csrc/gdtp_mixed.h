@@ -51,11 +51,17 @@ struct MixedSystem {
   double* weights = nullptr;       // device [kcap][2]: per-column (mass, stiffness) weights of weighted solves
   std::vector<void*> allocations;
   size_t bytes = 0;                // device bytes held
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graphExec = nullptr;
-  int graphK = 0;
-  const void *graphX = nullptr, *graphB = nullptr;
-  bool graphWeighted = false;
+  // captured refinement steps, one per (K, x, b, weighted) in use (least recently used evicted)
+  struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int K = 0;
+    const void *x = nullptr, *b = nullptr;
+    bool weighted = false;
+    long lastUse = 0;
+  };
+  std::vector<GraphEntry> graphs;
+  long graphClock = 0;
   hipStream_t stream = nullptr;
   ~MixedSystem();
 };

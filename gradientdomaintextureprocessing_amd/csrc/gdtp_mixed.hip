@@ -487,8 +487,10 @@ __global__ void K64Update(int N, const int* __restrict__ g2d, const float* __res
 
 // ------------------------------------------------------------------ host
 MixedSystem::~MixedSystem() {
-  if (graphExec) (void)hipGraphExecDestroy(graphExec);
-  if (graph) (void)hipGraphDestroy(graph);
+  for (auto& g : graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
   if (stream) (void)hipStreamDestroy(stream);
   if (hostNorms) (void)hipHostFree(hostNorms);
   for (void* p : allocations) (void)hipFree(p);
@@ -797,10 +799,14 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
 }
 
 void InvalidateGraph(MixedSystem& ms) {
-  if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec);
-  if (ms.graph) (void)hipGraphDestroy(ms.graph);
-  ms.graphExec = nullptr, ms.graph = nullptr, ms.graphK = 0, ms.graphX = nullptr, ms.graphB = nullptr, ms.graphWeighted = false;
+  for (auto& g : ms.graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  ms.graphs.clear();
 }
+
+static constexpr size_t kMaxGraphs = 8;
 
 void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
   if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
@@ -916,20 +922,32 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     OuterResidual(ds, ms, x, b, K, weighted, st);
     GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));
   };
-  if (useGraph && !(ms.graphExec && ms.graphK == K && ms.graphX == x && ms.graphB == b && ms.graphWeighted == weighted)) {
-    // capture one refinement step on the solver's own stream (shape-static per K, x, b)
-    if (ms.graphExec) (void)hipGraphExecDestroy(ms.graphExec), ms.graphExec = nullptr;
-    if (ms.graph) (void)hipGraphDestroy(ms.graph), ms.graph = nullptr;
-    GDTP_HIP(hipStreamBeginCapture(ms.stream, hipStreamCaptureModeThreadLocal));
-    step(ms.stream);
-    GDTP_HIP(hipStreamEndCapture(ms.stream, &ms.graph));
-    GDTP_HIP(hipGraphInstantiate(&ms.graphExec, ms.graph, nullptr, nullptr, 0));
-    ms.graphK = K, ms.graphX = x, ms.graphB = b, ms.graphWeighted = weighted;
+  MixedSystem::GraphEntry* ge = nullptr;
+  if (useGraph) {
+    for (auto& g : ms.graphs)
+      if (g.K == K && g.x == x && g.b == b && g.weighted == weighted) ge = &g;
+    if (!ge) {  // capture one refinement step on the solver's own stream (shape-static per K, x, b)
+      if (ms.graphs.size() >= kMaxGraphs) {
+        auto old = std::min_element(ms.graphs.begin(), ms.graphs.end(), [](const auto& a, const auto& c) { return a.lastUse < c.lastUse; });
+        if (old->exec) (void)hipGraphExecDestroy(old->exec);
+        if (old->graph) (void)hipGraphDestroy(old->graph);
+        ms.graphs.erase(old);
+      }
+      MixedSystem::GraphEntry g;
+      GDTP_HIP(hipStreamBeginCapture(ms.stream, hipStreamCaptureModeThreadLocal));
+      step(ms.stream);
+      GDTP_HIP(hipStreamEndCapture(ms.stream, &g.graph));
+      GDTP_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      g.K = K, g.x = x, g.b = b, g.weighted = weighted;
+      ms.graphs.push_back(g);
+      ge = &ms.graphs.back();
+    }
+    ge->lastUse = ++ms.graphClock;
   }
   int it = 0;
   while (it < maxIter) {
     if (useGraph) {
-      GDTP_HIP(hipGraphLaunch(ms.graphExec, ms.stream));
+      GDTP_HIP(hipGraphLaunch(ge->exec, ms.stream));
       GDTP_HIP(hipStreamSynchronize(ms.stream));
     } else {
       step(s);

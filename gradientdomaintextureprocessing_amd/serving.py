@@ -1,0 +1,235 @@
+"""Request loop over resident meshes with continuous batching.
+
+Each resident mesh keeps its hierarchy, its multigrid systems and their factorisations on the GPU
+(one system per stiffness bucket, built once).  Requests name a mesh and carry their own weights:
+
+  - "filter":   (M + a S) x = M f + s a S f for the request's a (TextureFiltering.cpp:562-718),
+                x starts at f, one column per channel;
+  - "geodesic": heat-method distances from source texels (Geodesics.cpp:508-550), one column per
+                source.
+
+A worker per mesh forms batches from its queue as slots free up: requests are taken in arrival
+order while their columns fit the system's capacity, every column keeping its own operator
+weights (parametric solves share the bucket's V-cycle as preconditioner), so a request that
+arrives while others run joins the next batch.  Workers of different meshes run concurrently on
+their own HIP streams.  Meshes are admitted against an HBM budget.  Results are the same alone
+and in a batch up to the requested tolerance: every column is solved to it.
+"""
+import math
+import queue
+import threading
+import time
+
+import numpy as np
+
+from . import api
+
+
+class Request:
+    def __init__(self, mesh, kind, payload, weight, sharpness=0.0):
+        self.mesh, self.kind, self.payload, self.weight, self.sharpness = mesh, kind, payload, weight, sharpness
+        self.submitted = time.perf_counter()
+        self.done = threading.Event()
+        self.result = None
+        self.error = None
+        self.iterations = 0
+        self.latency = None
+
+    @property
+    def columns(self):
+        return int(self.payload.shape[0]) if self.kind == "filter" else len(self.payload)
+
+    def wait(self, timeout=None):
+        if not self.done.wait(timeout):
+            raise TimeoutError("request not finished")
+        if self.error is not None:
+            raise self.error
+        return self.result
+
+
+class ResidentMesh:
+    """Hierarchy and systems of one mesh: filtering buckets (M + a_b S) and, for geodesics, the
+    distance system (1e-6 M + S)."""
+
+    def __init__(self, name, hierarchy, buckets=(1e-4, 1e-3, 1e-2), max_cols=8, geodesics=True, heat_time=1e-3):
+        self.name, self.hierarchy, self.max_cols, self.heat_time = name, hierarchy, max_cols, heat_time
+        self.buckets = sorted(buckets)
+        self.systems = {a: api.UpdateLinearSystem(hierarchy, a, 1.0, max_cols=max_cols) for a in self.buckets}
+        self.distance = api.UpdateLinearSystem(hierarchy, 1.0, 1e-6, max_cols=max_cols) if geodesics else None
+        self.N = hierarchy.level_info(0)["texels"]
+
+    def bucket(self, a):
+        return min(self.buckets, key=lambda b: abs(math.log(a) - math.log(b)))
+
+    def memory(self):
+        total = 0
+        for s in list(self.systems.values()) + ([self.distance] if self.distance else []):
+            r, m = s.memory()
+            total += r + m
+        return total
+
+
+class SolverServer:
+    """Continuous-batching solve server (see the module docstring).  tol: relative residual per
+    column; hbm_budget: bytes of device memory the resident systems may hold (None: no limit)."""
+
+    def __init__(self, tol=1e-6, hbm_budget=None, max_iters=80):
+        import torch
+        self.torch = torch
+        self.tol, self.hbm_budget, self.max_iters = tol, hbm_budget, max_iters
+        self.meshes = {}
+        self.queues = {}
+        self.workers = {}
+        self.used = 0
+        self.stop = threading.Event()
+
+    # -- residency
+    def add_mesh(self, name, hierarchy, **kw):
+        m = ResidentMesh(name, hierarchy, **kw)
+        for s in m.systems.values():  # build the mixed solvers now: first requests should not pay for it
+            s.configure_mixed()
+        mem = m.memory()
+        if self.hbm_budget is not None and self.used + mem > self.hbm_budget:
+            del m
+            raise MemoryError(f"mesh {name} needs {mem / 2**30:.2f} GiB; {(self.hbm_budget - self.used) / 2**30:.2f} GiB left")
+        self.used += mem
+        self.meshes[name] = m
+        self.queues[name] = queue.Queue()
+        t = threading.Thread(target=self._work, args=(name,), daemon=True)
+        self.workers[name] = t
+        t.start()
+        return m
+
+    # -- requests
+    def submit_filter(self, mesh, texels, stiffness, sharpness=0.0):
+        """texels: [channels, N] FP64 GPU tensor."""
+        r = Request(mesh, "filter", texels, stiffness, sharpness)
+        self._enqueue(r)
+        return r
+
+    def submit_geodesic(self, mesh, sources):
+        r = Request(mesh, "geodesic", list(sources), self.meshes[mesh].heat_time)
+        self._enqueue(r)
+        return r
+
+    def _enqueue(self, r):
+        m = self.meshes.get(r.mesh)
+        if m is None:
+            raise KeyError(f"mesh {r.mesh} is not resident")
+        if r.columns < 1 or r.columns > m.max_cols:
+            raise ValueError(f"a request needs 1..{m.max_cols} columns")
+        if r.kind == "geodesic" and m.distance is None:
+            raise ValueError("mesh was added without geodesics")
+        self.queues[r.mesh].put(r)
+
+    def close(self):
+        self.stop.set()
+        for name, q in self.queues.items():
+            q.put(None)
+        for t in self.workers.values():
+            t.join(timeout=60)
+
+    # -- the per-mesh loop
+    def _work(self, name):
+        torch = self.torch
+        m = self.meshes[name]
+        q = self.queues[name]
+        stream = torch.cuda.Stream()
+        pending = []
+        while not self.stop.is_set():
+            if not pending:
+                r = q.get()
+                if r is None:
+                    return
+                pending.append(r)
+            while True:  # everything that arrived meanwhile joins the candidates
+                try:
+                    r = q.get_nowait()
+                except queue.Empty:
+                    break
+                if r is None:
+                    self.stop.set()
+                    break
+                pending.append(r)
+            batch, rest = self._form_batch(m, pending)
+            pending = rest
+            with torch.cuda.stream(stream):
+                try:
+                    self._run(m, batch)
+                except Exception as e:  # a failed batch fails its requests, not the server
+                    for r in batch:
+                        r.error = e
+                        r.done.set()
+
+    @staticmethod
+    def _form_batch(m, pending):
+        """Requests in arrival order of the first request's kind (and, for filtering, bucket) while
+        their columns fit; the others stay pending."""
+        first = pending[0]
+        key = (first.kind, m.bucket(first.weight) if first.kind == "filter" else None)
+        batch, rest, cols = [], [], 0
+        for r in pending:
+            rkey = (r.kind, m.bucket(r.weight) if r.kind == "filter" else None)
+            if rkey == key and cols + r.columns <= m.max_cols:
+                batch.append(r)
+                cols += r.columns
+            else:
+                rest.append(r)
+        return batch, rest
+
+    def _buffers(self, m):
+        """Per-mesh device vectors of max_cols columns, reused by every batch: the first K
+        columns of a buffer are contiguous, so a batch of K columns always hands the solver the
+        same pointers and replays its captured refinement step."""
+        if not hasattr(m, "buf"):
+            torch = self.torch
+            z = lambda: torch.zeros(m.max_cols, m.N, dtype=torch.float64, device="cuda")
+            m.buf = dict(x=z(), b=z(), t0=z(), t1=z())
+        return m.buf
+
+    def _run(self, m, batch):
+        torch = self.torch
+        buf = self._buffers(m)
+        K = sum(r.columns for r in batch)
+        x, b, t0, t1 = (buf[k][:K] for k in ("x", "b", "t0", "t1"))
+        if batch[0].kind == "filter":
+            system = m.systems[m.bucket(batch[0].weight)]
+            torch.cat([r.payload for r in batch], dim=0, out=t0)  # f
+            w = np.concatenate([[[1.0, r.weight]] * r.columns for r in batch])
+            system.apply(api.OPERATOR_MASS, t0, b)
+            if any(r.sharpness for r in batch):
+                system.apply(api.OPERATOR_STIFFNESS, t0, t1)
+                scale = torch.tensor(np.concatenate([[r.sharpness * r.weight] * r.columns for r in batch]), dtype=b.dtype, device=b.device)
+                b += scale[:, None] * t1
+            x.copy_(t0)
+            its, _ = system.solve_weighted(x, b, w, tol=self.tol, max_iters=self.max_iters)
+            out = x
+        else:
+            system = m.systems[m.bucket(m.heat_time)]
+            src = torch.as_tensor([s for r in batch for s in r.payload], device="cuda")
+            rows = torch.arange(K, device="cuda")
+            b.zero_()
+            b[rows, src] = 1.0
+            x.zero_()
+            its, _ = system.solve_weighted(x, b, [[1.0, m.heat_time]] * K, tol=self.tol, max_iters=self.max_iters)
+            system.geodesic_divergence(x, t0)
+            t1.zero_()
+            its2, _ = m.distance.solve(t1, t0, tol=self.tol, max_iters=self.max_iters, mode="mixed")
+            its += its2
+            t1 -= t1[rows, src][:, None]
+            out = t1
+        results = out.clone()
+        torch.cuda.current_stream().synchronize()
+        now = time.perf_counter()
+        c = 0
+        for r in batch:
+            r.result = results[c:c + r.columns]
+            r.iterations = its
+            r.latency = now - r.submitted
+            c += r.columns
+            r.done.set()
+
+
+def latency_percentiles(requests):
+    lat = np.array([r.latency for r in requests if r.latency is not None]) * 1e3
+    return dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)), n=int(len(lat)))

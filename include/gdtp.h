@@ -127,8 +127,9 @@ int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol
  * batched requests with different weights. */
 int gdtp_solve_weighted(gdtp_system* s, double* x, const double* b, int ncols, const double* weights, double tol, int max_iters,
                         double* rel_res, int* iterations, void* stream);
-/* Mixed-solver smoothing: pre / post smoothing steps, boundary Jacobi sweeps per step and
- * their damping, hipGraph capture of the refinement step (1) or plain launches (0). */
+/* Mixed-solver smoothing: pre / post smoothing steps, boundary Schwarz sweeps per step and
+ * the damping of their block update, hipGraph capture of the refinement step (1) or plain
+ * launches (0).  Builds the mixed solver's device data if it does not exist yet. */
 int gdtp_mixed_configure(gdtp_system* s, int nu_pre, int nu_post, int boundary_sweeps, double omega, int use_graph);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */

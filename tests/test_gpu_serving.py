@@ -1,0 +1,85 @@
+"""Continuous-batching request loop: a concurrent mix of filtering requests (each with its own
+stiffness weight) and geodesic requests over two resident meshes.  Every result must match the
+same request served alone, and the direct solution of its own system, to the requested tolerance."""
+import threading
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+import torch
+
+import gradientdomaintextureprocessing_amd as g
+from gradientdomaintextureprocessing_amd.serving import SolverServer, latency_percentiles
+from helpers import mass_matrix, oracle_levels, stiffness_matrix
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _mesh(name):
+    if name == "cube":
+        return g.InitializeHierarchy(g.TexturedMesh.cube(16, 256, 256), 256, 256, 5)
+    return g.InitializeHierarchy(g.TexturedMesh.icosphere(4, 512, 512, 0.05, 3), 512, 512, 6)
+
+
+def test_mixed_requests_match_alone_and_direct():
+    server = SolverServer(tol=TOL, hbm_budget=8 << 30, max_iters=120)
+    try:
+        hs = {n: _mesh(n) for n in ("cube", "ico")}
+        for n, h in hs.items():
+            server.add_mesh(n, h, max_cols=6, geodesics=(n == "cube"))
+        rng = np.random.default_rng(9)
+        reqs = []
+        for i in range(10):
+            n = "cube" if i % 2 else "ico"
+            N = hs[n].level_info(0)["texels"]
+            f = torch.from_numpy(rng.random((3 if i % 3 else 1, N))).cuda()
+            reqs.append(("filter", n, f, float(10 ** rng.uniform(-4, -2))))
+        Nc = hs["cube"].level_info(0)["texels"]
+        for i in range(3):
+            reqs.append(("geodesic", "cube", [int(s) for s in rng.integers(0, Nc, 2)], None))
+        # in the mix: submitted from two threads at once
+        handles = [None] * len(reqs)
+
+        def submit(idx):
+            for i in idx:
+                kind, n, p, a = reqs[i]
+                handles[i] = server.submit_filter(n, p, a) if kind == "filter" else server.submit_geodesic(n, p)
+
+        ts = [threading.Thread(target=submit, args=(range(k, len(reqs), 2),)) for k in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        mixed = [h.wait(120).cpu().numpy() for h in handles]
+        print("[serving] mix latency", latency_percentiles(handles), flush=True)
+        # alone: one request at a time
+        alone = []
+        for kind, n, p, a in reqs:
+            h = server.submit_filter(n, p, a) if kind == "filter" else server.submit_geodesic(n, p)
+            alone.append(h.wait(120).cpu().numpy())
+        mats = {}
+        for n, h in hs.items():
+            L0 = oracle_levels(h, server.meshes[n].systems[1e-3])[0]
+            mats[n] = (mass_matrix(h, L0), stiffness_matrix(h, L0))
+        for (kind, n, p, a), xm, xa in zip(reqs, mixed, alone):
+            M, S = mats[n]
+            A = (M + a * S).tocsc() if kind == "filter" else (1e-6 * M + S).tocsc()
+            norm = lambda e: np.einsum("ij,ij->i", e, (A @ e.T).T)
+            assert np.all(norm(xm - xa) <= 1e-12 * norm(xa)), kind
+            if kind == "filter":
+                f = p.cpu().numpy()
+                want = spla.splu(A).solve((M @ f.T)).T
+                assert np.all(norm(xm - want) <= 1e-12 * norm(want))
+    finally:
+        server.close()
+
+
+def test_admission_by_memory_budget():
+    server = SolverServer(hbm_budget=1 << 20)  # 1 MiB: nothing fits
+    try:
+        with pytest.raises(MemoryError):
+            server.add_mesh("cube", _mesh("cube"), max_cols=2, geodesics=False)
+        assert not server.meshes
+    finally:
+        server.close()
