@@ -146,6 +146,74 @@ def profile_kernel(cfg, K, which, reps=10):
     log(f"[bench] profile-kernel {which}: {reps} launches done")
 
 
+SERVING_MESHES = [  # name, kind, params, width
+    ("cube512", "cube", dict(subdiv=16), 512),
+    ("ico1024", "icosphere", dict(levels=5, amplitude=0.05, seed=2), 1024),
+    ("torus1024", "torus", dict(nu=128, nv=64, cu=4, cv=2, amplitude=0.05, seed=3), 1024),
+    ("ico2048", "icosphere", dict(levels=6, amplitude=0.05, seed=4), 2048),
+]
+
+
+def serving_bench(qps, n_requests, tol, seed=0):
+    """Request-loop measurement: the meshes above stay resident; requests arrive as a Poisson
+    stream at `qps`: 60 % filtering (stiffness log-uniform in [1e-4, 1e-2], 3 channels, a uniform
+    [0,1) texture), 40 % geodesics (1-8 random source texels).  Reports p50/p99 latency per class
+    and the solved columns per second."""
+    import threading
+    import torch
+    import gradientdomaintextureprocessing_amd as g
+    from gradientdomaintextureprocessing_amd.serving import SolverServer, latency_percentiles
+    t0 = time.time()
+    server = SolverServer(tol=tol, max_iters=120)
+    sizes = {}
+    for name, kind, p, W in SERVING_MESHES:
+        if kind == "cube":
+            mesh = g.TexturedMesh.cube(p["subdiv"], W, W)
+        elif kind == "icosphere":
+            mesh = g.TexturedMesh.icosphere(p["levels"], W, W, p["amplitude"], p["seed"])
+        else:
+            mesh = g.TexturedMesh.torus(p["nu"], p["nv"], p["cu"], p["cv"], W, W, 1.0, 0.35, p["amplitude"], p["seed"])
+        levels = max(4, int(np.log2(W)) - 3)
+        H = g.InitializeHierarchy(mesh, W, W, levels)
+        server.add_mesh(name, H, max_cols=8)
+        sizes[name] = H.level_info(0)["texels"]
+    log(f"[bench] serving: {len(SERVING_MESHES)} meshes resident in {time.time() - t0:.1f}s, {server.used / 2**30:.2f} GiB")
+    rng = np.random.default_rng(seed)
+    names = [m[0] for m in SERVING_MESHES]
+    plan = []
+    for i in range(n_requests):
+        n = names[rng.integers(len(names))]
+        if rng.random() < 0.6:
+            plan.append(("filter", n, float(10 ** rng.uniform(-4, -2))))
+        else:
+            plan.append(("geodesic", n, [int(v) for v in rng.integers(0, sizes[n], rng.integers(1, 9))]))
+    payloads = {n: torch.rand(3, sizes[n], dtype=torch.float64, device="cuda") for n in names}
+    for n in names:  # warm every (mesh, kind) path once: graph capture and first-use set-up
+        server.submit_filter(n, payloads[n], 1e-3).wait(300)
+        server.submit_geodesic(n, [0]).wait(300)
+    gaps = rng.exponential(1.0 / qps, n_requests)
+    handles = []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for (kind, n, a), gap in zip(plan, gaps):
+        time.sleep(gap)
+        h = server.submit_filter(n, payloads[n], a) if kind == "filter" else server.submit_geodesic(n, a)
+        handles.append((kind, n, h))
+    for _, _, h in handles:
+        h.wait(600)
+    el = time.perf_counter() - start
+    server.close()
+    cols = sum(h.columns for _, _, h in handles)
+    out = {"metric": "serving: request latency at rel-residual %g" % tol, "qps_offered": qps, "requests": n_requests,
+           "wall_s": el, "columns_per_s": cols / el, "resident_gib": server.used / 2**30,
+           "all": latency_percentiles([h for _, _, h in handles])}
+    for kind in ("filter", "geodesic"):
+        out[kind] = latency_percentiles([h for k, _, h in handles if k == kind])
+    out["filter_1024"] = latency_percentiles([h for k, n, h in handles if k == "filter" and n.endswith("1024")])
+    out["meshes"] = {n: sizes[n] for n in names}
+    print(json.dumps(out))
+
+
 def load_traffic(kernel, workload_key):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE passes, scaled
     by the calibration copy) and recorded in profiles/traffic.json; None when not measured."""
@@ -204,6 +272,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
+    ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
+    ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver shards by requests / RHS columns, "
@@ -214,6 +285,9 @@ def main():
     max_cycles = 40
     if args.profile_kernel:
         profile_kernel(args.config, K, args.profile_kernel)
+        return
+    if args.serving:
+        serving_bench(args.qps, args.requests, args.tol)
         return
     results = {}
     modes = {"mixed": ["mixed", "fp64"], "mixed-only": ["mixed"], "fp64": ["fp64"]}[args.mode]
