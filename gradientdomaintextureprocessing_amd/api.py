@@ -245,8 +245,12 @@ class MultigridSystem:
                                                    ctypes.byref(it), _stream()))
         return it.value, rel
 
-    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True):
-        _lib.check(_lib.load().gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
+    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=True):
+        """Smoothing schedule of the FP32 V-cycle, graph capture, and the outer iteration:
+        flexible CG (krylov=True) or plain iterative refinement."""
+        lib = _lib.load()
+        _lib.check(lib.gdtp_mixed_acceleration(self._h, 1 if krylov else 0))
+        _lib.check(lib.gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
 
     def mixed_deep_sweep(self, level, ncols):
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))

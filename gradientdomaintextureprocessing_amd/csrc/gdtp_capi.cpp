@@ -44,7 +44,7 @@ struct gdtp_system {
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 1;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
@@ -433,6 +433,7 @@ static MixedSystem& Mixed(gdtp_system* s) {
   }
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
+  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0;
   return m;
 }
 
@@ -446,6 +447,16 @@ int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySwee
     Mixed(s);  // build the mixed solver now (device set-up off the first solve's path)
     return GDTP_OK;
   })
+}
+
+int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg) {
+  if (!s) return Fail(GDTP_ERROR_INVALID, "null system");
+  s->krylov = flexible_cg ? 1 : 0;
+  if (s->mixed) {
+    s->mixed->krylov = s->krylov != 0;
+    InvalidateGraph(*s->mixed);
+  }
+  return GDTP_OK;
 }
 
 int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream) {

@@ -49,6 +49,13 @@ struct MixedSystem {
   double* norms = nullptr;         // device [2][kcap][64]: squared residual / rhs norm partial sums
   double* hostNorms = nullptr;     // pinned copy
   double* weights = nullptr;       // device [kcap][2]: per-column (mass, stiffness) weights of weighted solves
+  double screen = 1.0, stiff = 0.0;  // the system's own weights
+  // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
+  bool krylov = true;
+  double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *z64 = nullptr;  // [kcap][N0] each
+  double* dots = nullptr;          // [3][kcap][kSlots]: (z,q), (r,p), (p,q)
+  double *alpha = nullptr, *beta = nullptr, *pqPrev = nullptr;  // [kcap]
+  int* first = nullptr;            // first step of a solve (beta = 0)
   std::vector<void*> allocations;
   size_t bytes = 0;                // device bytes held
   // captured refinement steps, one per (K, x, b, weighted) in use (least recently used evicted)

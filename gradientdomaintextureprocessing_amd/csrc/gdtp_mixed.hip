@@ -351,7 +351,8 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __
 __global__ __launch_bounds__(256) void K64ResidualDeepW(int nLines, const int* __restrict__ lines, const double* __restrict__ coefM,
                                                         const double* __restrict__ coefS, int D, const double* __restrict__ x,
                                                         const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                        float* e0, long Nd, double* norms, int K, const double* __restrict__ w) {
+                                                        float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
+                                                        double* r64) {
   const int l = blockIdx.x;
   const int* L = lines + 5 * l;
   const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
@@ -376,6 +377,7 @@ __global__ __launch_bounds__(256) void K64ResidualDeepW(int nLines, const int* _
           sm += cm[k] * xv, ss += cs[k] * xv;
         }
         const double r = b[(long)ch * N + start + i] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
+        if (r64) r64[(long)ch * N + start + i] = r;
         rhs0[(long)ch * Nd + di] = (float)r;
         e0[(long)ch * Nd + di] = 0.0f;
         acc[cc] += r * r;
@@ -394,7 +396,8 @@ __global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __
                                                             const int* __restrict__ srpb, const int* __restrict__ scolb,
                                                             const double* __restrict__ svalb, const double* __restrict__ x,
                                                             const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                            float* e0, long Nd, double* norms, int K, const double* __restrict__ w) {
+                                                            float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
+                                                            double* r64) {
   const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
@@ -421,6 +424,7 @@ __global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __
     if (active && lane < kc) {
       const int ch = c0 + lane, g = bgi[r];
       const double rv = b[(long)ch * N + g] - (w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane));
+      if (r64) r64[(long)ch * N + g] = rv;
       rhs0[(long)ch * Nd + g2d[g]] = (float)rv;
       e0[(long)ch * Nd + g2d[g]] = 0.0f;
       sq = rv * rv;
@@ -767,6 +771,13 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
   ms.coarseY = Zeros<double>(ms, (size_t)Nc * kcap);
   ms.norms = Zeros<double>(ms, 2 * (size_t)kcap * kSlots);
   ms.weights = Zeros<double>(ms, 2 * (size_t)kcap);
+  {
+    const size_t n0 = (size_t)ms.L[0].N * kcap;
+    ms.r64 = Zeros<double>(ms, n0), ms.p64 = Zeros<double>(ms, n0), ms.q64 = Zeros<double>(ms, n0), ms.z64 = Zeros<double>(ms, n0);
+    ms.dots = Zeros<double>(ms, 3 * (size_t)kcap * kSlots);
+    ms.alpha = Zeros<double>(ms, kcap), ms.beta = Zeros<double>(ms, kcap), ms.pqPrev = Zeros<double>(ms, kcap);
+    ms.first = Zeros<int>(ms, 1);
+  }
   GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * (size_t)kcap * kSlots * sizeof(double), hipHostMallocDefault));
   GDTP_HIP(hipStreamCreateWithFlags(&ms.stream, hipStreamNonBlocking));
   (void)ds;
@@ -851,8 +862,129 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
   (void)ds;
 }
 
+// ------------------------------------------------------------------ flexible conjugate gradients
+// q = A p for per-column operators A_c = w[c][0] M + w[c][1] S (level 0; deep rows by raster line)
+__global__ __launch_bounds__(256) void K64ApplyDeepW(int nLines, const int* __restrict__ lines, const double* __restrict__ coefM,
+                                                     const double* __restrict__ coefS, int D, const double* __restrict__ p, int N, double* q,
+                                                     int K, const double* __restrict__ w) {
+  const int* L = lines + 5 * blockIdx.x;
+  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    double cm[9], cs[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d0 + i], cs[k] = coefS[(long)k * D + d0 + i];
+    for (int ch = 0; ch < K; ch++) {
+      const double* P = p + (long)ch * N;
+      double sm = 0.0, ss = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int base = k < 3 ? prev : (k < 6 ? start : next);
+        const double v = P[base + i + k % 3 - 1];
+        sm += cm[k] * v, ss += cs[k] * v;
+      }
+      q[(long)ch * N + start + i] = w[2 * ch] * sm + w[2 * ch + 1] * ss;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
+                                                         const int* __restrict__ mcold, const double* __restrict__ mvald,
+                                                         const int* __restrict__ mrpb, const int* __restrict__ mcolb,
+                                                         const double* __restrict__ mvalb, const int* __restrict__ srpd,
+                                                         const int* __restrict__ scold, const double* __restrict__ svald,
+                                                         const int* __restrict__ srpb, const int* __restrict__ scolb,
+                                                         const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q,
+                                                         int K, const double* __restrict__ w) {
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  if (r >= B) return;  // whole row groups leave together
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double am[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
+    auto rows = [&](const int* rp, const int* cl, const double* vl, bool viaBoundary, double (&acc)[4]) {
+      for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes) {
+        const double v = vl[k];
+        const int g = viaBoundary ? bgi[cl[k]] : cl[k];
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++)
+          if (qq < kc) acc[qq] += v * p[(long)(c0 + qq) * N + g];
+      }
+    };
+    rows(mrpb, mcolb, mvalb, true, am);
+    rows(mrpd, mcold, mvald, false, am);
+    rows(srpb, scolb, svalb, true, as);
+    rows(srpd, scold, svald, false, as);
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) am[qq] = RowSum(am[qq]), as[qq] = RowSum(as[qq]);
+    if (lane < kc) {
+      const int ch = c0 + lane;
+      q[(long)ch * N + bgi[r]] = w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane);
+    }
+  }
+}
+
+// column dot products a_c . b_c into kSlots partial sums per column (blockIdx.y = column)
+__global__ __launch_bounds__(256) void K64Dot(int N, const double* __restrict__ a, const double* __restrict__ b, double* out) {
+  const int ch = blockIdx.y;
+  double v = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) v += a[(long)ch * N + i] * b[(long)ch * N + i];
+  BlockAddSquares(v, out + (long)ch * kSlots);
+}
+
+__device__ __forceinline__ double SlotSum(const double* s) {
+  double t = 0.0;
+  for (int q = 0; q < kSlots; q++) t += s[q];
+  return t;
+}
+
+// beta_c = (z, q_prev) / (p_prev, q_prev); zero on the first step of a solve (flag set by the host)
+__global__ void KFcgBeta(int K, const double* __restrict__ zq, const double* __restrict__ pqPrev, const int* __restrict__ first, double* beta) {
+  const int c = threadIdx.x;
+  if (c >= K) return;
+  beta[c] = (*first || !(pqPrev[c] > 0)) ? 0.0 : SlotSum(zq + (long)c * kSlots) / pqPrev[c];
+}
+
+__global__ void KFcgDirection(long n, int N, const double* __restrict__ z, const double* __restrict__ beta, double* p) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double bt = beta[t / N];
+  p[t] = bt == 0.0 ? z[t] : z[t] - bt * p[t];
+}
+
+// alpha_c = (r, p) / (p, q); keeps (p, q) for the next beta and clears the first-step flag
+__global__ void KFcgAlpha(int K, const double* __restrict__ rp, const double* __restrict__ pq, double* alpha, double* pqPrev, int* first) {
+  const int c = threadIdx.x;
+  if (c < K) {
+    const double d = SlotSum(pq + (long)c * kSlots);
+    alpha[c] = d > 0 ? SlotSum(rp + (long)c * kSlots) / d : 0.0;
+    pqPrev[c] = d;
+  }
+  if (c == 0) *first = 0;
+}
+
+// x += alpha p, r -= alpha q; the new residual into the FP32 correction rhs (e zeroed), its norms
+__global__ __launch_bounds__(256) void KFcgUpdate(int N, const int* __restrict__ g2d, const double* __restrict__ alpha,
+                                                  const double* __restrict__ p, const double* __restrict__ q, double* x, double* r,
+                                                  float* rhs0, float* e0, long Nd, double* norms) {
+  const int ch = blockIdx.y;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0;
+  if (i < N) {
+    const long o = (long)ch * N + i;
+    const double al = alpha[ch];
+    x[o] += al * p[o];
+    const double rv = r[o] - al * q[o];
+    r[o] = rv;
+    const long d = (long)ch * Nd + g2d[i];
+    rhs0[d] = (float)rv;
+    e0[d] = 0.0f;
+    a = rv * rv;
+  }
+  BlockAddSquares(a, norms + (long)ch * kSlots);
+}
+
 // residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
-static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s) {
+static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s,
+                          double* r64 = nullptr) {
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
@@ -860,11 +992,11 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
     if (L0.nRaster > 0)
       K64ResidualDeepW<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, M.coefRaw, S.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
-                                                  ms.norms, K, ms.weights);
+                                                  ms.norms, K, ms.weights, r64);
     if (L0.B > 0)
       K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(
           L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr,
-          S.bb.col, S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights);
+          S.bb.col, S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64);
     return;
   }
   if (L0.nRaster > 0)
@@ -872,6 +1004,36 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   if (L0.B > 0)
     K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
                                                                           L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+}
+
+// One flexible-CG step after the V-cycle left z = V(r) in the level-0 FP32 correction:
+// beta = (z, q_prev)/(p_prev, q_prev), p = z - beta p, q = A p, alpha = (r, p)/(p, q),
+// x += alpha p, r -= alpha q (all scalars stay on the device).
+static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
+  DevLevel& L0 = ds.L[0];
+  DevMixLevel& d = ms.L[0];
+  const int N = d.N;
+  const long nk = (long)N * K;
+  const size_t slab = (size_t)ms.kcap * kSlots;
+  double *zq = ms.dots, *rp = ms.dots + slab, *pq = ms.dots + 2 * slab;
+  const unsigned gb = std::min(NBlk(N, 256), 1024u), cb = (unsigned)((K + 63) / 64 * 64);
+  KMixZeroD<<<NBlk(3 * (long)slab, 256), 256, 0, s>>>((int)(3 * slab), ms.dots);
+  KMixGather<<<NBlk(nk, 256), 256, 0, s>>>(N, d.g2d, d.e, d.Nd, ms.z64, K);
+  K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.z64, ms.q64, zq);
+  KFcgBeta<<<1, cb, 0, s>>>(K, zq, ms.pqPrev, ms.first, ms.beta);
+  KFcgDirection<<<NBlk(nk, 256), 256, 0, s>>>(nk, N, ms.z64, ms.beta, ms.p64);
+  const DevLevel &M = ds.mass0, &S = ds.stiff0;
+  if (L0.nRaster > 0)
+    K64ApplyDeepW<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, M.coefRaw, S.coefRaw, L0.D, ms.p64, N, ms.q64, K, ms.weights);
+  if (L0.B > 0)
+    K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
+                                                                       M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
+                                                                       S.bb.val, ms.p64, N, ms.q64, K, ms.weights);
+  K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.r64, ms.p64, rp);
+  K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.p64, ms.q64, pq);
+  KFcgAlpha<<<1, cb, 0, s>>>(K, rp, pq, ms.alpha, ms.pqPrev, ms.first);
+  KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  KFcgUpdate<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, d.g2d, ms.alpha, ms.p64, ms.q64, x, ms.r64, d.rhs, d.e, d.Nd, ms.norms);
 }
 
 static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
@@ -884,6 +1046,14 @@ static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hip
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
                hipStream_t s, bool useGraph, const double* weights) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
+  if (K > 1024) throw std::runtime_error("at most 1024 columns per solve");
+  const bool krylov = ms.krylov;
+  std::vector<double> own;
+  if (krylov && !weights) {  // flexible CG applies A through the level-0 mass and stiffness
+    own.resize(2 * (size_t)K);
+    for (int k = 0; k < K; k++) own[2 * k] = ms.screen, own[2 * k + 1] = ms.stiff;
+    weights = own.data();
+  }
   const bool weighted = weights != nullptr;
   if (weighted) {
     for (int k = 0; k < 2 * K; k++)
@@ -896,7 +1066,8 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
   K64SquaredNorm<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, b, bNorms);
-  OuterResidual(ds, ms, x, b, K, weighted, s);
+  if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
+  OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr);
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
   auto slotSum = [&](const double* p) {
@@ -918,8 +1089,12 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   // one refinement step = V-cycle on the current residual, update, new residual
   auto step = [&](hipStream_t st) {
     MixedVCycle(ds, ms, K, st);
-    OuterUpdate(ds, ms, x, K, st);
-    OuterResidual(ds, ms, x, b, K, weighted, st);
+    if (krylov) {
+      FcgStep(ds, ms, x, K, st);
+    } else {
+      OuterUpdate(ds, ms, x, K, st);
+      OuterResidual(ds, ms, x, b, K, weighted, st);
+    }
     GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));
   };
   MixedSystem::GraphEntry* ge = nullptr;
