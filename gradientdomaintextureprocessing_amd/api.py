@@ -245,7 +245,7 @@ class MultigridSystem:
                                                    ctypes.byref(it), _stream()))
         return it.value, rel
 
-    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=True):
+    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=False):
         """Smoothing schedule of the FP32 V-cycle, graph capture, and the outer iteration:
         flexible CG (krylov=True) or plain iterative refinement."""
         lib = _lib.load()
@@ -398,19 +398,36 @@ class LineIntegralConvolution:
         self.lic = UpdateLinearSystem(self.anisotropic, convolution_weight, 1.0, max_cols=max_cols)
         self.sharpen = UpdateLinearSystem(self.embedding, stiffness_weight, 1.0, max_cols=max_cols)
 
-    def convolve(self, noise, tol=1e-6, mode="mixed", max_iters=80):
-        """(x, y): the anisotropically diffused noise and its sharpened version ([K, N] FP64)."""
+    def convolve(self, noise, passes=4, tol=None, mode="mixed", max_iters=200):
+        """(x, y): the anisotropically diffused noise and its sharpened version ([K, N] FP64).
+
+        Default (tol=None): the reference's schedule (LineIntegralConvolution.cpp:503-527, 707),
+        `passes` rounds of one reference-schedule V-cycle on the LIC system, the sharpening rhs
+        from the current x, one V-cycle on the sharpening system, from x = y = 0.  Strongly
+        anisotropic diffusion converges slowly under geometric multigrid, so the app runs a
+        fixed number of rounds.  With tol: both systems solved to that relative residual
+        (flexible CG around the mixed V-cycle)."""
         import torch
         b = torch.empty_like(noise)
         self.lic.apply(OPERATOR_MASS, noise, b)
-        x = torch.zeros_like(noise)
-        self.lic.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        x, y = torch.zeros_like(noise), torch.zeros_like(noise)
         m, s = torch.empty_like(x), torch.empty_like(x)
-        self.sharpen.apply(OPERATOR_MASS, x, m)
-        self.sharpen.apply(OPERATOR_STIFFNESS, x, s)
-        rhs = m + (self.sharpening * self.stiffness) * s
-        y = x.clone()
-        self.sharpen.solve(y, rhs, tol=tol, max_iters=max_iters, mode=mode)
+
+        def sharpen_rhs():
+            self.sharpen.apply(OPERATOR_MASS, x, m)
+            self.sharpen.apply(OPERATOR_STIFFNESS, x, s)
+            return (m + (self.sharpening * self.stiffness) * s).contiguous()
+
+        if tol is None:
+            for _ in range(passes):
+                self.lic.vcycle(x, b)
+                self.sharpen.vcycle(y, sharpen_rhs())
+            return x, y
+        for system in (self.lic, self.sharpen):
+            system.configure_mixed(krylov=True)
+        self.lic.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        y.copy_(x)
+        self.sharpen.solve(y, sharpen_rhs(), tol=tol, max_iters=max_iters, mode=mode)
         return x, y
 
 

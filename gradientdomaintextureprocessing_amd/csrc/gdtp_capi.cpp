@@ -44,7 +44,7 @@ struct gdtp_system {
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 1;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;

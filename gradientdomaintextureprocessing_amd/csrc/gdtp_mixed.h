@@ -51,9 +51,9 @@ struct MixedSystem {
   double* weights = nullptr;       // device [kcap][2]: per-column (mass, stiffness) weights of weighted solves
   double screen = 1.0, stiff = 0.0;  // the system's own weights
   // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
-  bool krylov = true;
+  bool krylov = false;
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *z64 = nullptr;  // [kcap][N0] each
-  double* dots = nullptr;          // [3][kcap][kSlots]: (z,q), (r,p), (p,q)
+  double* dots = nullptr;          // [3][kcap][1024]: per-block partial sums of (z,q), (r,p), (p,q)
   double *alpha = nullptr, *beta = nullptr, *pqPrev = nullptr;  // [kcap]
   int* first = nullptr;            // first step of a solve (beta = 0)
   std::vector<void*> allocations;

@@ -24,6 +24,7 @@ namespace gdtp {
 
 static constexpr int kTA = 64, kTB = 4;  // tile of the sub-grid per workgroup: 64 x 4 (256 threads)
 static constexpr int kDenseCoarseMax = 6000;  // coarsest levels up to this size use a dense inverse
+static constexpr int kDotBlocks = 1024;       // partial sums per column of the flexible-CG dot products
 static inline unsigned NBlk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 __device__ __forceinline__ long DenseIdx(const MixChart& c, long plane, int i, int j) {
@@ -774,7 +775,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
   {
     const size_t n0 = (size_t)ms.L[0].N * kcap;
     ms.r64 = Zeros<double>(ms, n0), ms.p64 = Zeros<double>(ms, n0), ms.q64 = Zeros<double>(ms, n0), ms.z64 = Zeros<double>(ms, n0);
-    ms.dots = Zeros<double>(ms, 3 * (size_t)kcap * kSlots);
+    ms.dots = Zeros<double>(ms, 3 * (size_t)kcap * kDotBlocks);
     ms.alpha = Zeros<double>(ms, kcap), ms.beta = Zeros<double>(ms, kcap), ms.pqPrev = Zeros<double>(ms, kcap);
     ms.first = Zeros<int>(ms, 1);
   }
@@ -922,25 +923,31 @@ __global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __res
   }
 }
 
-// column dot products a_c . b_c into kSlots partial sums per column (blockIdx.y = column)
+// column dot products a_c . b_c: one partial sum per block (out[c][blockIdx.x]), no atomics, so
+// that the flexible-CG scalars are reproducible run to run (blockIdx.y = column)
 __global__ __launch_bounds__(256) void K64Dot(int N, const double* __restrict__ a, const double* __restrict__ b, double* out) {
   const int ch = blockIdx.y;
   double v = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) v += a[(long)ch * N + i] * b[(long)ch * N + i];
-  BlockAddSquares(v, out + (long)ch * kSlots);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __shared__ double part[4];
+  if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(long)ch * kDotBlocks + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-__device__ __forceinline__ double SlotSum(const double* s) {
+__device__ __forceinline__ double DotSum(const double* s, int n) {
   double t = 0.0;
-  for (int q = 0; q < kSlots; q++) t += s[q];
+  for (int q = 0; q < n; q++) t += s[q];
   return t;
 }
 
 // beta_c = (z, q_prev) / (p_prev, q_prev); zero on the first step of a solve (flag set by the host)
-__global__ void KFcgBeta(int K, const double* __restrict__ zq, const double* __restrict__ pqPrev, const int* __restrict__ first, double* beta) {
+__global__ void KFcgBeta(int K, int nb, const double* __restrict__ zq, const double* __restrict__ pqPrev, const int* __restrict__ first,
+                         double* beta) {
   const int c = threadIdx.x;
   if (c >= K) return;
-  beta[c] = (*first || !(pqPrev[c] > 0)) ? 0.0 : SlotSum(zq + (long)c * kSlots) / pqPrev[c];
+  beta[c] = (*first || !(pqPrev[c] > 0)) ? 0.0 : DotSum(zq + (long)c * kDotBlocks, nb) / pqPrev[c];
 }
 
 __global__ void KFcgDirection(long n, int N, const double* __restrict__ z, const double* __restrict__ beta, double* p) {
@@ -951,11 +958,12 @@ __global__ void KFcgDirection(long n, int N, const double* __restrict__ z, const
 }
 
 // alpha_c = (r, p) / (p, q); keeps (p, q) for the next beta and clears the first-step flag
-__global__ void KFcgAlpha(int K, const double* __restrict__ rp, const double* __restrict__ pq, double* alpha, double* pqPrev, int* first) {
+__global__ void KFcgAlpha(int K, int nb, const double* __restrict__ rp, const double* __restrict__ pq, double* alpha, double* pqPrev,
+                          int* first) {
   const int c = threadIdx.x;
   if (c < K) {
-    const double d = SlotSum(pq + (long)c * kSlots);
-    alpha[c] = d > 0 ? SlotSum(rp + (long)c * kSlots) / d : 0.0;
+    const double d = DotSum(pq + (long)c * kDotBlocks, nb);
+    alpha[c] = d > 0 ? DotSum(rp + (long)c * kDotBlocks, nb) / d : 0.0;
     pqPrev[c] = d;
   }
   if (c == 0) *first = 0;
@@ -1014,13 +1022,12 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   DevMixLevel& d = ms.L[0];
   const int N = d.N;
   const long nk = (long)N * K;
-  const size_t slab = (size_t)ms.kcap * kSlots;
+  const size_t slab = (size_t)ms.kcap * kDotBlocks;
   double *zq = ms.dots, *rp = ms.dots + slab, *pq = ms.dots + 2 * slab;
-  const unsigned gb = std::min(NBlk(N, 256), 1024u), cb = (unsigned)((K + 63) / 64 * 64);
-  KMixZeroD<<<NBlk(3 * (long)slab, 256), 256, 0, s>>>((int)(3 * slab), ms.dots);
+  const unsigned gb = std::min(NBlk(N, 256), (unsigned)kDotBlocks), cb = (unsigned)((K + 63) / 64 * 64);
   KMixGather<<<NBlk(nk, 256), 256, 0, s>>>(N, d.g2d, d.e, d.Nd, ms.z64, K);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.z64, ms.q64, zq);
-  KFcgBeta<<<1, cb, 0, s>>>(K, zq, ms.pqPrev, ms.first, ms.beta);
+  KFcgBeta<<<1, cb, 0, s>>>(K, (int)gb, zq, ms.pqPrev, ms.first, ms.beta);
   KFcgDirection<<<NBlk(nk, 256), 256, 0, s>>>(nk, N, ms.z64, ms.beta, ms.p64);
   const DevLevel &M = ds.mass0, &S = ds.stiff0;
   if (L0.nRaster > 0)
@@ -1031,7 +1038,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
                                                                        S.bb.val, ms.p64, N, ms.q64, K, ms.weights);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.r64, ms.p64, rp);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.p64, ms.q64, pq);
-  KFcgAlpha<<<1, cb, 0, s>>>(K, rp, pq, ms.alpha, ms.pqPrev, ms.first);
+  KFcgAlpha<<<1, cb, 0, s>>>(K, (int)gb, rp, pq, ms.alpha, ms.pqPrev, ms.first);
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
   KFcgUpdate<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, d.g2d, ms.alpha, ms.p64, ms.q64, x, ms.r64, d.rhs, d.e, d.Nd, ms.norms);
 }

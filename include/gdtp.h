@@ -131,9 +131,9 @@ int gdtp_solve_weighted(gdtp_system* s, double* x, const double* b, int ncols, c
  * the damping of their block update, hipGraph capture of the refinement step (1) or plain
  * launches (0).  Builds the mixed solver's device data if it does not exist yet. */
 int gdtp_mixed_configure(gdtp_system* s, int nu_pre, int nu_post, int boundary_sweeps, double omega, int use_graph);
-/* Outer iteration of the mixed solver: flexible conjugate gradients preconditioned by the FP32
- * V-cycle (1, the default: robust for anisotropic and nearly singular operators, e.g. LIC and
- * geodesic distance) or plain iterative refinement x += V(b - A x) (0). */
+/* Outer iteration of the mixed solver: plain iterative refinement x += V(b - A x) (0, the
+ * default: fewest passes over the operator per iteration) or flexible conjugate gradients
+ * preconditioned by the FP32 V-cycle (1: fewer iterations on harder operators). */
 int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */

@@ -105,7 +105,7 @@ def test_weighted_columns_match_direct_solves():
     M, Kst = mass_matrix(H, L0), stiffness_matrix(H, L0)
     weights = np.array([[1.0, 1e-4], [1.0, 1e-3], [1.0, 1e-2], [0.5, 3e-3]])
     f = np.random.default_rng(8).random((4, L0.N))
-    b = (M @ f.T).T
+    b = np.ascontiguousarray((M @ f.T).T)
     x = torch.zeros((4, L0.N), dtype=torch.float64, device="cuda")
     its, rel = S.solve_weighted(x, torch.from_numpy(b).cuda(), weights, tol=1e-10, max_iters=60)
     print(f"[weighted] {its} iterations, rel res {rel}", flush=True)
@@ -116,3 +116,24 @@ def test_weighted_columns_match_direct_solves():
         want = spla.splu(A).solve(b[k])
         e = xs[k] - want
         assert e @ (A @ e) <= 1e-14 * (want @ (A @ want))
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_flexible_cg_matches_direct_and_replays_exactly(case, graph):
+    name, H, S, L, A = case
+    _, b = noise_texture_rhs(H, L[0], K=3, seed=6)
+    want = spla.splu(A).solve(b.T).T
+    S.configure_mixed(krylov=True, use_graph=graph)
+    try:
+        outs = []
+        for rep in range(2):
+            x = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+            its, rel = S.solve(x, torch.from_numpy(b).cuda(), tol=1e-10, max_iters=60, mode="mixed")
+            outs.append(x.cpu().numpy())
+        print(f"[{name}] flexible CG: {its} iterations, rel res {rel.max():.2e}", flush=True)
+        assert rel.max() <= 1e-10
+        assert np.array_equal(outs[0], outs[1])  # deterministic reductions: identical repeats
+        e = outs[0] - want
+        assert np.all(np.einsum("ij,ij->i", e, (A @ e.T).T) <= 1e-14 * np.einsum("ij,ij->i", want, (A @ want.T).T))
+    finally:
+        S.configure_mixed()
