@@ -78,8 +78,8 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 // Solve A x = b (level-0 compact FP64 vectors, K columns) until every column's relative residual
 // is <= tol or maxIter refinement steps; x is used as the initial guess.  Returns the iterations
 // run; relRes (host, K entries) receives the final relative residuals.  With `weights` (host,
-// K pairs), column c solves (w[c][0] M + w[c][1] S) x = b instead, the system's own V-cycle
-// serving as the preconditioner of every column.
+// K pairs), column c solves (w[c][0] M + w[c][1] S) x = b instead, by flexible CG with the
+// system's own V-cycle as the preconditioner of every column.
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
                hipStream_t s, bool useGraph, const double* weights = nullptr);
 

@@ -1054,7 +1054,9 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
                hipStream_t s, bool useGraph, const double* weights) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
   if (K > 1024) throw std::runtime_error("at most 1024 columns per solve");
-  const bool krylov = ms.krylov;
+  // weighted columns run flexible CG: their operators differ from the one the V-cycle was built
+  // for, and plain refinement x += V(r) diverges once the weights are a few times apart
+  const bool krylov = ms.krylov || weights != nullptr;
   std::vector<double> own;
   if (krylov && !weights) {  // flexible CG applies A through the level-0 mass and stiffness
     own.resize(2 * (size_t)K);

@@ -119,9 +119,10 @@ int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rh
  * (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached first. */
 int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int max_iters, int mode, double* rel_res,
                int* iterations, void* stream);
-/* Parametric solve (mixed precision): column c solves (w[2c] M + w[2c+1] S) x = b to `tol`, the
- * FP64 residual using the level-0 mass and stiffness operators with each column's weights and the
- * system's own FP32 V-cycle (built for its create/update weights) as preconditioner; weights is
+/* Parametric solve (mixed precision): column c solves (w[2c] M + w[2c+1] S) x = b to `tol` by
+ * flexible CG, the FP64 operator applied from the level-0 mass and stiffness with each column's
+ * weights and the system's own FP32 V-cycle (built for its create/update weights) as
+ * preconditioner; weights is
  * a host array of ncols (screen, stiffness) pairs, screen > 0.  No refactorisation: the per-request
  * operator update of the apps' sliders (UpdateLinearSystem, UpdateCoefficients.inl:463-513) and of
  * batched requests with different weights. */
