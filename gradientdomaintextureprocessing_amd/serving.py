@@ -84,21 +84,42 @@ class SolverServer:
         self.stop = threading.Event()
 
     # -- residency
-    def add_mesh(self, name, hierarchy, **kw):
+    def add_mesh(self, name, hierarchy, warm=True, **kw):
         m = ResidentMesh(name, hierarchy, **kw)
-        for s in m.systems.values():  # build the mixed solvers now: first requests should not pay for it
-            s.configure_mixed()
+        for s in list(m.systems.values()) + ([m.distance] if m.distance else []):
+            s.configure_mixed()  # builds the mixed solver now: first requests should not pay for it
         mem = m.memory()
         if self.hbm_budget is not None and self.used + mem > self.hbm_budget:
             del m
             raise MemoryError(f"mesh {name} needs {mem / 2**30:.2f} GiB; {(self.hbm_budget - self.used) / 2**30:.2f} GiB left")
         self.used += mem
+        if warm:
+            self._warm(m)
         self.meshes[name] = m
         self.queues[name] = queue.Queue()
         t = threading.Thread(target=self._work, args=(name,), daemon=True)
         self.workers[name] = t
         t.start()
         return m
+
+    def _warm(self, m):
+        """Capture the refinement graphs of every batch width the worker will hand each system (the
+        per-mesh buffers make the pointers of a width fixed), so that no request pays a capture."""
+        torch = self.torch
+        buf = self._buffers(m)
+        g = torch.Generator(device="cuda").manual_seed(0)
+        for K in range(1, m.max_cols + 1):
+            x, b = buf["x"][:K], buf["b"][:K]
+            for a, system in m.systems.items():
+                b.copy_(torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g))
+                x.zero_()
+                system.solve_weighted(x, b, [[1.0, a]] * K, tol=1e-2, max_iters=self.max_iters)
+            if m.distance is not None:
+                t0, t1 = buf["t0"][:K], buf["t1"][:K]
+                t0.copy_(torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g))
+                t1.zero_()
+                m.distance.solve(t1, t0, tol=1e-2, max_iters=self.max_iters, mode="mixed")
+        torch.cuda.synchronize()
 
     # -- requests
     def submit_filter(self, mesh, texels, stiffness, sharpness=0.0):
