@@ -116,7 +116,9 @@ class SolverServer:
                 system.solve_weighted(x, b, [[1.0, a]] * K, tol=1e-2, max_iters=self.max_iters)
             if m.distance is not None:
                 t0, t1 = buf["t0"][:K], buf["t1"][:K]
+                # like a divergence: orthogonal to constants (the near-null mode of 1e-6 M + S)
                 t0.copy_(torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g))
+                t0 -= t0.mean(dim=1, keepdim=True)
                 t1.zero_()
                 m.distance.solve(t1, t0, tol=1e-2, max_iters=self.max_iters, mode="mixed")
         torch.cuda.synchronize()
