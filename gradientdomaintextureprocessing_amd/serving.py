@@ -8,11 +8,12 @@ Each resident mesh keeps its hierarchy, its multigrid systems and their factoris
   - "geodesic": heat-method distances from source texels (Geodesics.cpp:508-550), one column per
                 source.
 
-A worker per mesh forms batches from its queue as slots free up: requests are taken in arrival
-order while their columns fit the system's capacity, every column keeping its own operator
-weights (parametric solves share the bucket's V-cycle as preconditioner), so a request that
-arrives while others run joins the next batch.  Workers of different meshes run concurrently on
-their own HIP streams.  Meshes are admitted against an HBM budget.  Results are the same alone
+Each mesh has two worker lanes, filtering and geodesics, with their own systems, buffers and HIP
+streams, so that short filtering requests never queue behind geodesic batches.  A lane forms
+batches from its queue as slots free up: requests are taken in arrival order while their columns
+fit the system's capacity, every column keeping its own operator weights (parametric solves share
+the bucket's V-cycle as preconditioner), so a request that arrives while others run joins the
+next batch.  Lanes of all meshes run concurrently.  Meshes are admitted against an HBM budget.  Results are the same alone
 and in a batch up to the requested tolerance: every column is solved to it.
 """
 import math
@@ -48,22 +49,26 @@ class Request:
 
 
 class ResidentMesh:
-    """Hierarchy and systems of one mesh: filtering buckets (M + a_b S) and, for geodesics, the
-    distance system (1e-6 M + S)."""
+    """Hierarchy and systems of one mesh: filtering buckets (M + a_b S) and, for geodesics, a heat
+    system (M + t S) and the distance system (1e-6 M + S)."""
 
     def __init__(self, name, hierarchy, buckets=(1e-4, 1e-3, 1e-2), max_cols=8, geodesics=True, heat_time=1e-3):
         self.name, self.hierarchy, self.max_cols, self.heat_time = name, hierarchy, max_cols, heat_time
         self.buckets = sorted(buckets)
         self.systems = {a: api.UpdateLinearSystem(hierarchy, a, 1.0, max_cols=max_cols) for a in self.buckets}
+        self.heat = api.UpdateLinearSystem(hierarchy, heat_time, 1.0, max_cols=max_cols) if geodesics else None
         self.distance = api.UpdateLinearSystem(hierarchy, 1.0, 1e-6, max_cols=max_cols) if geodesics else None
         self.N = hierarchy.level_info(0)["texels"]
+
+    def all_systems(self):
+        return list(self.systems.values()) + [s for s in (self.heat, self.distance) if s is not None]
 
     def bucket(self, a):
         return min(self.buckets, key=lambda b: abs(math.log(a) - math.log(b)))
 
     def memory(self):
         total = 0
-        for s in list(self.systems.values()) + ([self.distance] if self.distance else []):
+        for s in self.all_systems():
             r, m = s.memory()
             total += r + m
         return total
@@ -86,7 +91,7 @@ class SolverServer:
     # -- residency
     def add_mesh(self, name, hierarchy, warm=True, **kw):
         m = ResidentMesh(name, hierarchy, **kw)
-        for s in list(m.systems.values()) + ([m.distance] if m.distance else []):
+        for s in m.all_systems():
             s.configure_mixed()  # builds the mixed solver now: first requests should not pay for it
         mem = m.memory()
         if self.hbm_budget is not None and self.used + mem > self.hbm_budget:
@@ -96,28 +101,34 @@ class SolverServer:
         if warm:
             self._warm(m)
         self.meshes[name] = m
-        self.queues[name] = queue.Queue()
-        t = threading.Thread(target=self._work, args=(name,), daemon=True)
-        self.workers[name] = t
-        t.start()
+        for lane in ("filter", "geodesic"):
+            self.queues[(name, lane)] = queue.Queue()
+            t = threading.Thread(target=self._work, args=(name, lane), daemon=True)
+            self.workers[(name, lane)] = t
+            t.start()
         return m
 
     def _warm(self, m):
-        """Capture the refinement graphs of every batch width the worker will hand each system (the
-        per-mesh buffers make the pointers of a width fixed), so that no request pays a capture."""
+        """Capture the refinement graphs of every batch width each lane will hand its systems (the
+        per-lane buffers make the pointers of a width fixed), so that no request pays a capture."""
         torch = self.torch
-        buf = self._buffers(m)
         g = torch.Generator(device="cuda").manual_seed(0)
+        rand = lambda K: torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g)
+        fb = self._buffers(m, "filter")
         for K in range(1, m.max_cols + 1):
-            x, b = buf["x"][:K], buf["b"][:K]
+            x, b = fb["x"][:K], fb["b"][:K]
             for a, system in m.systems.items():
-                b.copy_(torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g))
+                b.copy_(rand(K))
                 x.zero_()
                 system.solve_weighted(x, b, [[1.0, a]] * K, tol=1e-2, max_iters=self.max_iters)
-            if m.distance is not None:
-                t0, t1 = buf["t0"][:K], buf["t1"][:K]
+            if m.heat is not None:
+                gb = self._buffers(m, "geodesic")
+                x, b, t0, t1 = gb["x"][:K], gb["b"][:K], gb["t0"][:K], gb["t1"][:K]
+                b.copy_(rand(K))
+                x.zero_()
+                m.heat.solve(x, b, tol=1e-2, max_iters=self.max_iters, mode="mixed")
                 # like a divergence: orthogonal to constants (the near-null mode of 1e-6 M + S)
-                t0.copy_(torch.rand(K, m.N, dtype=torch.float64, device="cuda", generator=g))
+                t0.copy_(rand(K))
                 t0 -= t0.mean(dim=1, keepdim=True)
                 t1.zero_()
                 m.distance.solve(t1, t0, tol=1e-2, max_iters=self.max_iters, mode="mixed")
@@ -143,7 +154,7 @@ class SolverServer:
             raise ValueError(f"a request needs 1..{m.max_cols} columns")
         if r.kind == "geodesic" and m.distance is None:
             raise ValueError("mesh was added without geodesics")
-        self.queues[r.mesh].put(r)
+        self.queues[(r.mesh, r.kind)].put(r)
 
     def close(self):
         self.stop.set()
@@ -153,10 +164,10 @@ class SolverServer:
             t.join(timeout=60)
 
     # -- the per-mesh loop
-    def _work(self, name):
+    def _work(self, name, lane):
         torch = self.torch
         m = self.meshes[name]
-        q = self.queues[name]
+        q = self.queues[(name, lane)]
         stream = torch.cuda.Stream()
         pending = []
         while not self.stop.is_set():
@@ -178,7 +189,7 @@ class SolverServer:
             pending = rest
             with torch.cuda.stream(stream):
                 try:
-                    self._run(m, batch)
+                    self._run(m, batch, lane)
                 except Exception as e:  # a failed batch fails its requests, not the server
                     for r in batch:
                         r.error = e
@@ -200,19 +211,21 @@ class SolverServer:
                 rest.append(r)
         return batch, rest
 
-    def _buffers(self, m):
-        """Per-mesh device vectors of max_cols columns, reused by every batch: the first K
+    def _buffers(self, m, lane):
+        """Per-lane device vectors of max_cols columns, reused by every batch: the first K
         columns of a buffer are contiguous, so a batch of K columns always hands the solver the
         same pointers and replays its captured refinement step."""
         if not hasattr(m, "buf"):
+            m.buf = {}
+        if lane not in m.buf:
             torch = self.torch
             z = lambda: torch.zeros(m.max_cols, m.N, dtype=torch.float64, device="cuda")
-            m.buf = dict(x=z(), b=z(), t0=z(), t1=z())
-        return m.buf
+            m.buf[lane] = dict(x=z(), b=z(), t0=z(), t1=z())
+        return m.buf[lane]
 
-    def _run(self, m, batch):
+    def _run(self, m, batch, lane):
         torch = self.torch
-        buf = self._buffers(m)
+        buf = self._buffers(m, lane)
         K = sum(r.columns for r in batch)
         x, b, t0, t1 = (buf[k][:K] for k in ("x", "b", "t0", "t1"))
         if batch[0].kind == "filter":
@@ -228,13 +241,13 @@ class SolverServer:
             its, _ = system.solve_weighted(x, b, w, tol=self.tol, max_iters=self.max_iters)
             out = x
         else:
-            system = m.systems[m.bucket(m.heat_time)]
+            system = m.heat
             src = torch.as_tensor([s for r in batch for s in r.payload], device="cuda")
             rows = torch.arange(K, device="cuda")
             b.zero_()
             b[rows, src] = 1.0
             x.zero_()
-            its, _ = system.solve_weighted(x, b, [[1.0, m.heat_time]] * K, tol=self.tol, max_iters=self.max_iters)
+            its, _ = system.solve(x, b, tol=self.tol, max_iters=self.max_iters, mode="mixed")
             system.geodesic_divergence(x, t0)
             t1.zero_()
             its2, _ = m.distance.solve(t1, t0, tol=self.tol, max_iters=self.max_iters, mode="mixed")
