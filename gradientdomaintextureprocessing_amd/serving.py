@@ -13,7 +13,9 @@ streams, so that short filtering requests never queue behind geodesic batches.  
 batches from its queue as slots free up: requests are taken in arrival order while their columns
 fit the system's capacity, every column keeping its own operator weights (parametric solves share
 the bucket's V-cycle as preconditioner), so a request that arrives while others run joins the
-next batch.  Lanes of all meshes run concurrently.  Meshes are admitted against an HBM budget.  Results are the same alone
+next batch.  Lanes of all meshes run concurrently; a geodesic lane holds its next batch back
+while any filtering batch runs (at most `yield_ms`), a host-side priority for the short requests
+(HIP stream priorities are not used).  Meshes are admitted against an HBM budget.  Results are the same alone
 and in a batch up to the requested tolerance: every column is solved to it.
 """
 import math
@@ -78,10 +80,12 @@ class SolverServer:
     """Continuous-batching solve server (see the module docstring).  tol: relative residual per
     column; hbm_budget: bytes of device memory the resident systems may hold (None: no limit)."""
 
-    def __init__(self, tol=1e-6, hbm_budget=None, max_iters=80):
+    def __init__(self, tol=1e-6, hbm_budget=None, max_iters=80, yield_ms=25.0):
         import torch
         self.torch = torch
-        self.tol, self.hbm_budget, self.max_iters = tol, hbm_budget, max_iters
+        self.tol, self.hbm_budget, self.max_iters, self.yield_ms = tol, hbm_budget, max_iters, yield_ms
+        self.filters_running = 0
+        self.running_lock = threading.Condition()
         self.meshes = {}
         self.queues = {}
         self.workers = {}
@@ -187,6 +191,12 @@ class SolverServer:
                 pending.append(r)
             batch, rest = self._form_batch(m, pending)
             pending = rest
+            if lane == "geodesic":  # let running filtering batches finish first (bounded wait)
+                with self.running_lock:
+                    self.running_lock.wait_for(lambda: self.filters_running == 0, timeout=self.yield_ms / 1e3)
+            else:
+                with self.running_lock:
+                    self.filters_running += 1
             with torch.cuda.stream(stream):
                 try:
                     self._run(m, batch, lane)
@@ -194,6 +204,10 @@ class SolverServer:
                     for r in batch:
                         r.error = e
                         r.done.set()
+            if lane == "filter":
+                with self.running_lock:
+                    self.filters_running -= 1
+                    self.running_lock.notify_all()
 
     @staticmethod
     def _form_batch(m, pending):
