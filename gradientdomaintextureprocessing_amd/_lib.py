@@ -45,6 +45,7 @@ SIGNATURES = {
     "gdtp_geodesic_divergence": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_mixed_configure": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int]),
+    "gdtp_mixed_cycles": (c_int, [c_void_p, c_int]),
     "gdtp_mixed_acceleration": (c_int, [c_void_p, c_int]),
     "gdtp_mixed_deep_sweep": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "gdtp_system_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),

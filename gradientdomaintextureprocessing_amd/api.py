@@ -245,11 +245,13 @@ class MultigridSystem:
                                                    ctypes.byref(it), _stream()))
         return it.value, rel
 
-    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=False):
-        """Smoothing schedule of the FP32 V-cycle, graph capture, and the outer iteration:
-        flexible CG (krylov=True) or plain iterative refinement."""
+    def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=False, cycles=0):
+        """Smoothing schedule of the FP32 V-cycle, graph capture, the outer iteration (flexible CG
+        with krylov=True, else plain iterative refinement) and the V-cycles per outer iteration (0:
+        automatic, two unless the operator is nearly singular in FP32)."""
         lib = _lib.load()
         _lib.check(lib.gdtp_mixed_acceleration(self._h, 1 if krylov else 0))
+        _lib.check(lib.gdtp_mixed_cycles(self._h, int(cycles)))
         _lib.check(lib.gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
 
     def mixed_deep_sweep(self, level, ncols):

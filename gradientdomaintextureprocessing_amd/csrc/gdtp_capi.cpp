@@ -44,7 +44,7 @@ struct gdtp_system {
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
@@ -433,7 +433,7 @@ static MixedSystem& Mixed(gdtp_system* s) {
   }
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
-  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0;
+  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles;
   return m;
 }
 
@@ -454,6 +454,16 @@ int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg) {
   s->krylov = flexible_cg ? 1 : 0;
   if (s->mixed) {
     s->mixed->krylov = s->krylov != 0;
+    InvalidateGraph(*s->mixed);
+  }
+  return GDTP_OK;
+}
+
+int gdtp_mixed_cycles(gdtp_system* s, int cycles) {
+  if (!s || cycles < 0 || cycles > 16) return Fail(GDTP_ERROR_INVALID, "V-cycles per iteration must be in [0, 16] (0: automatic)");
+  s->cycles = cycles;
+  if (s->mixed) {
+    s->mixed->cycles = cycles;
     InvalidateGraph(*s->mixed);
   }
   return GDTP_OK;

@@ -52,6 +52,15 @@ struct MixedSystem {
   double screen = 1.0, stiff = 0.0;  // the system's own weights
   // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
   bool krylov = false;
+  // V-cycles per outer iteration (the level-0 correction carries over between them); 0 = automatic:
+  // 2 unless the operator is nearly singular for FP32 (see rowSumRatio), then 1
+  int cycles = 0;
+  // median over level-0 deep rows of (row sum / diagonal) of the FP64 operator.  Continuing the FP32
+  // correction through a second V-cycle evaluates A e in FP32, whose stencil rounding (~1e-7 of the
+  // diagonal, coherent over regular regions) swamps the lowest eigenvalues once this ratio is tiny
+  // (e.g. 1e-6 M + S, the geodesic distance system): the second cycle then amplifies the near-null
+  // mode instead of reducing it.
+  double rowSumRatio = 1.0;
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *z64 = nullptr;  // [kcap][N0] each
   double* dots = nullptr;          // [3][kcap][1024]: per-block partial sums of (z,q), (r,p), (p,q)
   double *alpha = nullptr, *beta = nullptr, *pqPrev = nullptr;  // [kcap]
@@ -62,7 +71,7 @@ struct MixedSystem {
   struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int K = 0;
+    int K = 0, cycles = 1;
     const void *x = nullptr, *b = nullptr;
     bool weighted = false;
     long lastUse = 0;

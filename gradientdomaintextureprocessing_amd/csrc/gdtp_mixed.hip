@@ -698,6 +698,17 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         for (int k = 0; k < 10; k++) st[(size_t)k * d.Nd + di] = (float)rc[10 * (size_t)dd + k];
       }
       d.stencil = Up(ms, st);
+      if (l == 0 && H.numDeep > 0) {
+        std::vector<double> ratio(H.numDeep);
+        for (int dd = 0; dd < H.numDeep; dd++) {
+          double s = 1.0;
+          for (int k = 0; k < 9; k++)
+            if (k != 4) s += rc[10 * (size_t)dd + k];
+          ratio[dd] = s;
+        }
+        std::nth_element(ratio.begin(), ratio.begin() + ratio.size() / 2, ratio.end());
+        ms.rowSumRatio = ratio[ratio.size() / 2];
+      }
       // boundary rows: merged A_BB (off-diagonal) and A_BD with dense columns
       const Csr &bb = oh.levels[l].bb, &bd = oh.levels[l].bd;
       std::vector<int> rp(1, 0), cl, bidx(d.B);
@@ -819,6 +830,8 @@ void InvalidateGraph(MixedSystem& ms) {
 }
 
 static constexpr size_t kMaxGraphs = 8;
+// automatic V-cycles per iteration: 2 when the level-0 row-sum ratio is at least this, else 1
+static constexpr double kTwoCycleRatio = 2e-6;
 
 void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
   if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
@@ -1095,9 +1108,10 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     return ok;
   };
   if (converged()) return 0;
-  // one refinement step = V-cycle on the current residual, update, new residual
-  auto step = [&](hipStream_t st) {
-    MixedVCycle(ds, ms, K, st);
+  int cycles = ms.cycles > 0 ? ms.cycles : (ms.rowSumRatio >= kTwoCycleRatio ? 2 : 1);
+  // one refinement step = V-cycles on the current residual, update, new residual
+  auto step = [&](hipStream_t st, int cyc) {
+    for (int c = 0; c < cyc; c++) MixedVCycle(ds, ms, K, st);
     if (krylov) {
       FcgStep(ds, ms, x, K, st);
     } else {
@@ -1106,11 +1120,12 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     }
     GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));
   };
-  MixedSystem::GraphEntry* ge = nullptr;
-  if (useGraph) {
+  // the captured step for (K, x, b, weighted, cycles): shape-static, replayed on the solver's stream
+  auto graphFor = [&](int cyc) {
+    MixedSystem::GraphEntry* ge = nullptr;
     for (auto& g : ms.graphs)
-      if (g.K == K && g.x == x && g.b == b && g.weighted == weighted) ge = &g;
-    if (!ge) {  // capture one refinement step on the solver's own stream (shape-static per K, x, b)
+      if (g.K == K && g.x == x && g.b == b && g.weighted == weighted && g.cycles == cyc) ge = &g;
+    if (!ge) {
       if (ms.graphs.size() >= kMaxGraphs) {
         auto old = std::min_element(ms.graphs.begin(), ms.graphs.end(), [](const auto& a, const auto& c) { return a.lastUse < c.lastUse; });
         if (old->exec) (void)hipGraphExecDestroy(old->exec);
@@ -1119,26 +1134,38 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
       }
       MixedSystem::GraphEntry g;
       GDTP_HIP(hipStreamBeginCapture(ms.stream, hipStreamCaptureModeThreadLocal));
-      step(ms.stream);
+      step(ms.stream, cyc);
       GDTP_HIP(hipStreamEndCapture(ms.stream, &g.graph));
       GDTP_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-      g.K = K, g.x = x, g.b = b, g.weighted = weighted;
+      g.K = K, g.x = x, g.b = b, g.weighted = weighted, g.cycles = cyc;
       ms.graphs.push_back(g);
       ge = &ms.graphs.back();
     }
     ge->lastUse = ++ms.graphClock;
-  }
+    return ge;
+  };
+  MixedSystem::GraphEntry* ge = useGraph ? graphFor(cycles) : nullptr;
+  std::vector<double> prev(relRes, relRes + K);
   int it = 0;
   while (it < maxIter) {
     if (useGraph) {
       GDTP_HIP(hipGraphLaunch(ge->exec, ms.stream));
       GDTP_HIP(hipStreamSynchronize(ms.stream));
     } else {
-      step(s);
+      step(s, cycles);
       GDTP_HIP(hipStreamSynchronize(s));
     }
     it++;
     if (converged()) break;
+    // Safety net for several V-cycles per iteration: a residual that grows means the FP32
+    // operator misrepresents a near-null mode; continue with one V-cycle per FP64 residual.
+    bool grew = false;
+    for (int k = 0; k < K; k++) grew = grew || !(relRes[k] <= prev[k]);
+    if (grew && cycles > 1) {
+      cycles = 1;
+      if (useGraph) ge = graphFor(cycles);
+    }
+    prev.assign(relRes, relRes + K);
   }
   return it;
 }

@@ -136,6 +136,14 @@ int gdtp_mixed_configure(gdtp_system* s, int nu_pre, int nu_post, int boundary_s
  * default: fewest passes over the operator per iteration) or flexible conjugate gradients
  * preconditioned by the FP32 V-cycle (1: fewer iterations on harder operators). */
 int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg);
+/* V-cycles per outer iteration of the mixed solver (1..16, or 0 = automatic, the default): with
+ * c > 1 the FP32 correction e of A e = r takes c V-cycles (each continuing from the previous e)
+ * before the FP64 update and residual, so the FP64 passes over the operator run once per c
+ * V-cycles.  Automatic: 2, or 1 for operators that are nearly singular in FP32 (median row sum /
+ * diagonal of level 0 below 2e-6, e.g. the geodesic distance system 1e-6 M + S).  A solve whose
+ * residual grows continues with 1.  No counterpart in the reference, whose solve loop is VCycle
+ * repeated (TextureFiltering.cpp:562-579). */
+int gdtp_mixed_cycles(gdtp_system* s, int cycles);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */
 int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream);

@@ -46,6 +46,18 @@ def test_host_only_system_rejects_device_calls():
     assert rc == 1 and b"max_cols" in lib.gdtp_last_error()
 
 
+def test_mixed_settings_are_validated():
+    import gradientdomaintextureprocessing_amd as g
+    mesh = g.TexturedMesh.cube(4, 64, 64)
+    H = g.InitializeHierarchy(mesh, 64, 64, 2)
+    S = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=0)
+    lib = _lib.load()
+    for bad in (-1, 17):
+        assert lib.gdtp_mixed_cycles(S._h, bad) == 1 and b"V-cycles" in lib.gdtp_last_error()
+    assert lib.gdtp_mixed_cycles(S._h, 3) == 0
+    assert lib.gdtp_mixed_configure(S._h, 0, 0, 1, 1.0, 1) == 1
+
+
 def test_geometry_errors_surface():
     import gradientdomaintextureprocessing_amd as g
     # two triangles sharing an edge with the same orientation: not an oriented manifold

@@ -78,6 +78,33 @@ def test_geodesics_on_a_sphere(mode):
         assert abs(phi[k, s]) == 0.0
 
 
+@pytest.mark.parametrize("cycles", [0, 2])
+def test_distance_system_with_several_cycles_per_iteration(cycles):
+    """1e-6 M + S is nearly singular in FP32: the automatic setting (0) runs one V-cycle per FP64
+    residual, and a forced two-cycle setting falls back to it once the residual grows; both reach
+    the tolerance and agree with the one-cycle solve."""
+    mesh = g.TexturedMesh.icosphere(4, 512, 512, 0.0, 3)
+    H = g.InitializeHierarchy(mesh, 512, 512, 5)
+    geo = g.Geodesics(H, heat_time=1e-3, max_sources=2)
+    N = H.level_info(0)["texels"]
+    delta = torch.zeros(2, N, dtype=torch.float64, device="cuda")
+    delta[0, 10] = delta[1, N // 2] = 1.0
+    u = torch.zeros_like(delta)
+    geo.heat.solve(u, delta, tol=1e-8, max_iters=80, mode="mixed")
+    div = torch.empty_like(u)
+    geo.heat.geodesic_divergence(u, div)
+    out = []
+    for c in (1, cycles):
+        geo.distance.configure_mixed(cycles=c)
+        phi = torch.zeros_like(u)
+        its, rel = geo.distance.solve(phi, div, tol=1e-8, max_iters=80, mode="mixed")
+        print(f"[distance] cycles={c}: {its} iterations, rel res {rel.max():.2e}", flush=True)
+        assert rel.max() <= 1e-8
+        out.append(phi - phi.mean(dim=1, keepdim=True))
+    scale = out[0].abs().max().item()
+    assert (out[0] - out[1]).abs().max().item() <= 1e-4 * scale
+
+
 @pytest.mark.parametrize("sharpness", [0.0, 2.0])
 def test_texture_filter_matches_direct_solve(sharpness):
     import scipy.sparse.linalg as spla

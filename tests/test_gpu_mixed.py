@@ -96,6 +96,25 @@ def test_columns_alone_and_together(case):
         assert np.sqrt(e @ (A @ e) / (xd @ (A @ xd))) < 1e-5
 
 
+@pytest.mark.parametrize("cycles", [1, 3])
+def test_cycles_per_iteration_reach_tolerance(case, cycles):
+    """c V-cycles per outer iteration (the correction continues between them) converge to the same
+    solution; the outer iteration count shrinks about c-fold."""
+    name, H, S, L, A = case
+    _, b = noise_texture_rhs(H, L[0], K=2, seed=9)
+    want = spla.splu(A).solve(b.T).T
+    S.configure_mixed(cycles=cycles)
+    try:
+        x = torch.zeros((2, L[0].N), dtype=torch.float64, device="cuda")
+        its, rel = S.solve(x, torch.from_numpy(b).cuda(), tol=1e-9, max_iters=60, mode="mixed")
+        print(f"[{name}] {cycles} V-cycles per iteration: {its} iterations, rel res {rel.max():.2e}", flush=True)
+        assert rel.max() <= 1e-9
+        e = x.cpu().numpy() - want
+        assert np.all(np.einsum("ij,ij->i", e, (A @ e.T).T) <= 1e-14 * np.einsum("ij,ij->i", want, (A @ want.T).T))
+    finally:
+        S.configure_mixed()
+
+
 def test_weighted_columns_match_direct_solves():
     """Parametric solve: one system (built for alpha = 1e-3) serves columns with other weights."""
     import scipy.sparse.linalg as spla
