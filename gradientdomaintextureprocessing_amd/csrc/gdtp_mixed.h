@@ -18,6 +18,7 @@ struct DevMixLevel {
   int8_t* type = nullptr;          // [Nd]: -1 inactive, 0/1 boundary, 2 deep
   int* g2d = nullptr;              // [N] compact global index -> dense index
   int nTiles = 0;
+  bool chartSweeps = false;         // small charts: one workgroup per chart runs all four colours of a sweep
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;

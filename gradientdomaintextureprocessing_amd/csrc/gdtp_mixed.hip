@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 #include <unordered_map>
 
 namespace gdtp {
@@ -25,6 +26,7 @@ namespace gdtp {
 static constexpr int kTA = 64, kTB = 4;  // tile of the sub-grid per workgroup: 64 x 4 (256 threads)
 static constexpr int kDenseCoarseMax = 6000;  // coarsest levels up to this size use a dense inverse
 static constexpr int kDotBlocks = 1024;       // partial sums per column of the flexible-CG dot products
+static constexpr long kChartSweepMax = 2048;  // largest chart colour plane (texels) swept by one workgroup
 static inline unsigned NBlk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 __device__ __forceinline__ long DenseIdx(const MixChart& c, long plane, int i, int j) {
@@ -32,62 +34,136 @@ __device__ __forceinline__ long DenseIdx(const MixChart& c, long plane, int i, i
   return (long)q * plane + c.base2 + (long)(j >> 1) * c.w2 + (i >> 1);
 }
 
+// Dense vectors are addressed with 32-bit byte offsets (each column is under 2 GiB, checked at
+// upload) from a uniform per-column base held in a buffer resource: one offset register per
+// neighbour serves all columns of a chunk, which keeps the register count low while every load
+// of the chunk is in flight.
+__device__ __forceinline__ float Ld(const float* column, int bytes) {
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(column), 0, 0x7fffffff, 0x00020000);
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, bytes, 0, 0));
+}
+__device__ __forceinline__ int DenseByte(const MixChart& c, long plane, int i, int j) {
+  return 4 * (((i & 1) | ((j & 1) << 1)) * (int)plane + c.base2 + (j >> 1) * c.w2 + (i >> 1));
+}
+
+// Columns in chunks of up to four with the chunk width as a compile-time constant, so that a
+// kernel can issue every load of a chunk before its first store (one memory round trip per chunk
+// instead of one per column).
+template <class F>
+__device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    switch (min(4, K - c0)) {
+      case 4: f(c0, std::integral_constant<int, 4>()); break;
+      case 3: f(c0, std::integral_constant<int, 3>()); break;
+      case 2: f(c0, std::integral_constant<int, 2>()); break;
+      default: f(c0, std::integral_constant<int, 1>()); break;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ deep four-colour Gauss-Seidel
+// Update of deep texel (i, j) of plane q (sub-grid (a, b)): e = c4 rhs - sum_{k!=4} c_k e_k.  The
+// neighbours lie in the other three planes, so every load of a chunk of columns is issued before
+// its stores (few memory round trips per texel: small levels are latency bound).
+__device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float* __restrict__ st,
+                                            float* e, const float* __restrict__ rhs, int K) {
+  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  if (i >= c.w || j >= c.h) return;
+  const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
+  const float c4 = st[4 * Nd + idx];
+  if (c4 == 0.0f) return;
+  float cf[9];
+  int nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+    nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
+  }
+  ForColumnChunks(K, [&](int c0, auto kc) {
+    constexpr int KC = decltype(kc)::value;
+    float v[KC][9], r[KC];
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) {
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+        if (k != 4) v[ch][k] = Ld(e + (c0 + ch) * Nd, nb[k]);
+      r[ch] = Ld(rhs + (c0 + ch) * Nd, nb[4]);
+    }
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+        if (k != 4) s += cf[k] * v[ch][k];
+      e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s;
+    }
+  });
+}
+
 __global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                   long Nd, const float* __restrict__ st, float* e, const float* __restrict__ rhs, int K) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
-  if (a >= c.w2 || b >= c.h2 || i >= c.w || j >= c.h) return;
-  const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-  const float c4 = st[4 * Nd + idx];
-  if (c4 == 0.0f) return;
-  float cf[9];
-  long nb[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int di = k % 3 - 1, dj = k / 3 - 1;
-    cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
-    nb[k] = k == 4 ? idx : DenseIdx(c, plane, i + di, j + dj);
-  }
-  for (int ch = 0; ch < K; ch++) {
-    const float* E = e + (long)ch * Nd;
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; k++)
-      if (k != 4) s += cf[k] * E[nb[k]];
-    e[(long)ch * Nd + idx] = c4 * rhs[(long)ch * Nd + idx] - s;
+  if (a >= c.w2 || b >= c.h2) return;
+  DeepGsTexel(c, plane, Nd, q, a, b, st, e, rhs, K);
+}
+
+// The same four-colour sweep with one 1024-thread workgroup per chart, colours in order (post:
+// 3..0) separated by barriers: deep stencils never reach across charts, so this is exactly the
+// four launches of KMixDeepGs in one, for levels whose charts are small enough that launch
+// latency, not memory, bounds a sweep.  All threads reach every barrier.
+__global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __restrict__ charts, long plane, long Nd,
+                                                    const float* __restrict__ st, float* e, const float* __restrict__ rhs, int K) {
+  const MixChart c = charts[blockIdx.x];
+  const int aw = (c.w + 1) / 2, n = aw * c.h2;
+  for (int qq = 0; qq < 4; qq++) {
+    const int q = post ? 3 - qq : qq;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel(c, plane, Nd, q, t % aw, t / aw, st, e, rhs, K);
+    __syncthreads();
   }
 }
 
-// res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4); blockIdx.y = plane
+// res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4).  A workgroup
+// covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
+// once rather than once per plane.
 __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                         long Nd, const float* __restrict__ st, const float* __restrict__ e,
                                                         const float* __restrict__ rhs, float* res, int K) {
-  const int q = blockIdx.y;
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
-  if (a >= c.w2 || b >= c.h2 || i >= c.w || j >= c.h) return;
-  const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-  const float a4 = st[9 * Nd + idx];
-  if (a4 == 0.0f) return;
-  float cf[9];
-  long nb[9];
+  if (a >= c.w2 || b >= c.h2) return;
+  for (int q = 0; q < 4; q++) {
+    const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+    if (i >= c.w || j >= c.h) continue;
+    const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
+    const float a4 = st[9 * Nd + idx];
+    if (a4 == 0.0f) continue;
+    float cf[9];
+    int nb[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int di = k % 3 - 1, dj = k / 3 - 1;
-    cf[k] = k == 4 ? 1.0f : st[k * Nd + idx];
-    nb[k] = k == 4 ? idx : DenseIdx(c, plane, i + di, j + dj);
-  }
-  for (int ch = 0; ch < K; ch++) {
-    const float* E = e + (long)ch * Nd;
-    float s = 0.0f;
+    for (int k = 0; k < 9; k++) {
+      cf[k] = k == 4 ? 1.0f : st[k * Nd + idx];
+      nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
+    }
+    ForColumnChunks(K, [&](int c0, auto kc) {
+      constexpr int KC = decltype(kc)::value;
+      float v[KC][9], r[KC];
 #pragma unroll
-    for (int k = 0; k < 9; k++) s += cf[k] * E[nb[k]];
-    res[(long)ch * Nd + idx] = rhs[(long)ch * Nd + idx] - s * a4;
+      for (int ch = 0; ch < KC; ch++) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[ch][k] = Ld(e + (c0 + ch) * Nd, nb[k]);
+        r[ch] = Ld(rhs + (c0 + ch) * Nd, nb[4]);
+      }
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++) s += cf[k] * v[ch][k];
+        res[(long)(c0 + ch) * Nd + idx] = r[ch] - s * a4;
+      }
+    });
   }
 }
 
@@ -180,27 +256,39 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
 }
 
 // ------------------------------------------------------------------ transfers
-// Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c.
+// Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c (all four
+// coarse colour planes of a tile per workgroup: they read overlapping fine neighbourhoods).
 __global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ tiles, const MixChart* __restrict__ cch,
                                                         const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
                                                         const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
                                                         int K) {
-  const int q = blockIdx.y;
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = cch[T[0]], f = fch[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  const int ci = 2 * a + (q & 1), cj = 2 * b + (q >> 1);
-  if (a >= c.w2 || b >= c.h2 || ci >= c.w || cj >= c.h) return;
-  const long idx = (long)q * cplane + c.base2 + (long)b * c.w2 + a;
-  if (ctype[idx] != 2) return;
-  const int fi = f.cofx + 2 * (ci - c.cofx), fj = f.cofy + 2 * (cj - c.cofy);
-  long n[9];
+  if (a >= c.w2 || b >= c.h2) return;
+  for (int q = 0; q < 4; q++) {
+    const int ci = 2 * a + (q & 1), cj = 2 * b + (q >> 1);
+    if (ci >= c.w || cj >= c.h) continue;
+    const long idx = (long)q * cplane + c.base2 + (long)b * c.w2 + a;
+    if (ctype[idx] != 2) continue;
+    const int fi = f.cofx + 2 * (ci - c.cofx), fj = f.cofy + 2 * (cj - c.cofy);
+    int n[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) n[k] = DenseIdx(f, fplane, fi + k % 3 - 1, fj + k / 3 - 1);
-  for (int ch = 0; ch < K; ch++) {
-    const float* R = fres + (long)ch * fNd;
-    crhs[(long)ch * cNd + idx] = R[n[4]] + (R[n[1]] + R[n[3]] + R[n[5]] + R[n[7]]) * 0.5f + (R[n[0]] + R[n[2]] + R[n[6]] + R[n[8]]) * 0.25f;
-    ce[(long)ch * cNd + idx] = 0.0f;
+    for (int k = 0; k < 9; k++) n[k] = DenseByte(f, fplane, fi + k % 3 - 1, fj + k / 3 - 1);
+    ForColumnChunks(K, [&](int c0, auto kc) {
+      constexpr int KC = decltype(kc)::value;
+      float v[KC][9];
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++)
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[ch][k] = Ld(fres + (c0 + ch) * fNd, n[k]);
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        const float* R = v[ch];
+        crhs[(long)(c0 + ch) * cNd + idx] = R[4] + (R[1] + R[3] + R[5] + R[7]) * 0.5f + (R[0] + R[2] + R[6] + R[8]) * 0.25f;
+        ce[(long)(c0 + ch) * cNd + idx] = 0.0f;
+      }
+    });
   }
 }
 
@@ -220,29 +308,43 @@ __global__ void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const
 
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
-// e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid).
+// e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid), all four
+// fine colour planes of a tile per workgroup (they share their coarse neighbours).  Deep texels of
+// plane `skipQ` are left alone: the Gauss-Seidel sweep that follows overwrites them without
+// reading them (skipQ = -1 when no sweep follows).
 __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ tiles, const MixChart* __restrict__ fch,
                                                       const MixChart* __restrict__ cch, long fplane, long fNd, long cplane, long cNd,
-                                                      const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K) {
-  const int q = blockIdx.y;
+                                                      const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K,
+                                                      int skipQ) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart f = fch[T[0]], c = cch[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  const int fi = 2 * a + (q & 1), fj = 2 * b + (q >> 1);
-  if (a >= f.w2 || b >= f.h2 || fi >= f.w || fj >= f.h) return;
-  const long idx = (long)q * fplane + f.base2 + (long)b * f.w2 + a;
-  if (ftype[idx] < 0) return;
-  const int ui = fi - f.cofx, uj = fj - f.cofy;
-  const int ci = FloorHalf(ui) + c.cofx, cj = FloorHalf(uj) + c.cofy;
-  const bool oi = (ui & 1) != 0, oj = (uj & 1) != 0;
-  const long n00 = DenseIdx(c, cplane, ci, cj);
-  const long n10 = oi ? DenseIdx(c, cplane, ci + 1, cj) : n00;
-  const long n01 = oj ? DenseIdx(c, cplane, ci, cj + 1) : n00;
-  const long n11 = (oi && oj) ? DenseIdx(c, cplane, ci + 1, cj + 1) : (oi ? n10 : n01);
-  for (int ch = 0; ch < K; ch++) {
-    const float* E = ce + (long)ch * cNd;
-    float v = (E[n00] + E[n10] + E[n01] + E[n11]) * 0.25f;
-    fe[(long)ch * fNd + idx] += v;
+  if (a >= f.w2 || b >= f.h2) return;
+  for (int q = 0; q < 4; q++) {
+    const int fi = 2 * a + (q & 1), fj = 2 * b + (q >> 1);
+    if (fi >= f.w || fj >= f.h) continue;
+    const long idx = (long)q * fplane + f.base2 + (long)b * f.w2 + a;
+    const int t = ftype[idx];
+    if (t < 0 || (q == skipQ && t == 2)) continue;
+    const int ui = fi - f.cofx, uj = fj - f.cofy;
+    const int ci = FloorHalf(ui) + c.cofx, cj = FloorHalf(uj) + c.cofy;
+    const bool oi = (ui & 1) != 0, oj = (uj & 1) != 0;
+    const int n00 = DenseByte(c, cplane, ci, cj);
+    const int n10 = oi ? DenseByte(c, cplane, ci + 1, cj) : n00;
+    const int n01 = oj ? DenseByte(c, cplane, ci, cj + 1) : n00;
+    const int n11 = (oi && oj) ? DenseByte(c, cplane, ci + 1, cj + 1) : (oi ? n10 : n01);
+    ForColumnChunks(K, [&](int c0, auto kc) {
+      constexpr int KC = decltype(kc)::value;
+      float v[KC][4], old[KC];
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        const float* E = ce + (c0 + ch) * cNd;
+        v[ch][0] = Ld(E, n00), v[ch][1] = Ld(E, n10), v[ch][2] = Ld(E, n01), v[ch][3] = Ld(E, n11);
+        old[ch] = Ld(fe + (c0 + ch) * fNd, (int)(4 * idx));
+      }
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) fe[(long)(c0 + ch) * fNd + idx] = old[ch] + (v[ch][0] + v[ch][1] + v[ch][2] + v[ch][3]) * 0.25f;
+    });
   }
 }
 
@@ -652,7 +754,13 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       base += (long)m.w2 * m.h2;
     }
     if (base > (1L << 29)) throw std::runtime_error("dense colour plane too large");
+    {
+      long largest = 0;
+      for (const MixChart& m : d.hcharts) largest = std::max(largest, (long)((m.w + 1) / 2) * m.h2);
+      d.chartSweeps = largest <= kChartSweepMax;
+    }
     d.plane = base, d.Nd = 4 * base;
+    if (4 * d.Nd >= (1L << 31)) throw std::runtime_error("texel atlas too large for the mixed-precision solver (2 GiB per dense FP32 column)");
     auto dense = [&](int c, int i, int j) {
       const MixChart& m = d.hcharts[c];
       int q = (i & 1) | ((j & 1) << 1);
@@ -812,11 +920,14 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
   const int nu = post ? ms.nuPost : ms.nuPre;
   for (int it = 0; it < nu; it++) {
     if (!post) BoundarySweeps(ms, d, K, s);
-    if (d.nTiles > 0)
+    if (d.nTiles > 0 && d.chartSweeps) {
+      KMixChartGs<<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+    } else if (d.nTiles > 0) {
       for (int qq = 0; qq < 4; qq++) {
         int q = post ? 3 - qq : qq;
         KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
       }
+    }
     if (post) BoundarySweeps(ms, d, K, s);
   }
 }
@@ -845,12 +956,12 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     Smooth(ms, l, K, false, s);  // e of this level was zeroed by whoever wrote its rhs
     if (d.nTiles > 0)
-      KMixResidualDeep<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
+      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
     if (d.B > 0)
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
                                                                             0, d.Nd, K);
     if (c.nTiles > 0)
-      KMixRestrictDeep<<<dim3(c.nTiles, 4), 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
+      KMixRestrictDeep<<<c.nTiles, 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
                                                          K);
     if (c.B > 0)
       KMixRestrictBoundary<<<NBlk((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs, c.e, c.Nd,
@@ -870,7 +981,8 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
   for (int l = nl - 2; l >= 0; l--) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     if (d.nTiles > 0)
-      KMixProlongAdd<<<dim3(d.nTiles, 4), 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K);
+      KMixProlongAdd<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K,
+                                              ms.nuPost > 0 ? 3 : -1);
     Smooth(ms, l, K, true, s);
   }
   (void)ds;
