@@ -262,6 +262,85 @@ def cpu_baseline(H, S, K, tol, budget_s=60.0):
                 sample=f"one {K}-column solve of the same system to rel-res {tol:g} ({cycles} oracle V-cycles, single thread)")
 
 
+def geodesics_bench(steps, warmup, tol, sources=64, seed=7):
+    """Heat-method geodesics batched as columns: noisy icosphere (327,680 triangles, amplitude
+    0.05) on a 2048^2 atlas, 64 seeded source texels; (M + t S) u = delta with the reference's
+    t = 1e-3 of the unit-area surface (Geodesics.cpp:19; with t = h^2 the heat decays below any
+    solve tolerance within a few hundred texels and the far-field gradient direction is noise),
+    normalised-gradient divergence, (1e-6 M + S) phi = div, phi -= phi[source]; every solve to
+    `tol`.  One step = all 64 sources."""
+    import torch
+    import gradientdomaintextureprocessing_amd as g
+    t0 = time.time()
+    mesh = g.TexturedMesh.icosphere(7, 2048, 2048, 0.05, 3)
+    H = g.InitializeHierarchy(mesh, 2048, 2048, 7)
+    N = H.level_info(0)["texels"]
+    geo = g.Geodesics(H, heat_time=1e-3, max_sources=sources)
+    for s in (geo.heat, geo.distance):
+        s.configure_mixed()
+    log(f"[bench] geodesics setup {time.time() - t0:.1f}s, {N} texels")
+    src = [int(v) for v in np.random.default_rng(seed).choice(N, sources, replace=False)]
+    lat = []
+    for step in range(warmup + steps):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        phi = geo.distances(src, tol=tol, mode="mixed", max_iters=200)
+        torch.cuda.synchronize()
+        if step >= warmup:
+            lat.append(time.perf_counter() - ts)
+    ms = float(np.mean(lat)) * 1e3
+    return dict(metric=f"geodesic sources/s at rel-residual {tol:g} (2048² noisy icosphere)",
+                value=sources / (ms / 1e3), unit="sources/s", n_gpus=1, steps=steps, warmup=warmup, ms_per_step=ms,
+                higher_is_better=True, dtype="f32 relax / f64 refine", data="synthetic",
+                config=dict(workload=f"c3: heat-method geodesics, {N} texels, {sources} sources as columns, t=1e-3",
+                            max_abs_phi=float(phi.abs().max())),
+                latency_ms=dict(p50=float(np.percentile(lat, 50)) * 1e3, p99=float(np.percentile(lat, 99)) * 1e3))
+
+
+def lic_bench(steps, warmup, columns=16, rounds=4, seed=9):
+    """Line integral convolution as anisotropic diffusion: noisy torus (R 1, r 0.35, 262,144
+    triangles, 32 charts) on a 2048^2 atlas, field = surface gradient of seeded smooth noise,
+    cross-field weight 1e-4, float32 white-noise input, 16 columns.  Geometric multigrid does not
+    converge on this anisotropy (the reference runs a fixed number of rounds), so one step is the
+    reference's schedule: `rounds` FP64 V-cycles of (M_a + 1e-4 S_a) x = M_a noise, with the
+    relative residual it leaves reported."""
+    import torch
+    import gradientdomaintextureprocessing_amd as g
+    t0 = time.time()
+    mesh = g.TexturedMesh.torus(512, 256, 8, 4, 2048, 2048, 1.0, 0.35, 0.05, 5)
+    v, t, _ = mesh.arrays()
+    rng = np.random.default_rng(seed)
+    ph = rng.random(4) * 6.28
+    f = np.sin(3 * v[:, 0] + ph[0]) + np.cos(2 * v[:, 1] + ph[1]) + np.sin(4 * v[:, 2] + ph[2]) + 0.5 * np.sin(5 * v[:, 0] * v[:, 1] + ph[3])
+    lic = g.LineIntegralConvolution(mesh, 2048, 2048, 7, g.surface_gradient_field(v, t, f), max_cols=columns)
+    S = lic.lic
+    N = lic.anisotropic.level_info(0)["texels"]
+    log(f"[bench] lic setup {time.time() - t0:.1f}s, {N} texels")
+    noise = torch.from_numpy(rng.random((columns, N), dtype=np.float32).astype(np.float64)).cuda()
+    b = torch.empty_like(noise)
+    S.apply(g.OPERATOR_MASS, noise, b)
+    lat = []
+    for step in range(warmup + steps):
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(rounds):
+            S.vcycle(x, b)
+        torch.cuda.synchronize()
+        if step >= warmup:
+            lat.append(time.perf_counter() - ts)
+    r = torch.empty_like(b)
+    S.residual(x, b, r)
+    rel = float((r.norm(dim=1) / b.norm(dim=1)).max())
+    ms = float(np.mean(lat)) * 1e3
+    return dict(metric="LIC columns/s, reference schedule (2048² noisy torus)",
+                value=columns / (ms / 1e3), unit="columns/s", n_gpus=1, steps=steps, warmup=warmup, ms_per_step=ms,
+                higher_is_better=True, dtype="f64", data="synthetic",
+                config=dict(workload=f"c4: anisotropic diffusion, {N} texels, {columns} columns, {rounds} FP64 V-cycles",
+                            rel_res_after=rel),
+                latency_ms=dict(p50=float(np.percentile(lat, 50)) * 1e3, p99=float(np.percentile(lat, 99)) * 1e3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,12 +352,14 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
+    ap.add_argument("--workload", default=None, choices=["c3", "c4"],
+                    help="other workloads: c3 batched geodesics, c4 line integral convolution (one JSON line each)")
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver shards by requests / RHS columns, "
-              "see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); across GPUs the solver would shard by "
+              "requests / RHS columns with every mesh resident on each GPU, which needs no collective", file=sys.stderr)
         sys.exit(2)
     import torch
     K = 3
@@ -288,6 +369,11 @@ def main():
         return
     if args.serving:
         serving_bench(args.qps, args.requests, args.tol)
+        return
+    if args.workload:
+        steps = min(args.steps, 10)
+        res = geodesics_bench(steps, args.warmup, args.tol) if args.workload == "c3" else lic_bench(steps, args.warmup)
+        print(json.dumps(res), flush=True)
         return
     results = {}
     modes = {"mixed": ["mixed", "fp64"], "mixed-only": ["mixed"], "fp64": ["fp64"]}[args.mode]

@@ -46,6 +46,12 @@ SIGNATURES = {
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_mixed_configure": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int]),
     "gdtp_mixed_cycles": (c_int, [c_void_p, c_int]),
+    "gdtp_modulation_create": (c_int, [c_void_p, c_void_p, c_int, P(c_void_p)]),
+    "gdtp_modulated_stiffness": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "gdtp_modulation_info": (c_int, [c_void_p, c_void_p]),
+    "gdtp_modulation_array_f64": (c_int, [c_void_p, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_modulation_array_i32": (c_int, [c_void_p, c_char_p, c_void_p, P(c_size_t)]),
+    "gdtp_modulation_destroy": (None, [c_void_p]),
     "gdtp_mixed_acceleration": (c_int, [c_void_p, c_int]),
     "gdtp_mixed_deep_sweep": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "gdtp_system_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),

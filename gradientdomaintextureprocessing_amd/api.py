@@ -119,9 +119,61 @@ class HierarchicalSystem:
         cj = charts[ni[:, 0], 3] + ni[:, 2]
         return ci, cj, ni[:, 0]
 
+    def cell_coordinates(self, level=0):
+        """(atlas column, atlas row, chart, type) of every active cell of `level` in global cell
+        order (chart by chart, rows then columns; type 1 interior, 0 boundary); a cell spans the
+        texels (column, row) .. (column + 1, row + 1)."""
+        info = self.array(level, "cellInfo").reshape(-1, 4)
+        charts = self.array(level, "charts").reshape(-1, 9)
+        return charts[info[:, 0], 2] + info[:, 1], charts[info[:, 0], 3] + info[:, 2], info[:, 0], info[:, 3]
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.gdtp_hierarchy_destroy(self._h)
+            self._h = None
+
+
+class StiffnessModulation:
+    """Per-cell stiffness of fixed signal columns, weighted per cell on demand: the sharpening
+    brush of TextureFiltering.  Set-up applies every active cell's stiffness to the signal
+    (InitializeMassAndStiffness with computeCellBasedStiffness, MassAndStiffness.h:296-395,
+    490-549); `apply(mask)` is CellStiffnessToTexelStiffness (IterativeSolvers.inl:1170-1208):
+    t[c] = sum over cells k of mask[k] * S_k signal[c], so a mask of ones gives S signal."""
+
+    def __init__(self, hierarchy, signal):
+        f = signal.detach().cpu().numpy() if hasattr(signal, "detach") else np.asarray(signal)
+        f = np.ascontiguousarray(np.atleast_2d(f), dtype=np.float64)
+        self.hierarchy = hierarchy
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().gdtp_modulation_create(hierarchy._h, f.ctypes.data, f.shape[0], ctypes.byref(h)))
+        self._h = h
+        info = np.zeros(4, np.int32)
+        _lib.check(_lib.load().gdtp_modulation_info(self._h, info.ctypes.data))
+        self.columns, self.texels, self.num_cells, self.boundary_entries = (int(v) for v in info)
+
+    def apply(self, mask, out=None):
+        """mask: [num_cells] FP64 GPU tensor; returns out ([columns, texels] FP64)."""
+        import torch
+        if mask.shape != (self.num_cells,) or mask.dtype != torch.float64:
+            raise ValueError(f"mask must be a float64 vector of {self.num_cells} cells")
+        if out is None:
+            out = torch.empty(self.columns, self.texels, dtype=torch.float64, device=mask.device)
+        _lib.check(_lib.load().gdtp_modulated_stiffness(self._h, _ptr(mask), _ptr(out), _stream()))
+        return out
+
+    def array(self, name, dtype=np.float64):
+        lib = _lib.load()
+        fn = lib.gdtp_modulation_array_f64 if dtype == np.float64 else lib.gdtp_modulation_array_i32
+        n = ctypes.c_size_t(0)
+        _lib.check(fn(self._h, name.encode(), None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype)
+        if n.value:
+            _lib.check(fn(self._h, name.encode(), out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.gdtp_modulation_destroy(self._h)
             self._h = None
 
 
@@ -311,29 +363,40 @@ class TextureFilter:
     """Screened-Poisson texture filtering (TextureFiltering.cpp:581-718, 562-579):
         (M + a S) x = M x0 + s a S x0,   x starts at x0,
     with a the stiffness (smoothing) weight and s the sharpness weight (s = 0: pure smoothing,
-    s > 1: sharpening).  Columns are channels."""
+    s > 1: sharpening).  Columns are channels.  A per-cell sharpness (the app's brush) replaces
+    s S x0 by sum_k s_k S_k x0 after `set_signal(x0)` (TextureFiltering.cpp:569-570)."""
 
     def __init__(self, hierarchy, stiffness_weight=1e-3, sharpness_weight=1.0, max_cols=3):
         self.hierarchy, self.stiffness, self.sharpness = hierarchy, stiffness_weight, sharpness_weight
         self.system = UpdateLinearSystem(hierarchy, stiffness_weight, 1.0, max_cols=max_cols)
+        self.modulation = None
+
+    def set_signal(self, x0):
+        """Cell-based stiffness of the input texels, for per-cell sharpness masks."""
+        self.modulation = StiffnessModulation(self.hierarchy, x0)
 
     def set_stiffness(self, stiffness_weight):
         """UpdateLinearSystem on a new weight (the app's slider, TextureFiltering.cpp:515-545)."""
         self.stiffness = stiffness_weight
         self.system.update(stiffness_weight, 1.0)
 
-    def rhs(self, x0):
+    def rhs(self, x0, mask=None):
+        """M x0 + s a S x0, or with a per-cell mask (after set_signal(x0)) M x0 + a t(mask)."""
         import torch
         m, s = torch.empty_like(x0), torch.empty_like(x0)
         self.system.apply(OPERATOR_MASS, x0, m)
+        if mask is not None:
+            if self.modulation is None:
+                raise ValueError("set_signal(x0) builds the cell-based stiffness a mask weights")
+            return m + self.stiffness * self.modulation.apply(mask)
         if self.sharpness == 0:
             return m
         self.system.apply(OPERATOR_STIFFNESS, x0, s)
         return m + (self.sharpness * self.stiffness) * s
 
-    def filter(self, x0, tol=1e-6, mode="mixed", max_iters=60):
+    def filter(self, x0, tol=1e-6, mode="mixed", max_iters=60, mask=None):
         """Filtered texels for input texels x0 ([K, N] FP64 on the GPU); returns (x, iterations)."""
-        b = self.rhs(x0)
+        b = self.rhs(x0, mask)
         x = x0.clone()
         its, _ = self.system.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
         return x, its

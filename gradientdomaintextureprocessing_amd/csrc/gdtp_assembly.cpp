@@ -138,10 +138,33 @@ static void IntegrateCellPolygon(const std::vector<V2>& poly, int i, int j, doub
   }
 }
 
+namespace {
+// boundary-row entries of the cell-based stiffness products of one chart: (row, cell) + K values
+struct CellRows {
+  std::vector<int> row, cell;
+  std::vector<double> val;  // K per entry
+};
+}  // namespace
+
 int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric, FineOperator& op,
-                         std::string& err) {
+                         std::string& err, const double* signal, int K, CellStiffnessRhs* cellRhs) {
   const Level& L0 = h.levels[0];
   const int nCharts = (int)L0.charts.size();
+  const long N = L0.numTexels;
+  const bool cells = cellRhs && signal && K > 0;
+  std::vector<CellRows> crows(cells ? nCharts : 0);
+  if (cells) {
+    cellRhs->K = K, cellRhs->numCells = L0.numCells;
+    cellRhs->deep.assign((size_t)K * 4 * L0.numDeep, 0.0);
+  }
+  // S_c f_k at corner k of a cell whose corners (order (0,0),(1,0),(1,1),(0,1)) are texels gl
+  auto cellProducts = [&](const std::array<double, 16>& S, const std::array<int, 4>& gl, int k, double* out) {
+    for (int kk = 0; kk < K; kk++) {
+      double p = 0;
+      for (int l = 0; l < 4; l++) p += S[4 * k + l] * signal[kk * N + gl[l]];
+      out[kk] = p;
+    }
+  };
   op.deepMass.assign(10 * (size_t)L0.numDeep, 0.0);
   op.deepStiffness.assign(10 * (size_t)L0.numDeep, 0.0);
   std::vector<std::vector<Triplet>> bbM(nCharts), bbS(nCharts), bdM(nCharts), bdS(nCharts);
@@ -176,9 +199,21 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
           poly.assign(p, p + 3);
           if (!ClipToCell(poly, i, j, g.cellW, g.cellH)) continue;
           if (lb >= 0) {  // boundary cell: pieces touching a seam carry the averaged seam values
+            const size_t n0 = bbS[c].size();
             int rc = AssembleSeamPiece(seams, c, t, i, j, poly, G, bbM[c], bbS[c], errs[c]);
             if (rc < 0) break;
-            if (rc > 0) continue;
+            if (rc > 0) {
+              if (cells) {  // the piece's texel-space stiffness applied to the signal, charged to this cell
+                const int cell = g.cellOffset + g.localCell[q];
+                for (size_t e = n0; e < bbS[c].size(); e++) {
+                  const Triplet& tr = bbS[c][e];
+                  const long col = L0.boundaryGlobalIndex[tr.c];
+                  crows[c].row.push_back(tr.r), crows[c].cell.push_back(cell);
+                  for (int kk = 0; kk < K; kk++) crows[c].val.push_back(tr.v * signal[kk * N + col]);
+                }
+              }
+              continue;
+            }
           }
           double m[4][4] = {}, s[4][4] = {};
           IntegrateCellPolygon(poly, i, j, g.cellW, g.cellH, Ginv, gscale, m, s);
@@ -190,10 +225,22 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     }
     if (!errs[c].empty()) continue;
     // interior cells: deep rows and the boundary rows they touch (MassAndStiffness.h:277-353)
+    std::vector<double> prod(std::max(K, 1));
     for (int ic = 0; ic < g.numInteriorCells; ic++) {
       const auto& gl = g.interiorCellGlobalCorners[ic];
+      const int cell = g.cellOffset + g.interiorCellToLocal[ic];
       for (int k = 0; k < 4; k++) {
         int bd = L0.boundaryAndDeepIndex[gl[k]];
+        if (cells) {
+          cellProducts(cellS[ic], gl, k, prod.data());
+          if (bd < 0) {
+            const int slot = (1 - offJ[k]) * 2 + (1 - offI[k]);  // where this cell sits around texel gl[k]
+            for (int kk = 0; kk < K; kk++) cellRhs->deep[(size_t)kk * 4 * L0.numDeep + 4 * (size_t)(-bd - 1) + slot] = prod[kk];
+          } else {
+            crows[c].row.push_back(bd - 1), crows[c].cell.push_back(cell);
+            crows[c].val.insert(crows[c].val.end(), prod.begin(), prod.begin() + K);
+          }
+        }
         if (bd < 0) {
           int d = -bd - 1;
           for (int l = 0; l < 4; l++) {
@@ -218,6 +265,16 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     }
     // boundary cells (pieces away from seams)
     AssembleBoundaryCells(h, c, bcellM, bcellS, bbM[c], bbS[c], errs[c]);
+    if (cells)
+      for (int bc = 0; bc < g.numBoundaryCells; bc++) {
+        const int local = g.boundaryCellToLocal[bc];
+        const auto& gl = g.cellIndices[local];
+        for (int k = 0; k < 4; k++) {
+          cellProducts(bcellS[bc], gl, k, prod.data());
+          crows[c].row.push_back(L0.boundaryAndDeepIndex[gl[k]] - 1), crows[c].cell.push_back(g.cellOffset + local);
+          crows[c].val.insert(crows[c].val.end(), prod.begin(), prod.begin() + K);
+        }
+      }
   }
   for (int c = 0; c < nCharts; c++)
     if (!errs[c].empty()) { err = errs[c]; return 0; }
@@ -226,6 +283,44 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     for (auto& x : v) all.insert(all.end(), x.begin(), x.end());
     return all;
   };
+  if (cells) {
+    // deep texels: the cells (i-1,j-1) and (i-1,j) (their right neighbours follow in the same row)
+    cellRhs->deepCells.assign(2 * (size_t)L0.numDeep, -1);
+    for (const GridChart& g : L0.charts)
+      for (int j = 1; j + 1 < g.height; j++)
+        for (int i = 1; i + 1 < g.width; i++) {
+          const int d = g.deep(i, j);
+          if (d < 0) continue;
+          cellRhs->deepCells[2 * (size_t)d] = g.cellOffset + g.localCell[(size_t)(j - 1) * (g.width - 1) + i - 1];
+          cellRhs->deepCells[2 * (size_t)d + 1] = g.cellOffset + g.localCell[(size_t)j * (g.width - 1) + i - 1];
+        }
+    // boundary rows: entries sorted by (row, cell), duplicates (several pieces of a cell) summed
+    std::vector<std::array<int, 3>> ent;  // row, cell, (chart, entry) packed below
+    std::vector<std::pair<int, size_t>> src;
+    for (int c = 0; c < nCharts; c++)
+      for (size_t e = 0; e < crows[c].row.size(); e++) ent.push_back({crows[c].row[e], crows[c].cell[e], (int)src.size()}), src.push_back({c, e});
+    std::sort(ent.begin(), ent.end(), [](const auto& a, const auto& b) { return a[0] != b[0] ? a[0] < b[0] : a[1] < b[1]; });
+    std::vector<int> rowOf;
+    std::vector<double> acc;  // K per merged entry
+    for (size_t e = 0; e < ent.size(); e++) {
+      const bool same = !rowOf.empty() && rowOf.back() == ent[e][0] && cellRhs->bCell.back() == ent[e][1];
+      if (!same) {
+        rowOf.push_back(ent[e][0]), cellRhs->bCell.push_back(ent[e][1]);
+        acc.resize(acc.size() + K, 0.0);
+      }
+      const auto& sr = src[ent[e][2]];
+      const double* v = &crows[sr.first].val[sr.second * K];
+      double* a = &acc[acc.size() - K];
+      for (int kk = 0; kk < K; kk++) a[kk] += v[kk];
+    }
+    const size_t nnz = rowOf.size();
+    cellRhs->bRowptr.assign(L0.numBoundary + 1, 0);
+    for (int r : rowOf) cellRhs->bRowptr[r + 1]++;
+    for (int r = 0; r < L0.numBoundary; r++) cellRhs->bRowptr[r + 1] += cellRhs->bRowptr[r];
+    cellRhs->bVal.assign((size_t)K * nnz, 0.0);
+    for (size_t e = 0; e < nnz; e++)
+      for (int kk = 0; kk < K; kk++) cellRhs->bVal[(size_t)kk * nnz + e] = acc[e * K + kk];
+  }
   {
     auto t = merge(bbM); op.bbMass = Csr::FromTriplets(L0.numBoundary, L0.numBoundary, t);
     t = merge(bbS); op.bbStiffness = Csr::FromTriplets(L0.numBoundary, L0.numBoundary, t);

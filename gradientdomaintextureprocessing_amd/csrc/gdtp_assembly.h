@@ -30,8 +30,24 @@ void ComputeParameterMetricFromSurface(const TexturedMesh& mesh, const Atlas& at
 // false if nothing is left (ClipTriangleToPrimalCell, PolygonClipping.h:85-118).
 bool ClipToCell(std::vector<V2>& poly, int i, int j, double cw, double ch);
 
-// Assemble mass and stiffness over interior and boundary cells (MassAndStiffness.h:5-549).
+// Cell-based stiffness right-hand side of the sharpening brush (MassAndStiffness.h:296-395,
+// IterativeSolvers.inl:1170-1208): for signal columns f_k, every active cell's stiffness applied to
+// f_k, kept per (texel, cell), so that per-cell weights w give t_k = sum_c w_c S_c f_k (w = 1:
+// t_k = S f_k).  Deep texels have exactly their four interior cells; boundary texels a sparse row.
+struct CellStiffnessRhs {
+  int K = 0, numCells = 0;
+  std::vector<double> deep;            // [K][4 numDeep]: deep texel d, cell slot s at 4d + s; slots
+                                       // (i-1,j-1), (i,j-1), (i-1,j), (i,j) around texel (i,j)
+  std::vector<int> deepCells;          // [2 numDeep]: global cells (i-1,j-1) and (i-1,j); +1 gives (i,j-1), (i,j)
+  std::vector<int> bRowptr{0}, bCell;  // boundary rows (boundary index) -> global cells
+  std::vector<double> bVal;            // [K][nnz]
+};
+
+// Assemble mass and stiffness over interior and boundary cells (MassAndStiffness.h:5-549).  With
+// `cellRhs`, also the cell-based stiffness products of the K signal columns `signal` (host, [K][N]
+// in global texel order).
 int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric,
-                         FineOperator& op, std::string& err);
+                         FineOperator& op, std::string& err, const double* signal = nullptr, int K = 0,
+                         CellStiffnessRhs* cellRhs = nullptr);
 
 }  // namespace gdtp

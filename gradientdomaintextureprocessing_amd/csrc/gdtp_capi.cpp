@@ -36,6 +36,13 @@ static const VectorFieldSamples& Samples(const gdtp_hierarchy* h) {
   return *h->vf;
 }
 
+struct gdtp_modulation {
+  DevModulation d;       // on the GPU when one is present
+  bool onDevice = false;
+  CellStiffnessRhs host;  // kept on the host only without a GPU (set-up checks on CPU machines)
+  int N = 0, D = 0, B = 0;
+};
+
 struct gdtp_system {
   const gdtp_hierarchy* H = nullptr;
   double screen = 1, stiff = 0;
@@ -250,6 +257,19 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
   } else if (name == "nodeType") { for (auto& c : L.charts) for (auto t : c.nodeType) v.push_back(t); }
   else if (name == "cellType") { for (auto& c : L.charts) for (auto t : c.cellType) v.push_back(t); }
   else if (name == "chartTexelIndex") { for (auto& c : L.charts) v.insert(v.end(), c.globalTexel.begin(), c.globalTexel.end()); }
+  else if (name == "cellInfo") {  // per active cell in global order: chart, i, j, type (0 boundary, 1 interior)
+    v.assign(4 * (size_t)L.numCells, 0);
+    for (int ch = 0; ch < (int)L.charts.size(); ch++) {
+      const GridChart& g = L.charts[ch];
+      for (int j = 0; j + 1 < g.height; j++)
+        for (int i = 0; i + 1 < g.width; i++) {
+          const int lc = g.localCell[(size_t)j * (g.width - 1) + i];
+          if (lc < 0) continue;
+          const size_t at = 4 * (size_t)(g.cellOffset + lc);
+          v[at] = ch, v[at + 1] = i, v[at + 2] = j, v[at + 3] = g.cell(i, j);
+        }
+    }
+  }
   else if (CsrI32(L.boundaryRestriction, name, "boundaryRestriction", v) || CsrI32(L.boundaryCoarseFineProlongation, name, "boundaryCoarseFineProlongation", v) ||
            CsrI32(L.boundaryFineCoarseRestriction, name, "boundaryFineCoarseRestriction", v)) {
   } else if (level == 0 && name == "vf.cellCorners") v = Samples(h).cellCorners;
@@ -554,3 +574,77 @@ int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol
     return GDTP_OK;
   })
 }
+
+// ---------------------------------------------------------------- stiffness modulation
+int gdtp_modulation_create(const gdtp_hierarchy* h, const double* signal, int ncols, gdtp_modulation** out) {
+  if (!h || !signal || !out || ncols < 1 || ncols > 1024) return Fail(GDTP_ERROR_INVALID, "invalid modulation arguments");
+  GDTP_TRY({
+    auto m = std::make_unique<gdtp_modulation>();
+    FineOperator op;
+    std::string err;
+    if (!AssembleFineOperator(h->mesh, h->h, h->metric, op, err, signal, ncols, &m->host)) return Fail(GDTP_ERROR_GEOMETRY, err);
+    const Level& L0 = h->h.levels[0];
+    m->N = L0.numTexels;
+    m->D = L0.numDeep;
+    m->B = L0.numBoundary;
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess) devices = 0;
+    if (devices > 0) {
+      UploadModulation(L0, m->host, m->d);
+      m->onDevice = true;
+      m->host = CellStiffnessRhs();
+    }
+    *out = m.release();
+    return GDTP_OK;
+  })
+}
+
+int gdtp_modulated_stiffness(const gdtp_modulation* m, const double* mask, double* out, void* stream) {
+  if (!m || !mask || !out) return Fail(GDTP_ERROR_INVALID, "null argument");
+  if (!m->onDevice) return Fail(GDTP_ERROR_DEVICE, "modulation was built without a GPU");
+  GDTP_TRY({
+    ModulatedStiffness(m->d, mask, out, (hipStream_t)stream);
+    GDTP_HIP(hipGetLastError());
+    return GDTP_OK;
+  })
+}
+
+int gdtp_modulation_info(const gdtp_modulation* m, int* info) {
+  if (!m || !info) return Fail(GDTP_ERROR_INVALID, "null argument");
+  if (m->onDevice) info[0] = m->d.K, info[2] = m->d.numCells, info[3] = (int)m->d.nnz;
+  else info[0] = m->host.K, info[2] = m->host.numCells, info[3] = (int)m->host.bCell.size();
+  info[1] = m->N;
+  return GDTP_OK;
+}
+
+template <class T>
+static int DeviceArray(const T* p, size_t n, T* out, size_t* count) {
+  if (!count) return Fail(GDTP_ERROR_INVALID, "null count");
+  if (out) {
+    if (*count < n) return Fail(GDTP_ERROR_INVALID, "output buffer too small");
+    if (n) GDTP_TRY({ GDTP_HIP(hipMemcpy(out, p, n * sizeof(T), hipMemcpyDeviceToHost)); })
+  }
+  *count = n;
+  return GDTP_OK;
+}
+
+int gdtp_modulation_array_f64(const gdtp_modulation* m, const char* name, double* out, size_t* count) {
+  if (!m || !name) return Fail(GDTP_ERROR_INVALID, "null argument");
+  const DevModulation& d = m->d;
+  const std::string n(name);
+  if (n == "deep") return m->onDevice ? DeviceArray(d.deep, (size_t)d.K * 4 * d.D, out, count) : Emit(m->host.deep, out, count);
+  if (n == "boundaryVal") return m->onDevice ? DeviceArray(d.bVal, (size_t)d.K * d.nnz, out, count) : Emit(m->host.bVal, out, count);
+  return Fail(GDTP_ERROR_INVALID, "unknown modulation array " + n);
+}
+
+int gdtp_modulation_array_i32(const gdtp_modulation* m, const char* name, int* out, size_t* count) {
+  if (!m || !name) return Fail(GDTP_ERROR_INVALID, "null argument");
+  const DevModulation& d = m->d;
+  const std::string n(name);
+  if (n == "deepCells") return m->onDevice ? DeviceArray(d.deepCells, 2 * (size_t)d.D, out, count) : Emit(m->host.deepCells, out, count);
+  if (n == "boundaryRowptr") return m->onDevice ? DeviceArray(d.bRowptr, (size_t)d.B + 1, out, count) : Emit(m->host.bRowptr, out, count);
+  if (n == "boundaryCell") return m->onDevice ? DeviceArray(d.bCell, (size_t)d.nnz, out, count) : Emit(m->host.bCell, out, count);
+  return Fail(GDTP_ERROR_INVALID, "unknown modulation array " + n);
+}
+
+void gdtp_modulation_destroy(gdtp_modulation* m) { delete m; }

@@ -115,4 +115,23 @@ void UploadVectorField(const VectorFieldSamples& vs, DevVectorField& d);
 // overwritten.
 void VectorFieldDivergence(const DevVectorField& d, const double* u, int ldu, double* rhs, int ldr, int N, int K, hipStream_t s);
 
+// Per-cell modulated stiffness of fixed signal columns (the sharpening brush's right-hand side,
+// CellStiffnessToTexelStiffness, IterativeSolvers.inl:1170-1208): t_k = sum_c mask_c S_c f_k.
+struct DevModulation {
+  int K = 0, N = 0, D = 0, B = 0, numCells = 0;
+  long nnz = 0;
+  double* deep = nullptr;       // [K][4D]: the four cells around each deep texel
+  int* deepCells = nullptr;     // [2D]: cells (i-1,j-1) and (i-1,j) of deep texel d
+  int* deepGlobal = nullptr;    // [D]
+  int *bRowptr = nullptr, *bCell = nullptr, *bGlobal = nullptr;
+  double* bVal = nullptr;       // [K][nnz]
+  size_t bytes = 0;
+  std::vector<void*> allocations;
+  ~DevModulation();
+};
+void UploadModulation(const Level& L0, const CellStiffnessRhs& c, DevModulation& m);
+// out[k][t] = sum over the cells c of texel t of mask[c] * (S_c f_k)_t for every level-0 texel t
+// (out is [K][N], overwritten; mask has one weight per active cell).
+void ModulatedStiffness(const DevModulation& m, const double* mask, double* out, hipStream_t s);
+
 }  // namespace gdtp

@@ -824,4 +824,62 @@ void VectorFieldDivergence(const DevVectorField& d, const double* u, int ldu, do
                                                                 ldr, K);
 }
 
+// ------------------------------------------------------------------ modulated stiffness
+// Deep texels: the four cell products weighted by the masks of the cells (i-1,j-1), (i,j-1),
+// (i-1,j), (i,j), summed in that order (IterativeSolvers.inl:1184-1192).
+__global__ __launch_bounds__(256) void KModulatedDeep(int D, const int* __restrict__ deepGlobal, const int* __restrict__ cells,
+                                                      const double* __restrict__ coef, const double* __restrict__ mask, double* out, int N,
+                                                      int K) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D) return;
+  const int cp = cells[2 * d], cn = cells[2 * d + 1];
+  const double m0 = mask[cp], m1 = mask[cp + 1], m2 = mask[cn], m3 = mask[cn + 1];
+  const long g = deepGlobal[d];
+  for (int k = 0; k < K; k++) {
+    const double* c = coef + (long)k * 4 * D + 4L * d;
+    out[(long)k * N + g] = c[0] * m0 + c[1] * m1 + c[2] * m2 + c[3] * m3;
+  }
+}
+
+// Boundary texels: a sparse row over cells (IterativeSolvers.inl:1177-1181).
+__global__ __launch_bounds__(256) void KModulatedBoundary(int B, const int* __restrict__ bGlobal, const int* __restrict__ rp,
+                                                          const int* __restrict__ cell, const double* __restrict__ val, long nnz,
+                                                          const double* __restrict__ mask, double* out, int N, int K) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  const int e0 = rp[r], e1 = rp[r + 1];
+  const long g = bGlobal[r];
+  for (int k = 0; k < K; k++) {
+    const double* v = val + (long)k * nnz;
+    double s = 0.0;
+    for (int e = e0; e < e1; e++) s += v[e] * mask[cell[e]];
+    out[(long)k * N + g] = s;
+  }
+}
+
+DevModulation::~DevModulation() {
+  for (void* p : allocations) (void)hipFree(p);
+}
+
+void UploadModulation(const Level& L0, const CellStiffnessRhs& c, DevModulation& m) {
+  m.K = c.K, m.N = L0.numTexels, m.D = L0.numDeep, m.B = L0.numBoundary, m.numCells = c.numCells;
+  m.nnz = (long)c.bCell.size();
+  const size_t before = g_uploadBytes;
+  m.deep = UploadTo(m.allocations, c.deep);
+  m.deepCells = UploadTo(m.allocations, c.deepCells);
+  m.deepGlobal = UploadTo(m.allocations, L0.deepGlobalIndex);
+  m.bRowptr = UploadTo(m.allocations, c.bRowptr), m.bCell = UploadTo(m.allocations, c.bCell);
+  m.bGlobal = UploadTo(m.allocations, L0.boundaryGlobalIndex);
+  m.bVal = UploadTo(m.allocations, c.bVal);
+  m.bytes = g_uploadBytes - before;
+}
+
+void ModulatedStiffness(const DevModulation& m, const double* mask, double* out, hipStream_t s) {
+  if (m.D > 0)
+    KModulatedDeep<<<(unsigned)((m.D + 255) / 256), 256, 0, s>>>(m.D, m.deepGlobal, m.deepCells, m.deep, mask, out, m.N, m.K);
+  if (m.B > 0)
+    KModulatedBoundary<<<(unsigned)((m.B + 255) / 256), 256, 0, s>>>(m.B, m.bGlobal, m.bRowptr, m.bCell, m.bVal, m.nnz, mask, out, m.N,
+                                                                     m.K);
+}
+
 }  // namespace gdtp

@@ -34,6 +34,7 @@ enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
 typedef struct gdtp_mesh gdtp_mesh;
 typedef struct gdtp_hierarchy gdtp_hierarchy;
 typedef struct gdtp_system gdtp_system;
+typedef struct gdtp_modulation gdtp_modulation;
 
 const char* gdtp_last_error(void);
 int gdtp_device_count(int* count);
@@ -109,6 +110,27 @@ int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, i
  * Replaces IntergrateNormalizedVectorFieldFromSamples (VectorFieldIntergration.inl:452-582) with
  * the boundary prolongation / restriction around it (Geodesics.cpp:522-535).  rhs is overwritten. */
 int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rhs, int ncols, void* stream);
+
+/* ---- stiffness modulation (the sharpening brush of TextureFiltering)
+ * Cell-based stiffness of fixed signal columns: every active level-0 cell's stiffness applied to
+ * each column of `signal` (host, [ncols][N] in global texel order), kept per (texel, cell).
+ * Replaces InitializeMassAndStiffness(..., computeCellBasedStiffness = true, ...) and the
+ * texelToCellCoeffs / boundaryCellBasedStiffnessRHSMatrix it fills (MassAndStiffness.h:296-395,
+ * 490-549; TextureFiltering.cpp:621-649).  Cells are numbered as the hierarchy array "cellInfo". */
+int gdtp_modulation_create(const gdtp_hierarchy* h, const double* signal, int ncols, gdtp_modulation** out);
+/* out[c][t] = sum over the cells k of texel t of mask[k] * (S_k signal_c)_t, all level-0 texels
+ * (mask: device, one weight per cell; out: device [ncols][N], overwritten).  With mask = 1 this is
+ * S signal.  Replaces CellStiffnessToTexelStiffness (IterativeSolvers.inl:1170-1208), whose result
+ * the filtering rhs adds as M f + alpha * out (TextureFiltering.cpp:569-570). */
+int gdtp_modulated_stiffness(const gdtp_modulation* m, const double* mask, double* out, void* stream);
+/* info[0..3] = ncols, texels, cells, boundary entries. */
+int gdtp_modulation_info(const gdtp_modulation* m, int* info);
+/* Device arrays copied to the host (count in / out as for the hierarchy arrays): "deep" (f64,
+ * [ncols][4 D]), "boundaryVal" (f64, [ncols][nnz]), "deepCells" (i32, [2 D]), "boundaryRowptr",
+ * "boundaryCell" (i32). */
+int gdtp_modulation_array_f64(const gdtp_modulation* m, const char* name, double* out, size_t* count);
+int gdtp_modulation_array_i32(const gdtp_modulation* m, const char* name, int* out, size_t* count);
+void gdtp_modulation_destroy(gdtp_modulation* m);
 
 /* ---- solves to a relative residual (every column: ||b - A x|| <= tol ||b||)
  * mode GDTP_SOLVE_FP64_REFERENCE: reference V-cycles with an FP64 residual check per cycle.

@@ -292,3 +292,24 @@ def vf_divergence(S, u, N):
             X, Y = _normalise(n, S["gten"])
             np.add.at(out[k], S["gtex"], X[seg] * S["gV"][:, 0] + Y[seg] * S["gV"][:, 1])
     return out
+
+
+def modulated_stiffness(deep, deep_cells, deep_global, b_rowptr, b_cell, b_val, b_global, mask, K, N):
+    """CellStiffnessToTexelStiffness (include/Src/IterativeSolvers.inl:1170-1208): boundary texels
+    take their sparse row over cells times the cell mask (:1177-1181); deep texel d takes its four
+    cell products weighted by the masks of the cells (i-1,j-1), (i,j-1), (i-1,j), (i,j), summed in
+    that order (:1184-1192).  deep: [K][4D]; deep_cells: [2D] (the two left cells; the right ones
+    follow); b_val: [K][nnz].  Returns [K][N]."""
+    D = len(deep_global)
+    out = np.zeros((K, N))
+    cp, cn = deep_cells[0::2], deep_cells[1::2]
+    m = np.stack([mask[cp], mask[cp + 1], mask[cn], mask[cn + 1]], axis=1)
+    c = deep.reshape(K, D, 4)
+    for k in range(K):
+        out[k, deep_global] = ((c[k, :, 0] * m[:, 0] + c[k, :, 1] * m[:, 1]) + c[k, :, 2] * m[:, 2]) + c[k, :, 3] * m[:, 3]
+    nnz = len(b_cell)
+    v = b_val.reshape(K, nnz)
+    rows = np.repeat(np.arange(len(b_global)), np.diff(b_rowptr))
+    for k in range(K):
+        out[k, b_global] = np.bincount(rows, weights=v[k] * mask[b_cell], minlength=len(b_global))
+    return out

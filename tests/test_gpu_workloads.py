@@ -1,0 +1,105 @@
+"""The batched-geodesics and line-integral-convolution workloads at their full sizes (2048²
+atlases), checked through size-independent properties:
+
+geodesics (noisy icosphere, 64 sources as columns, the reference's heat time 1e-3): every
+column meets the tolerance,
+phi is zero at its source, columns solved in the 64-wide batch agree with the same sources solved
+alone, and d(a, b) ≈ d(b, a) for a pair of sources (the heat method is symmetric up to its
+discretisation error);
+
+LIC (noisy torus, anisotropic metric, 16 diffusion lengths as columns): a constant texture solves
+every column's system exactly ((M_a + l S_a) 1 = M_a 1 since S_a 1 = 0, so the weighted solve
+starting there stops at once; a dropped stencil or boundary term breaks it); the reference
+schedule's V-cycles reduce the residual every round and give a column the same values alone and
+inside the 16-column batch.  (Geometric multigrid does not converge on this anisotropy, so no
+solve-to-tolerance is asked of it.)"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+import gradientdomaintextureprocessing_amd as g  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def geodesics():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mesh = g.TexturedMesh.icosphere(7, 2048, 2048, 0.05, 3)
+    H = g.InitializeHierarchy(mesh, 2048, 2048, 7)
+    N = H.level_info(0)["texels"]
+    geo = g.Geodesics(H, heat_time=1e-3, max_sources=64)
+    print(f"[geodesics 2048] N={N}", flush=True)
+    return H, N, geo
+
+
+def test_geodesics_batch_of_64(geodesics):
+    H, N, geo = geodesics
+    src = [int(v) for v in np.random.default_rng(7).choice(N, 64, replace=False)]
+    phi = geo.distances(src, tol=1e-6, mode="mixed", max_iters=200)
+    rows = torch.arange(64, device="cuda")
+    assert torch.all(phi[rows, torch.as_tensor(src, device="cuda")] == 0)
+    scale = phi.abs().max().item()
+    assert phi.min().item() >= -0.02 * scale
+    # two of the sources alone
+    pair = geo.distances([src[5], src[40]], tol=1e-6, mode="mixed", max_iters=200)
+    diff = (pair - phi[[5, 40]]).abs().max().item()
+    print(f"[geodesics 2048] max |phi| {scale:.3f}, batch vs alone {diff:.2e}", flush=True)
+    assert diff <= 1e-3 * scale
+    ab, ba = phi[5, src[40]].item(), phi[40, src[5]].item()
+    print(f"[geodesics 2048] d(a,b) {ab:.4f} d(b,a) {ba:.4f}", flush=True)
+    assert abs(ab - ba) <= 0.05 * max(ab, ba)
+
+
+@pytest.fixture(scope="module")
+def lic():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mesh = g.TexturedMesh.torus(512, 256, 8, 4, 2048, 2048, 1.0, 0.35, 0.05, 5)
+    v, t, _ = mesh.arrays()
+    f = np.sin(3 * v[:, 0]) + np.cos(2 * v[:, 1]) + np.sin(4 * v[:, 2]) + 0.5 * np.sin(5 * v[:, 0] * v[:, 1])
+    lic = g.LineIntegralConvolution(mesh, 2048, 2048, 7, g.surface_gradient_field(v, t, f), max_cols=16)
+    print(f"[lic 2048] N={lic.anisotropic.level_info(0)['texels']}", flush=True)
+    return lic
+
+
+def test_lic_constant_texture_solves_every_column(lic):
+    S = lic.lic
+    N = lic.anisotropic.level_info(0)["texels"]
+    ones = torch.ones(16, N, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(ones)
+    S.apply(g.OPERATOR_MASS, ones, b)
+    lams = np.logspace(-5, -3, 16)
+    s1 = torch.empty_like(ones)
+    S.apply(g.OPERATOR_STIFFNESS, ones, s1)
+    res = (s1 * torch.tensor(lams, device="cuda")[:, None]).norm(dim=1) / b.norm(dim=1)
+    print(f"[lic 2048] |l S_a 1| / |M_a 1| per column: max {res.max().item():.1e}", flush=True)
+    assert res.max().item() <= 1e-7  # rounding of the strongly anisotropic stencils; a dropped term is O(1e-2) or more
+    x = ones.clone()
+    its, rel = S.solve_weighted(x, b, [[1.0, float(l)] for l in lams], tol=1e-6, max_iters=5)
+    assert its == 0 and rel.max() <= 1e-6 and torch.equal(x, ones)
+
+
+def test_lic_reference_rounds_alone_and_batched(lic):
+    S = lic.lic
+    N = lic.anisotropic.level_info(0)["texels"]
+    noise = torch.from_numpy(np.random.default_rng(9).random((16, N), dtype=np.float32).astype(np.float64)).cuda()
+    b = torch.empty_like(noise)
+    S.apply(g.OPERATOR_MASS, noise, b)
+    x = torch.zeros_like(b)
+    xa = torch.zeros((1, N), dtype=torch.float64, device="cuda")
+    ba = b[3:4].clone()
+    r = torch.empty_like(b)
+    prev = torch.ones(16, dtype=torch.float64, device="cuda")
+    for _ in range(4):
+        S.vcycle(x, b)
+        S.vcycle(xa, ba)
+        S.residual(x, b, r)
+        rel = r.norm(dim=1) / b.norm(dim=1)
+        assert torch.all(rel < prev)
+        prev = rel
+    print(f"[lic 2048] rel res after 4 rounds {prev.max().item():.2e}", flush=True)
+    diff = (x[3] - xa[0]).abs().max().item()
+    print(f"[lic 2048] column 3 alone vs in the batch: {diff:.1e}", flush=True)
+    assert diff <= 1e-12 * x[3].abs().max().item()
