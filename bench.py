@@ -146,19 +146,24 @@ def profile_kernel(cfg, K, which, reps=10):
     log(f"[bench] profile-kernel {which}: {reps} launches done")
 
 
-SERVING_MESHES = [  # name, kind, params, width
+SERVING_MESHES = [  # name, kind, params, width (tori also serve line integral convolution)
     ("cube512", "cube", dict(subdiv=16), 512),
+    ("cube1024", "cube", dict(subdiv=16), 1024),
     ("ico1024", "icosphere", dict(levels=5, amplitude=0.05, seed=2), 1024),
-    ("torus1024", "torus", dict(nu=128, nv=64, cu=4, cv=2, amplitude=0.05, seed=3), 1024),
     ("ico2048", "icosphere", dict(levels=6, amplitude=0.05, seed=4), 2048),
+    ("ico4096", "icosphere", dict(levels=7, amplitude=0.05, seed=1), 4096),
+    ("torus512", "torus", dict(nu=128, nv=64, cu=4, cv=2, amplitude=0.05, seed=6), 512),
+    ("torus1024", "torus", dict(nu=256, nv=128, cu=8, cv=4, amplitude=0.05, seed=3), 1024),
+    ("torus2048", "torus", dict(nu=512, nv=256, cu=8, cv=4, amplitude=0.05, seed=5), 2048),
 ]
 
 
 def serving_bench(qps, n_requests, tol, seed=0):
     """Request-loop measurement: the meshes above stay resident; requests arrive as a Poisson
     stream at `qps`: 60 % filtering (stiffness log-uniform in [1e-4, 1e-2], 3 channels, a uniform
-    [0,1) texture), 40 % geodesics (1-8 random source texels).  Reports p50/p99 latency per class
-    and the solved columns per second."""
+    [0,1) texture), 30 % geodesics (1-8 random source texels), 10 % line integral convolution on a
+    torus (3 noise channels, the reference's 4 rounds).  Reports p50/p99 latency per class and the
+    columns per second."""
     import threading
     import torch
     import gradientdomaintextureprocessing_amd as g
@@ -167,37 +172,54 @@ def serving_bench(qps, n_requests, tol, seed=0):
     server = SolverServer(tol=tol, max_iters=120)
     sizes = {}
     for name, kind, p, W in SERVING_MESHES:
+        levels = max(4, int(np.log2(W)) - 4)
+        lic = None
         if kind == "cube":
             mesh = g.TexturedMesh.cube(p["subdiv"], W, W)
         elif kind == "icosphere":
             mesh = g.TexturedMesh.icosphere(p["levels"], W, W, p["amplitude"], p["seed"])
         else:
             mesh = g.TexturedMesh.torus(p["nu"], p["nv"], p["cu"], p["cv"], W, W, 1.0, 0.35, p["amplitude"], p["seed"])
-        levels = max(4, int(np.log2(W)) - 3)
+            v, t, _ = mesh.arrays()
+            ph = np.random.default_rng(p["seed"]).random(3) * 6.28
+            field = g.surface_gradient_field(v, t, np.sin(3 * v[:, 0] + ph[0]) + np.cos(2 * v[:, 1] + ph[1]) + np.sin(4 * v[:, 2] + ph[2]))
+            lic = g.LineIntegralConvolution(mesh, W, W, levels, field, max_cols=8)
         H = g.InitializeHierarchy(mesh, W, W, levels)
-        server.add_mesh(name, H, max_cols=8)
+        server.add_mesh(name, H, max_cols=8, lic=lic)
         sizes[name] = H.level_info(0)["texels"]
+        log(f"[bench] serving: {name} resident ({sizes[name]} texels, {levels} levels), {time.time() - t0:.1f}s")
     log(f"[bench] serving: {len(SERVING_MESHES)} meshes resident in {time.time() - t0:.1f}s, {server.used / 2**30:.2f} GiB")
     rng = np.random.default_rng(seed)
     names = [m[0] for m in SERVING_MESHES]
     plan = []
+    tori = [m[0] for m in SERVING_MESHES if m[1] == "torus"]
     for i in range(n_requests):
         n = names[rng.integers(len(names))]
-        if rng.random() < 0.6:
+        u = rng.random()
+        if u < 0.6:
             plan.append(("filter", n, float(10 ** rng.uniform(-4, -2))))
-        else:
+        elif u < 0.9:
             plan.append(("geodesic", n, [int(v) for v in rng.integers(0, sizes[n], rng.integers(1, 9))]))
+        else:
+            plan.append(("lic", tori[rng.integers(len(tori))], None))
     payloads = {n: torch.rand(3, sizes[n], dtype=torch.float64, device="cuda") for n in names}
     for n in names:  # warm every (mesh, kind) path once: graph capture and first-use set-up
         server.submit_filter(n, payloads[n], 1e-3).wait(300)
         server.submit_geodesic(n, [0]).wait(300)
+        if n in tori:
+            server.submit_lic(n, payloads[n]).wait(300)
     gaps = rng.exponential(1.0 / qps, n_requests)
     handles = []
     torch.cuda.synchronize()
     start = time.perf_counter()
     for (kind, n, a), gap in zip(plan, gaps):
         time.sleep(gap)
-        h = server.submit_filter(n, payloads[n], a) if kind == "filter" else server.submit_geodesic(n, a)
+        if kind == "filter":
+            h = server.submit_filter(n, payloads[n], a)
+        elif kind == "geodesic":
+            h = server.submit_geodesic(n, a)
+        else:
+            h = server.submit_lic(n, payloads[n])
         handles.append((kind, n, h))
     for _, _, h in handles:
         h.wait(600)
@@ -207,8 +229,11 @@ def serving_bench(qps, n_requests, tol, seed=0):
     out = {"metric": "serving: request latency at rel-residual %g" % tol, "qps_offered": qps, "requests": n_requests,
            "wall_s": el, "columns_per_s": cols / el, "resident_gib": server.used / 2**30,
            "all": latency_percentiles([h for _, _, h in handles])}
-    for kind in ("filter", "geodesic"):
+    for kind in ("filter", "geodesic", "lic"):
         out[kind] = latency_percentiles([h for k, _, h in handles if k == kind])
+        its = [h.iterations for k, _, h in handles if k == kind]
+        if its:
+            out[kind]["iterations_p50_max"] = [float(np.percentile(its, 50)), int(max(its))]
     out["filter_1024"] = latency_percentiles([h for k, n, h in handles if k == "filter" and n.endswith("1024")])
     out["meshes"] = {n: sizes[n] for n in names}
     print(json.dumps(out))

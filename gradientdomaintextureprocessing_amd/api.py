@@ -463,37 +463,73 @@ class LineIntegralConvolution:
         self.lic = UpdateLinearSystem(self.anisotropic, convolution_weight, 1.0, max_cols=max_cols)
         self.sharpen = UpdateLinearSystem(self.embedding, stiffness_weight, 1.0, max_cols=max_cols)
 
-    def convolve(self, noise, passes=4, tol=None, mode="mixed", max_iters=200):
+    def convolve(self, noise, passes=4, tol=None, mode="fp64", max_iters=200, between_rounds=None):
         """(x, y): the anisotropically diffused noise and its sharpened version ([K, N] FP64).
 
         Default (tol=None): the reference's schedule (LineIntegralConvolution.cpp:503-527, 707),
-        `passes` rounds of one reference-schedule V-cycle on the LIC system, the sharpening rhs
-        from the current x, one V-cycle on the sharpening system, from x = y = 0.  Strongly
-        anisotropic diffusion converges slowly under geometric multigrid, so the app runs a
-        fixed number of rounds.  With tol: both systems solved to that relative residual
-        (flexible CG around the mixed V-cycle)."""
+        `passes` rounds of one V-cycle on the LIC system, the sharpening rhs from the current x,
+        one V-cycle on the sharpening system, from x = y = 0.  Strongly anisotropic diffusion
+        converges slowly under geometric multigrid, so the app runs a fixed number of rounds.
+        mode="fp64" runs the reference-schedule V-cycle; mode="mixed" one mixed-precision
+        refinement step instead (x += V(b - A x) with the FP32 V-cycle: short kernels, but a
+        different smoother, so after a few unconverged rounds its iterate is not the reference's).
+        `between_rounds` is called after each half-round (each V-cycle).  With tol: both systems solved to that
+        relative residual (flexible CG around the mixed V-cycle)."""
         import torch
-        b = torch.empty_like(noise)
+        K = noise.shape[0]
+        buf = self._buffers(K, noise)
+        x, y, b, r = buf["x"], buf["y"], buf["b"], buf["r"]
         self.lic.apply(OPERATOR_MASS, noise, b)
-        x, y = torch.zeros_like(noise), torch.zeros_like(noise)
-        m, s = torch.empty_like(x), torch.empty_like(x)
+        x.zero_()
+        y.zero_()
 
         def sharpen_rhs():
+            m, s = buf["m"], buf["s"]
             self.sharpen.apply(OPERATOR_MASS, x, m)
             self.sharpen.apply(OPERATOR_STIFFNESS, x, s)
-            return (m + (self.sharpening * self.stiffness) * s).contiguous()
+            torch.add(m, s, alpha=self.sharpening * self.stiffness, out=r)
+            return r
+
+        def one_round(system, u, rhs):
+            if mode == "fp64":
+                system.vcycle(u, rhs)
+                return
+            try:  # one refinement step; "not converged" is the expected outcome
+                system.solve(u, rhs, tol=1e-300, max_iters=1, mode="mixed")
+            except _lib.GdtpError as e:
+                if e.code != 3:
+                    raise
 
         if tol is None:
+            if mode != "fp64" and getattr(self, "_round_schedule", None) != "mixed":
+                for system in (self.lic, self.sharpen):
+                    system.configure_mixed(cycles=1)  # one V-cycle per round, as the reference's
+                self._round_schedule = "mixed"
             for _ in range(passes):
-                self.lic.vcycle(x, b)
-                self.sharpen.vcycle(y, sharpen_rhs())
-            return x, y
+                one_round(self.lic, x, b)
+                if between_rounds is not None:  # e.g. a serving loop letting short requests run
+                    between_rounds()
+                one_round(self.sharpen, y, sharpen_rhs())
+                if between_rounds is not None:
+                    between_rounds()
+            return x.clone(), y.clone()
         for system in (self.lic, self.sharpen):
             system.configure_mixed(krylov=True)
+        self._round_schedule = None
         self.lic.solve(x, b, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
         y.copy_(x)
         self.sharpen.solve(y, sharpen_rhs(), tol=tol, max_iters=max_iters, mode=mode)
-        return x, y
+        return x.clone(), y.clone()
+
+    def _buffers(self, K, like):
+        """Device vectors per column count, reused across calls (fixed pointers let the mixed
+        solver replay its captured steps)."""
+        import torch
+        if not hasattr(self, "_buf"):
+            self._buf = {}
+        if K not in self._buf:
+            self._buf[K] = {k: torch.zeros(K, like.shape[1], dtype=torch.float64, device=like.device) for k in ("x", "y", "b", "r", "m", "s")}
+        return self._buf[K]
 
 
 # ---------------------------------------------------------------------------- file formats and the apps' preprocessing

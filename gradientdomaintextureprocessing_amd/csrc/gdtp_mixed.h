@@ -54,7 +54,7 @@ struct MixedSystem {
   // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
   bool krylov = false;
   // V-cycles per outer iteration (the level-0 correction carries over between them); 0 = automatic:
-  // 2 unless the operator is nearly singular for FP32 (see rowSumRatio), then 1
+  // 2 for plain refinement unless the operator is nearly singular for FP32 (see rowSumRatio), else 1
   int cycles = 0;
   // median over level-0 deep rows of (row sum / diagonal) of the FP64 operator.  Continuing the FP32
   // correction through a second V-cycle evaluates A e in FP32, whose stencil rounding (~1e-7 of the

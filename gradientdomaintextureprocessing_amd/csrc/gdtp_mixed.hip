@@ -1220,7 +1220,10 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     return ok;
   };
   if (converged()) return 0;
-  int cycles = ms.cycles > 0 ? ms.cycles : (ms.rowSumRatio >= kTwoCycleRatio ? 2 : 1);
+  // automatic: two V-cycles per residual for refinement on operators that FP32 represents well;
+  // one for flexible CG (a product of two V-cycles preconditions it worse once the columns'
+  // weights differ from the V-cycle's: 11 iterations of 2 cycles against 11 of 1 at a weight 3x off)
+  int cycles = ms.cycles > 0 ? ms.cycles : (!krylov && ms.rowSumRatio >= kTwoCycleRatio ? 2 : 1);
   // one refinement step = V-cycles on the current residual, update, new residual
   auto step = [&](hipStream_t st, int cyc) {
     for (int c = 0; c < cyc; c++) MixedVCycle(ds, ms, K, st);

@@ -6,16 +6,27 @@ Each resident mesh keeps its hierarchy, its multigrid systems and their factoris
   - "filter":   (M + a S) x = M f + s a S f for the request's a (TextureFiltering.cpp:562-718),
                 x starts at f, one column per channel;
   - "geodesic": heat-method distances from source texels (Geodesics.cpp:508-550), one column per
-                source.
+                source;
+  - "lic":      line integral convolution of a noise texture (LineIntegralConvolution.cpp:503-527)
+                on meshes admitted with a LineIntegralConvolution object: the reference's schedule
+                of `rounds` V-cycle rounds (anisotropic diffusion does not converge under geometric
+                multigrid, so no tolerance applies), yielding to short requests after every V-cycle
+                (its FP64 band kernels run for milliseconds and hold up the hardware queue they share
+                with short requests); one column per channel.
 
-Each mesh has two worker lanes, filtering and geodesics, with their own systems, buffers and HIP
-streams, so that short filtering requests never queue behind geodesic batches.  A lane forms
+Each mesh has a worker lane per kind (filtering, geodesics, LIC) with its own systems, buffers and
+HIP streams, so that short filtering requests never queue behind geodesic or LIC batches.  A lane forms
 batches from its queue as slots free up: requests are taken in arrival order while their columns
 fit the system's capacity, every column keeping its own operator weights (parametric solves share
 the bucket's V-cycle as preconditioner), so a request that arrives while others run joins the
-next batch.  Lanes of all meshes run concurrently; a geodesic lane holds its next batch back
-while any filtering batch runs (at most `yield_ms`), a host-side priority for the short requests
-(HIP stream priorities are not used).  Meshes are admitted against an HBM budget.  Results are the same alone
+next batch.  Short work — filtering on meshes of at most `short_texels` texels — runs concurrently
+on all meshes; everything else (geodesics, LIC, filtering on large meshes) is heavy: at most
+`heavy_concurrency` heavy batches run at a time over all meshes, and each holds back while any
+short batch runs (at most `yield_ms`): host-side priorities for the short requests (HIP stream
+priorities are not used).  Heavy batches also yield between chunks of their solves (one
+refinement iteration, or one LIC round): while short requests are queued or running, the next
+chunk waits (at most `yield_ms` each time), so a short request shares the GPU with at most one
+chunk.  Meshes are admitted against an HBM budget.  Results are the same alone
 and in a batch up to the requested tolerance: every column is solved to it.
 """
 import math
@@ -25,7 +36,7 @@ import time
 
 import numpy as np
 
-from . import api
+from . import _lib, api
 
 
 class Request:
@@ -40,7 +51,7 @@ class Request:
 
     @property
     def columns(self):
-        return int(self.payload.shape[0]) if self.kind == "filter" else len(self.payload)
+        return len(self.payload) if self.kind == "geodesic" else int(self.payload.shape[0])
 
     def wait(self, timeout=None):
         if not self.done.wait(timeout):
@@ -54,8 +65,11 @@ class ResidentMesh:
     """Hierarchy and systems of one mesh: filtering buckets (M + a_b S) and, for geodesics, a heat
     system (M + t S) and the distance system (1e-6 M + S)."""
 
-    def __init__(self, name, hierarchy, buckets=(1e-4, 1e-3, 1e-2), max_cols=8, geodesics=True, heat_time=1e-3):
+    def __init__(self, name, hierarchy, buckets=(1e-4, 1e-3, 1e-2), max_cols=8, geodesics=True, heat_time=1e-3, lic=None, lic_rounds=4,
+                 lic_mode="fp64"):
         self.name, self.hierarchy, self.max_cols, self.heat_time = name, hierarchy, max_cols, heat_time
+        self.lic, self.lic_rounds = lic, lic_rounds  # api.LineIntegralConvolution (its own hierarchies and systems) or None
+        self.lic_mode = lic_mode  # the reference's FP64 V-cycle rounds, or mixed-precision refinement steps
         self.buckets = sorted(buckets)
         self.systems = {a: api.UpdateLinearSystem(hierarchy, a, 1.0, max_cols=max_cols) for a in self.buckets}
         self.heat = api.UpdateLinearSystem(hierarchy, heat_time, 1.0, max_cols=max_cols) if geodesics else None
@@ -63,7 +77,11 @@ class ResidentMesh:
         self.N = hierarchy.level_info(0)["texels"]
 
     def all_systems(self):
-        return list(self.systems.values()) + [s for s in (self.heat, self.distance) if s is not None]
+        extra = [self.lic.lic, self.lic.sharpen] if self.lic is not None else []
+        return list(self.systems.values()) + [s for s in (self.heat, self.distance) if s is not None] + extra
+
+    def lanes(self):
+        return ["filter", "geodesic"] + (["lic"] if self.lic is not None else [])
 
     def bucket(self, a):
         return min(self.buckets, key=lambda b: abs(math.log(a) - math.log(b)))
@@ -80,11 +98,13 @@ class SolverServer:
     """Continuous-batching solve server (see the module docstring).  tol: relative residual per
     column; hbm_budget: bytes of device memory the resident systems may hold (None: no limit)."""
 
-    def __init__(self, tol=1e-6, hbm_budget=None, max_iters=80, yield_ms=25.0):
+    def __init__(self, tol=1e-6, hbm_budget=None, max_iters=80, yield_ms=100.0, heavy_concurrency=1, short_texels=1_500_000):
         import torch
         self.torch = torch
         self.tol, self.hbm_budget, self.max_iters, self.yield_ms = tol, hbm_budget, max_iters, yield_ms
-        self.filters_running = 0
+        self.heavy = threading.BoundedSemaphore(heavy_concurrency) if heavy_concurrency else None
+        self.short_texels = short_texels
+        self.short_pending = 0  # short requests submitted and not finished (queued or running)
         self.running_lock = threading.Condition()
         self.meshes = {}
         self.queues = {}
@@ -94,6 +114,13 @@ class SolverServer:
 
     # -- residency
     def add_mesh(self, name, hierarchy, warm=True, **kw):
+        """Admit a mesh.  Unless `buckets` is given, meshes of at most `short_texels` texels get five
+        stiffness buckets log-spaced over [1e-4, 1e-2] (their filtering is the latency-critical
+        class: a weight at most a quarter decade from its bucket keeps the weighted solve short),
+        larger ones three."""
+        if "buckets" not in kw:
+            small = hierarchy.level_info(0)["texels"] <= self.short_texels
+            kw["buckets"] = tuple(np.logspace(-4, -2, 5 if small else 3))
         m = ResidentMesh(name, hierarchy, **kw)
         for s in m.all_systems():
             s.configure_mixed()  # builds the mixed solver now: first requests should not pay for it
@@ -105,7 +132,7 @@ class SolverServer:
         if warm:
             self._warm(m)
         self.meshes[name] = m
-        for lane in ("filter", "geodesic"):
+        for lane in m.lanes():
             self.queues[(name, lane)] = queue.Queue()
             t = threading.Thread(target=self._work, args=(name, lane), daemon=True)
             self.workers[(name, lane)] = t
@@ -136,6 +163,9 @@ class SolverServer:
                 t0 -= t0.mean(dim=1, keepdim=True)
                 t1.zero_()
                 m.distance.solve(t1, t0, tol=1e-2, max_iters=self.max_iters, mode="mixed")
+        if m.lic is not None:  # every batch width's buffers and captured steps
+            for K in range(1, m.max_cols + 1):
+                m.lic.convolve(rand(K), passes=1, mode=m.lic_mode)
         torch.cuda.synchronize()
 
     # -- requests
@@ -150,6 +180,13 @@ class SolverServer:
         self._enqueue(r)
         return r
 
+    def submit_lic(self, mesh, noise):
+        """noise: [channels, N] FP64 GPU tensor (texels of the mesh's anisotropic hierarchy, which
+        has the same atlas and texel order)."""
+        r = Request(mesh, "lic", noise, 0.0)
+        self._enqueue(r)
+        return r
+
     def _enqueue(self, r):
         m = self.meshes.get(r.mesh)
         if m is None:
@@ -158,7 +195,15 @@ class SolverServer:
             raise ValueError(f"a request needs 1..{m.max_cols} columns")
         if r.kind == "geodesic" and m.distance is None:
             raise ValueError("mesh was added without geodesics")
+        if r.kind == "lic" and m.lic is None:
+            raise ValueError("mesh was added without a line integral convolution")
+        if self._short(m, r.kind):
+            with self.running_lock:
+                self.short_pending += 1
         self.queues[(r.mesh, r.kind)].put(r)
+
+    def _short(self, m, lane):
+        return lane == "filter" and m.N <= self.short_texels
 
     def close(self):
         self.stop.set()
@@ -191,22 +236,26 @@ class SolverServer:
                 pending.append(r)
             batch, rest = self._form_batch(m, pending)
             pending = rest
-            if lane == "geodesic":  # let running filtering batches finish first (bounded wait)
+            short = self._short(m, lane)
+            heavy = not short and self.heavy is not None
+            if heavy:
+                self.heavy.acquire()
+            if not short:  # let submitted short requests finish first (bounded wait)
+                self._yield_to_filters()
+            try:
+                with torch.cuda.stream(stream):
+                    try:
+                        self._run(m, batch, lane)
+                    except Exception as e:  # a failed batch fails its requests, not the server
+                        for r in batch:
+                            r.error = e
+                            r.done.set()
+            finally:
+                if heavy:
+                    self.heavy.release()
+            if short:
                 with self.running_lock:
-                    self.running_lock.wait_for(lambda: self.filters_running == 0, timeout=self.yield_ms / 1e3)
-            else:
-                with self.running_lock:
-                    self.filters_running += 1
-            with torch.cuda.stream(stream):
-                try:
-                    self._run(m, batch, lane)
-                except Exception as e:  # a failed batch fails its requests, not the server
-                    for r in batch:
-                        r.error = e
-                        r.done.set()
-            if lane == "filter":
-                with self.running_lock:
-                    self.filters_running -= 1
+                    self.short_pending -= len(batch)
                     self.running_lock.notify_all()
 
     @staticmethod
@@ -237,11 +286,32 @@ class SolverServer:
             m.buf[lane] = dict(x=z(), b=z(), t0=z(), t1=z())
         return m.buf[lane]
 
+    def _yield_to_filters(self):
+        with self.running_lock:
+            self.running_lock.wait_for(lambda: self.short_pending == 0, timeout=self.yield_ms / 1e3)
+
+    def _solve_in_chunks(self, system, x, b, chunk=1):
+        """Mixed solve to self.tol from x in chunks of `chunk` refinement iterations (each chunk
+        continues from the previous x), yielding to running filtering batches in between."""
+        done = 0
+        while True:
+            try:
+                its, _ = system.solve(x, b, tol=self.tol, max_iters=min(chunk, self.max_iters - done), mode="mixed")
+                return done + its
+            except _lib.GdtpError as e:
+                if e.code != 3:  # anything but "not converged yet" is an error
+                    raise
+                done += chunk
+                if done >= self.max_iters:
+                    raise
+            self._yield_to_filters()
+
     def _run(self, m, batch, lane):
         torch = self.torch
-        buf = self._buffers(m, lane)
         K = sum(r.columns for r in batch)
-        x, b, t0, t1 = (buf[k][:K] for k in ("x", "b", "t0", "t1"))
+        if lane != "lic":
+            buf = self._buffers(m, lane)
+            x, b, t0, t1 = (buf[k][:K] for k in ("x", "b", "t0", "t1"))
         if batch[0].kind == "filter":
             system = m.systems[m.bucket(batch[0].weight)]
             torch.cat([r.payload for r in batch], dim=0, out=t0)  # f
@@ -254,6 +324,14 @@ class SolverServer:
             x.copy_(t0)
             its, _ = system.solve_weighted(x, b, w, tol=self.tol, max_iters=self.max_iters)
             out = x
+        elif batch[0].kind == "lic":
+            noise = torch.cat([r.payload for r in batch], dim=0)
+            def between():  # V-cycles launch asynchronously: finish this one before yielding
+                torch.cuda.current_stream().synchronize()
+                self._yield_to_filters()
+
+            _, out = m.lic.convolve(noise, passes=m.lic_rounds, mode=m.lic_mode, between_rounds=between)
+            its = m.lic_rounds
         else:
             system = m.heat
             src = torch.as_tensor([s for r in batch for s in r.payload], device="cuda")
@@ -261,10 +339,10 @@ class SolverServer:
             b.zero_()
             b[rows, src] = 1.0
             x.zero_()
-            its, _ = system.solve(x, b, tol=self.tol, max_iters=self.max_iters, mode="mixed")
+            its = self._solve_in_chunks(system, x, b)
             system.geodesic_divergence(x, t0)
             t1.zero_()
-            its2, _ = m.distance.solve(t1, t0, tol=self.tol, max_iters=self.max_iters, mode="mixed")
+            its2 = self._solve_in_chunks(m.distance, t1, t0)
             its += its2
             t1 -= t1[rows, src][:, None]
             out = t1
@@ -282,4 +360,6 @@ class SolverServer:
 
 def latency_percentiles(requests):
     lat = np.array([r.latency for r in requests if r.latency is not None]) * 1e3
+    if len(lat) == 0:
+        return dict(p50=None, p99=None, n=0)
     return dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)), n=int(len(lat)))

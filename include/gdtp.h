@@ -161,8 +161,9 @@ int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg);
 /* V-cycles per outer iteration of the mixed solver (1..16, or 0 = automatic, the default): with
  * c > 1 the FP32 correction e of A e = r takes c V-cycles (each continuing from the previous e)
  * before the FP64 update and residual, so the FP64 passes over the operator run once per c
- * V-cycles.  Automatic: 2, or 1 for operators that are nearly singular in FP32 (median row sum /
- * diagonal of level 0 below 2e-6, e.g. the geodesic distance system 1e-6 M + S).  A solve whose
+ * V-cycles.  Automatic: 2 for plain refinement, 1 for flexible CG and weighted solves and for
+ * operators that are nearly singular in FP32 (median row sum / diagonal of level 0 below 2e-6,
+ * e.g. the geodesic distance system 1e-6 M + S).  A solve whose
  * residual grows continues with 1.  No counterpart in the reference, whose solve loop is VCycle
  * repeated (TextureFiltering.cpp:562-579). */
 int gdtp_mixed_cycles(gdtp_system* s, int cycles);

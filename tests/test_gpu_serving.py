@@ -75,6 +75,39 @@ def test_mixed_requests_match_alone_and_direct():
         server.close()
 
 
+def test_lic_requests_beside_filtering():
+    """LIC requests (the reference's schedule of V-cycle rounds) served while filtering and geodesic
+    requests run on the same mesh: each result equals the same request run directly."""
+    mesh = g.TexturedMesh.torus(64, 32, 4, 2, 256, 256, 1.0, 0.35, 0.0, 5)
+    v, t, _ = mesh.arrays()
+    vf = g.surface_gradient_field(v, t, np.sin(3 * v[:, 0]) + np.cos(2 * v[:, 1]) + v[:, 2])
+    lic = g.LineIntegralConvolution(mesh, 256, 256, 4, vf, max_cols=4)
+    H = g.InitializeHierarchy(mesh, 256, 256, 4)
+    server = SolverServer(tol=1e-8, max_iters=120)
+    try:
+        server.add_mesh("torus", H, max_cols=4, geodesics=True, lic=lic)
+        N = H.level_info(0)["texels"]
+        rng = np.random.default_rng(3)
+        noises = [torch.from_numpy(rng.random((1 + i % 2, N))).cuda() for i in range(4)]
+        hs, geo = [], []
+        for i, nz in enumerate(noises):
+            hs.append(server.submit_lic("torus", nz))
+            hs.append(server.submit_filter("torus", torch.from_numpy(rng.random((3, N))).cuda(), 1e-3))
+            geo.append(server.submit_geodesic("torus", [int(rng.integers(N))]))
+        got = [h.wait(120) for h in hs[0::2]]
+        for h in hs[1::2] + geo:
+            h.wait(120)
+        for nz, y in zip(noises, got):
+            _, want = lic.convolve(nz, passes=4)
+            assert (y - want).abs().max().item() <= 1e-12 * want.abs().max().item()
+        # geodesic results against the same request served alone
+        for r in geo:
+            alone = server.submit_geodesic("torus", r.payload).wait(120)
+            assert (alone - r.result).abs().max().item() <= 1e-5 * alone.abs().max().item()
+    finally:
+        server.close()
+
+
 def test_admission_by_memory_budget():
     server = SolverServer(hbm_budget=1 << 20)  # 1 MiB: nothing fits
     try:

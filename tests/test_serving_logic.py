@@ -22,6 +22,10 @@ def _geo(n):
     return Request("m", "geodesic", list(range(n)), 1e-3)
 
 
+def _lic(cols):
+    return Request("m", "lic", np.zeros((cols, 4)), 0.0)
+
+
 def test_bucket_is_nearest_in_log_scale():
     m = _Mesh()
     assert m.bucket(3e-4) == 1e-4 and m.bucket(3.3e-4) == 1e-3 and m.bucket(0.5) == 1e-2 and m.bucket(1e-6) == 1e-4
@@ -55,3 +59,11 @@ def test_every_request_is_eventually_batched():
         assert b and sum(r.columns for r in b) <= m.max_cols
         seen += b
     assert len(seen) == 60 and len(set(map(id, seen))) == 60
+
+
+def test_lic_requests_batch_with_their_own_kind():
+    m = _Mesh()
+    p = [_lic(2), _filter(1, 1e-3), _lic(3), _geo(1), _lic(2)]
+    batch, rest = SolverServer._form_batch(m, p)
+    assert batch == [p[0], p[2]] and rest == [p[1], p[3], p[4]]  # 2 + 3 columns; the third LIC (2) no longer fits in 6
+    assert [r.columns for r in batch] == [2, 3]
