@@ -366,6 +366,25 @@ def lic_bench(steps, warmup, columns=16, rounds=4, seed=9):
                 latency_ms=dict(p50=float(np.percentile(lat, 50)) * 1e3, p99=float(np.percentile(lat, 99)) * 1e3))
 
 
+def assembly_bench():
+    """Mesh admission: hierarchy + finest-level assembly of the benchmark mesh (noisy icosphere,
+    327,680 triangles, 4096^2 atlas, 8 levels), interior cells integrated on the host (OpenMP over
+    charts) or on the GPU; best of two runs each."""
+    import gradientdomaintextureprocessing_amd as g
+    mesh = g.TexturedMesh.icosphere(7, 4096, 4096, 0.05, 1)
+    times = {}
+    for assembly in ("host", "device", "host", "device"):
+        t0 = time.perf_counter()
+        H = g.InitializeHierarchy(mesh, 4096, 4096, 8, assembly=assembly)
+        el = time.perf_counter() - t0
+        times[assembly] = min(times.get(assembly, 1e9), el)
+        log(f"[bench] hierarchy + assembly ({assembly}): {el:.2f}s")
+        del H
+    return dict(metric="mesh admission: hierarchy + assembly seconds (4096² icosphere)", host_s=times["host"],
+                device_s=times["device"], speedup=times["host"] / times["device"], cores=os.cpu_count(),
+                config=dict(workload="icosphere 327,680 triangles, 4096^2 atlas, 8 levels"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,8 +396,9 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
-    ap.add_argument("--workload", default=None, choices=["c3", "c4"],
-                    help="other workloads: c3 batched geodesics, c4 line integral convolution (one JSON line each)")
+    ap.add_argument("--workload", default=None, choices=["c3", "c4", "assembly"],
+                    help="other workloads: c3 batched geodesics, c4 line integral convolution, assembly = mesh admission "
+                         "time (one JSON line each)")
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
@@ -394,6 +414,9 @@ def main():
         return
     if args.serving:
         serving_bench(args.qps, args.requests, args.tol)
+        return
+    if args.workload == "assembly":
+        print(json.dumps(assembly_bench()), flush=True)
         return
     if args.workload:
         steps = min(args.steps, 10)

@@ -46,6 +46,7 @@ SIGNATURES = {
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_mixed_configure": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int]),
     "gdtp_mixed_cycles": (c_int, [c_void_p, c_int]),
+    "gdtp_hierarchy_create_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, P(c_void_p)]),
     "gdtp_modulation_create": (c_int, [c_void_p, c_void_p, c_int, P(c_void_p)]),
     "gdtp_modulated_stiffness": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "gdtp_modulation_info": (c_int, [c_void_p, c_void_p]),

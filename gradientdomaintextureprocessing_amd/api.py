@@ -177,18 +177,26 @@ class StiffnessModulation:
             self._h = None
 
 
-def InitializeHierarchy(mesh, width, height, levels=4, metric=METRIC_EMBEDDING, triangle_metric=None):
+def InitializeHierarchy(mesh, width, height, levels=4, metric=METRIC_EMBEDDING, triangle_metric=None, assembly="host"):
     """InitializeHierarchy + InitializeMetric + InitializeMassAndStiffness
     (HierarchyConstruction.inl:799-849, Metric.inl:448-466, MassAndStiffness.h:490-549).
     triangle_metric ([T, 2, 2], the surface metric in each triangle's edge basis) replaces the
-    embedding / uniform metric (Metric.inl:468-486)."""
+    embedding / uniform metric (Metric.inl:468-486).  assembly="device" integrates the interior
+    cells on the GPU (boundary cells and seams on the host)."""
+    if assembly not in ("host", "device"):
+        raise ValueError("assembly is 'host' or 'device'")
     h = ctypes.c_void_p()
-    if triangle_metric is None:
-        _lib.check(_lib.load().gdtp_hierarchy_create(mesh._h, width, height, levels, metric, ctypes.byref(h)))
-    else:
+    m = None
+    if triangle_metric is not None:
         g = np.asarray(triangle_metric, dtype=np.float64)
         m = np.ascontiguousarray(np.stack([g[:, 0, 0], 0.5 * (g[:, 0, 1] + g[:, 1, 0]), g[:, 1, 1]], axis=1))
-        _lib.check(_lib.load().gdtp_hierarchy_create_with_metric(mesh._h, width, height, levels, m.ctypes.data, ctypes.byref(h)))
+    lib = _lib.load()
+    if assembly == "device":
+        _lib.check(lib.gdtp_hierarchy_create_ex(mesh._h, width, height, levels, metric, None if m is None else m.ctypes.data, 1, ctypes.byref(h)))
+    elif m is None:
+        _lib.check(lib.gdtp_hierarchy_create(mesh._h, width, height, levels, metric, ctypes.byref(h)))
+    else:
+        _lib.check(lib.gdtp_hierarchy_create_with_metric(mesh._h, width, height, levels, m.ctypes.data, ctypes.byref(h)))
     return HierarchicalSystem(h, levels, width, height)
 
 

@@ -16,7 +16,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["gdtp_mesh.cpp", "gdtp_hierarchy.cpp", "gdtp_assembly.cpp", "gdtp_boundary.cpp", "gdtp_vectorfield.cpp", "gdtp_ldlt.cpp",
                 "gdtp_galerkin.cpp"]
-HIP_SOURCES = ["gdtp_device.hip", "gdtp_mixed.hip", "gdtp_capi.cpp"]
+HIP_SOURCES = ["gdtp_device.hip", "gdtp_mixed.hip", "gdtp_assembly_device.hip", "gdtp_capi.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-fopenmp",

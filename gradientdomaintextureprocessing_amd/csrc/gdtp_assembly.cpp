@@ -147,11 +147,32 @@ struct CellRows {
 }  // namespace
 
 int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric, FineOperator& op,
-                         std::string& err, const double* signal, int K, CellStiffnessRhs* cellRhs) {
+                         std::string& err, const double* signal, int K, CellStiffnessRhs* cellRhs, bool deviceInterior) {
   const Level& L0 = h.levels[0];
   const int nCharts = (int)L0.charts.size();
   const long N = L0.numTexels;
   const bool cells = cellRhs && signal && K > 0;
+  if (cells && deviceInterior) {
+    err = "the cell-based stiffness is assembled on the host";
+    return 0;
+  }
+  // device path: interior cells (and with them every deep row) on the GPU; the host keeps the
+  // boundary cells and needs the matrices of the interior cells that touch a boundary texel
+  std::vector<int> chartCellBase(nCharts + 1, 0), wantedPtr(nCharts + 1, 0), wanted;
+  std::vector<double> wantedM, wantedS;
+  if (deviceInterior) {
+    for (int c = 0; c < nCharts; c++) {
+      const GridChart& g = L0.charts[c];
+      chartCellBase[c + 1] = chartCellBase[c] + g.numInteriorCells;
+      for (int ic = 0; ic < g.numInteriorCells; ic++) {
+        bool touches = false;
+        for (int k = 0; k < 4; k++) touches = touches || L0.boundaryAndDeepIndex[g.interiorCellGlobalCorners[ic][k]] > 0;
+        if (touches) wanted.push_back(chartCellBase[c] + ic);
+      }
+      wantedPtr[c + 1] = (int)wanted.size();
+    }
+    AssembleInteriorCellsOnDevice(h, metric, wanted, op.deepMass, op.deepStiffness, wantedM, wantedS);
+  }
   std::vector<CellRows> crows(cells ? nCharts : 0);
   if (cells) {
     cellRhs->K = K, cellRhs->numCells = L0.numCells;
@@ -165,8 +186,10 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
       out[kk] = p;
     }
   };
-  op.deepMass.assign(10 * (size_t)L0.numDeep, 0.0);
-  op.deepStiffness.assign(10 * (size_t)L0.numDeep, 0.0);
+  if (!deviceInterior) {
+    op.deepMass.assign(10 * (size_t)L0.numDeep, 0.0);
+    op.deepStiffness.assign(10 * (size_t)L0.numDeep, 0.0);
+  }
   std::vector<std::vector<Triplet>> bbM(nCharts), bbS(nCharts), bdM(nCharts), bdS(nCharts);
   std::vector<std::string> errs(nCharts);
   SeamContext seams;
@@ -177,7 +200,8 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     const GridChart& g = L0.charts[c];
     const AtlasChart& ac = h.atlas.charts[c];
     const int nTri = (int)ac.triangles.size() / 3;
-    std::vector<std::array<double, 16>> cellM(g.numInteriorCells, std::array<double, 16>{}), cellS(g.numInteriorCells, std::array<double, 16>{});
+    const int nHostCells = deviceInterior ? 0 : g.numInteriorCells;  // interior cells integrated here
+    std::vector<std::array<double, 16>> cellM(nHostCells, std::array<double, 16>{}), cellS(nHostCells, std::array<double, 16>{});
     std::vector<std::array<double, 16>> bcellM(g.numBoundaryCells, std::array<double, 16>{}), bcellS(g.numBoundaryCells, std::array<double, 16>{});
     std::vector<V2> poly;
     for (int t = 0; t < nTri && errs[c].empty(); t++) {
@@ -196,6 +220,7 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
           size_t q = (size_t)j * (g.width - 1) + i;
           int li = g.localInteriorCell[q], lb = g.localBoundaryCell[q];
           if (li < 0 && lb < 0) continue;
+          if (li >= 0 && deviceInterior) continue;
           poly.assign(p, p + 3);
           if (!ClipToCell(poly, i, j, g.cellW, g.cellH)) continue;
           if (lb >= 0) {  // boundary cell: pieces touching a seam carry the averaged seam values
@@ -225,8 +250,28 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
     }
     if (!errs[c].empty()) continue;
     // interior cells: deep rows and the boundary rows they touch (MassAndStiffness.h:277-353)
+    auto boundaryRow = [&](int b, const std::array<int, 4>& gl, int k, const double* CM, const double* CS) {
+      for (int l = 0; l < 4; l++) {
+        int nbd = L0.boundaryAndDeepIndex[gl[l]];
+        if (nbd < 0) {
+          bdM[c].push_back({b, gl[l], CM[4 * k + l]});
+          bdS[c].push_back({b, gl[l], CS[4 * k + l]});
+        } else {
+          bbM[c].push_back({b, nbd - 1, CM[4 * k + l]});
+          bbS[c].push_back({b, nbd - 1, CS[4 * k + l]});
+        }
+      }
+    };
+    if (deviceInterior)  // deep rows came from the device; boundary rows of the cells that touch them
+      for (int w = wantedPtr[c]; w < wantedPtr[c + 1]; w++) {
+        const auto& gl = g.interiorCellGlobalCorners[wanted[w] - chartCellBase[c]];
+        for (int k = 0; k < 4; k++) {
+          const int bd = L0.boundaryAndDeepIndex[gl[k]];
+          if (bd > 0) boundaryRow(bd - 1, gl, k, &wantedM[16 * (size_t)w], &wantedS[16 * (size_t)w]);
+        }
+      }
     std::vector<double> prod(std::max(K, 1));
-    for (int ic = 0; ic < g.numInteriorCells; ic++) {
+    for (int ic = 0; ic < nHostCells; ic++) {
       const auto& gl = g.interiorCellGlobalCorners[ic];
       const int cell = g.cellOffset + g.interiorCellToLocal[ic];
       for (int k = 0; k < 4; k++) {
@@ -249,17 +294,7 @@ int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std
             op.deepStiffness[10 * (size_t)d + slot] += cellS[ic][4 * k + l];
           }
         } else {
-          int b = bd - 1;
-          for (int l = 0; l < 4; l++) {
-            int nbd = L0.boundaryAndDeepIndex[gl[l]];
-            if (nbd < 0) {
-              bdM[c].push_back({b, gl[l], cellM[ic][4 * k + l]});
-              bdS[c].push_back({b, gl[l], cellS[ic][4 * k + l]});
-            } else {
-              bbM[c].push_back({b, nbd - 1, cellM[ic][4 * k + l]});
-              bbS[c].push_back({b, nbd - 1, cellS[ic][4 * k + l]});
-            }
-          }
+          boundaryRow(bd - 1, gl, k, cellM[ic].data(), cellS[ic].data());
         }
       }
     }

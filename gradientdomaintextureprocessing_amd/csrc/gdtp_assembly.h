@@ -48,6 +48,13 @@ struct CellStiffnessRhs {
 // in global texel order).
 int AssembleFineOperator(const TexturedMesh& mesh, const Hierarchy& h, const std::vector<std::vector<M2>>& metric,
                          FineOperator& op, std::string& err, const double* signal = nullptr, int K = 0,
-                         CellStiffnessRhs* cellRhs = nullptr);
+                         CellStiffnessRhs* cellRhs = nullptr, bool deviceInterior = false);
+
+// Interior cells on the GPU (gdtp_assembly_device.hip): the deep stencils (10 per deep texel, as
+// above) and the 4x4 mass / stiffness matrices of the interior cells listed in `wanted` (global
+// interior-cell indices: charts in order, each chart's interior cells in raster order), 16 per cell.
+void AssembleInteriorCellsOnDevice(const Hierarchy& h, const std::vector<std::vector<M2>>& metric, const std::vector<int>& wanted,
+                                   std::vector<double>& deepMass, std::vector<double>& deepStiffness, std::vector<double>& wantedM,
+                                   std::vector<double>& wantedS);
 
 }  // namespace gdtp

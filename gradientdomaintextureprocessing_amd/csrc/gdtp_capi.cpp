@@ -160,29 +160,19 @@ int gdtp_mesh_read(const gdtp_mesh* m, double* V, int* T, double* uv) {
 void gdtp_mesh_destroy(gdtp_mesh* m) { delete m; }
 
 // ---------------------------------------------------------------- hierarchy
-int gdtp_hierarchy_create(const gdtp_mesh* mesh, int W, int H, int levels, int metricMode, gdtp_hierarchy** out) {
+// Shared by the hierarchy constructors: metric from the embedding / UV layout, or the given
+// per-triangle surface metric; interior cells assembled on the GPU with `device`.
+static int CreateHierarchy(const gdtp_mesh* mesh, int W, int H, int levels, int metricMode, const double* triangleMetric, bool device,
+                           gdtp_hierarchy** out) {
   if (!mesh || !out || W < 8 || H < 8 || levels < 1 || levels > 16) return Fail(GDTP_ERROR_INVALID, "invalid hierarchy arguments");
-  GDTP_TRY({
-    auto* h = new gdtp_hierarchy();
-    h->mesh = mesh->m;
-    std::string err;
-    BlockInfo bi;
-    if (!BuildHierarchy(h->mesh, W, H, levels, bi, h->h, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
-    ComputeParameterMetric(h->mesh, h->h.atlas, metricMode == GDTP_METRIC_UNIFORM ? METRIC_UNIFORM : METRIC_EMBEDDING, h->metric);
-    if (!AssembleFineOperator(h->mesh, h->h, h->metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
-    *out = h;
-    return GDTP_OK;
-  })
-}
-
-int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int W, int H, int levels, const double* triangleMetric, gdtp_hierarchy** out) {
-  if (!mesh || !out || !triangleMetric || W < 8 || H < 8 || levels < 1 || levels > 16)
-    return Fail(GDTP_ERROR_INVALID, "invalid hierarchy arguments");
-  const int nT = mesh->m.numTriangles();
-  std::vector<M2> g(nT);
-  for (int t = 0; t < nT; t++) {
-    g[t](0, 0) = triangleMetric[3 * t], g[t](0, 1) = g[t](1, 0) = triangleMetric[3 * t + 1], g[t](1, 1) = triangleMetric[3 * t + 2];
-    if (!(g[t](0, 0) > 0) || !(g[t].det() > 0)) return Fail(GDTP_ERROR_INVALID, "triangle metric is not positive definite");
+  std::vector<M2> g;
+  if (triangleMetric) {
+    const int nT = mesh->m.numTriangles();
+    g.resize(nT);
+    for (int t = 0; t < nT; t++) {
+      g[t](0, 0) = triangleMetric[3 * t], g[t](0, 1) = g[t](1, 0) = triangleMetric[3 * t + 1], g[t](1, 1) = triangleMetric[3 * t + 2];
+      if (!(g[t](0, 0) > 0) || !(g[t].det() > 0)) return Fail(GDTP_ERROR_INVALID, "triangle metric is not positive definite");
+    }
   }
   GDTP_TRY({
     auto* h = new gdtp_hierarchy();
@@ -190,11 +180,30 @@ int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int W, int H, int l
     std::string err;
     BlockInfo bi;
     if (!BuildHierarchy(h->mesh, W, H, levels, bi, h->h, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
-    ComputeParameterMetricFromSurface(h->mesh, h->h.atlas, g, h->metric);
-    if (!AssembleFineOperator(h->mesh, h->h, h->metric, h->op, err)) { delete h; return Fail(GDTP_ERROR_GEOMETRY, err); }
+    if (triangleMetric) ComputeParameterMetricFromSurface(h->mesh, h->h.atlas, g, h->metric);
+    else ComputeParameterMetric(h->mesh, h->h.atlas, metricMode == GDTP_METRIC_UNIFORM ? METRIC_UNIFORM : METRIC_EMBEDDING, h->metric);
+    if (!AssembleFineOperator(h->mesh, h->h, h->metric, h->op, err, nullptr, 0, nullptr, device)) {
+      delete h;
+      return Fail(GDTP_ERROR_GEOMETRY, err);
+    }
     *out = h;
     return GDTP_OK;
   })
+}
+
+int gdtp_hierarchy_create(const gdtp_mesh* mesh, int W, int H, int levels, int metricMode, gdtp_hierarchy** out) {
+  return CreateHierarchy(mesh, W, H, levels, metricMode, nullptr, false, out);
+}
+
+int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int W, int H, int levels, const double* triangleMetric, gdtp_hierarchy** out) {
+  if (!triangleMetric) return Fail(GDTP_ERROR_INVALID, "null triangle metric");
+  return CreateHierarchy(mesh, W, H, levels, GDTP_METRIC_EMBEDDING, triangleMetric, false, out);
+}
+
+int gdtp_hierarchy_create_ex(const gdtp_mesh* mesh, int W, int H, int levels, int metricMode, const double* triangleMetric, int flags,
+                             gdtp_hierarchy** out) {
+  if (flags & ~GDTP_ASSEMBLY_DEVICE) return Fail(GDTP_ERROR_INVALID, "unknown hierarchy flags");
+  return CreateHierarchy(mesh, W, H, levels, metricMode, triangleMetric, (flags & GDTP_ASSEMBLY_DEVICE) != 0, out);
 }
 
 int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info) {

@@ -30,6 +30,7 @@ enum {
 enum { GDTP_METRIC_EMBEDDING = 0, GDTP_METRIC_UNIFORM = 1 };
 enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS = 2 };
 enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
+enum { GDTP_ASSEMBLY_DEVICE = 1 };
 
 typedef struct gdtp_mesh gdtp_mesh;
 typedef struct gdtp_hierarchy gdtp_hierarchy;
@@ -65,6 +66,13 @@ int gdtp_hierarchy_create(const gdtp_mesh* mesh, int width, int height, int leve
  * (Metric.inl:12-67, 415-486) for line integral convolution. */
 int gdtp_hierarchy_create_with_metric(const gdtp_mesh* mesh, int width, int height, int levels, const double* triangle_metric,
                                       gdtp_hierarchy** out);
+/* Either of the above (triangle_metric NULL: metric_mode) with flags: GDTP_ASSEMBLY_DEVICE assembles
+ * the interior cells (every deep row and the interior-cell parts of the boundary rows) on the GPU —
+ * the same clipping, quadrature and per-cell triangle order as the host, so the operator agrees
+ * with the host assembly to rounding; boundary cells and seams stay on the host.  Cuts mesh
+ * admission time.  The reference assembles on the CPU (MassAndStiffness.h:5-549). */
+int gdtp_hierarchy_create_ex(const gdtp_mesh* mesh, int width, int height, int levels, int metric_mode, const double* triangle_metric,
+                             int flags, gdtp_hierarchy** out);
 /* info[0..7] = texels, deep, boundary, charts, raster lines, band tasks, blocks, segments */
 int gdtp_hierarchy_level_info(const gdtp_hierarchy* h, int level, int* info);
 /* Named integer / double arrays of the hierarchy (see gdtp_capi.cpp for the list); call with
