@@ -301,8 +301,6 @@ def geodesics_bench(steps, warmup, tol, sources=64, seed=7):
     H = g.InitializeHierarchy(mesh, 2048, 2048, 7)
     N = H.level_info(0)["texels"]
     geo = g.Geodesics(H, heat_time=1e-3, max_sources=sources)
-    for s in (geo.heat, geo.distance):
-        s.configure_mixed()
     log(f"[bench] geodesics setup {time.time() - t0:.1f}s, {N} texels")
     src = [int(v) for v in np.random.default_rng(seed).choice(N, sources, replace=False)]
     lat = []

@@ -63,6 +63,7 @@ struct MixedSystem {
   // mode instead of reducing it.
   double rowSumRatio = 1.0;
   double *r64 = nullptr, *p64 = nullptr, *q64 = nullptr, *z64 = nullptr;  // [kcap][N0] each
+  int* deepLine = nullptr;         // [D0]: raster line of each level-0 deep texel (FP64 outer kernels)
   double* dots = nullptr;          // [3][kcap][1024]: per-block partial sums of (z,q), (r,p), (p,q)
   double *alpha = nullptr, *beta = nullptr, *pqPrev = nullptr;  // [kcap]
   int* first = nullptr;            // first step of a solve (beta = 0)

@@ -412,20 +412,31 @@ __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
 
 // deep rows: r = b - A x with raw FP64 stencils along raster lines, up to four columns per pass
 // over the stencil; r -> FP32 dense rhs of the correction, e -> 0, |r|^2 -> norms[ch][slot]
-__global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __restrict__ lines, const double* __restrict__ coef, int D,
-                                                       const double* __restrict__ x, const double* __restrict__ b, int N,
-                                                       const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K) {
-  const int l = blockIdx.x;
-  const int* L = lines + 5 * l;
-  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+// One thread per deep texel d; its raster line gives the global indices of the rows above and
+// below (line = deepLine[d]: start, end, prev, next, first deep index).
+__device__ __forceinline__ void DeepRow(const int* __restrict__ deepLine, const int* __restrict__ lines, int d, int& cur, int& prev,
+                                        int& next) {
+  const int* L = lines + 5 * deepLine[d];
+  const int i = d - L[4];
+  cur = L[0] + i, prev = L[2] + i, next = L[3] + i;
+}
+
+__global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                       const double* __restrict__ coef, const double* __restrict__ x,
+                                                       const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                       float* e0, long Nd, double* norms, int K) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = d < D;
+  int cur = 0, prev = 0, next = 0;
+  if (on) DeepRow(deepLine, lines, d, cur, prev, next);
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
     double acc[4] = {0, 0, 0, 0};
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    if (on) {
       double cf[9];
 #pragma unroll
-      for (int k = 0; k < 9; k++) cf[k] = coef[(long)k * D + d0 + i];
-      const long di = g2d[start + i];
+      for (int k = 0; k < 9; k++) cf[k] = coef[(long)k * D + d];
+      const long di = g2d[cur];
 #pragma unroll
       for (int cc = 0; cc < 4; cc++) {
         if (cc >= kc) break;
@@ -434,16 +445,16 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-          const int base = k < 3 ? prev : (k < 6 ? start : next);
-          s += cf[k] * X[base + i + k % 3 - 1];
+          const int base = k < 3 ? prev : (k < 6 ? cur : next);
+          s += cf[k] * X[base + k % 3 - 1];
         }
-        const double r = b[(long)ch * N + start + i] - s;
+        const double r = b[(long)ch * N + cur] - s;
         rhs0[(long)ch * Nd + di] = (float)r;
         e0[(long)ch * Nd + di] = 0.0f;
         acc[cc] += r * r;
       }
     }
-    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);
+    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
   }
 }
 
@@ -451,22 +462,23 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int nLines, const int* __
 // The same residual for per-column operators A_c = w[c][0] M + w[c][1] S: both level-0 stencils are
 // read once per texel for up to four columns (parametric solves: one refinement loop serves
 // columns with different weights, its FP32 V-cycle built for one nearby pair of weights).
-__global__ __launch_bounds__(256) void K64ResidualDeepW(int nLines, const int* __restrict__ lines, const double* __restrict__ coefM,
-                                                        const double* __restrict__ coefS, int D, const double* __restrict__ x,
-                                                        const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                        float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
-                                                        double* r64) {
-  const int l = blockIdx.x;
-  const int* L = lines + 5 * l;
-  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+__global__ __launch_bounds__(256) void K64ResidualDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                        const double* __restrict__ coefM, const double* __restrict__ coefS,
+                                                        const double* __restrict__ x, const double* __restrict__ b, int N,
+                                                        const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
+                                                        const double* __restrict__ w, double* r64) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = d < D;
+  int cur = 0, prev = 0, next = 0;
+  if (on) DeepRow(deepLine, lines, d, cur, prev, next);
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
     double acc[4] = {0, 0, 0, 0};
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    if (on) {
       double cm[9], cs[9];
 #pragma unroll
-      for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d0 + i], cs[k] = coefS[(long)k * D + d0 + i];
-      const long di = g2d[start + i];
+      for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d], cs[k] = coefS[(long)k * D + d];
+      const long di = g2d[cur];
 #pragma unroll
       for (int cc = 0; cc < 4; cc++) {
         if (cc >= kc) break;
@@ -475,18 +487,18 @@ __global__ __launch_bounds__(256) void K64ResidualDeepW(int nLines, const int* _
         double sm = 0.0, ss = 0.0;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-          const int base = k < 3 ? prev : (k < 6 ? start : next);
-          const double xv = X[base + i + k % 3 - 1];
+          const int base = k < 3 ? prev : (k < 6 ? cur : next);
+          const double xv = X[base + k % 3 - 1];
           sm += cm[k] * xv, ss += cs[k] * xv;
         }
-        const double r = b[(long)ch * N + start + i] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
-        if (r64) r64[(long)ch * N + start + i] = r;
+        const double r = b[(long)ch * N + cur] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
+        if (r64) r64[(long)ch * N + cur] = r;
         rhs0[(long)ch * Nd + di] = (float)r;
         e0[(long)ch * Nd + di] = 0.0f;
         acc[cc] += r * r;
       }
     }
-    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);
+    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
   }
 }
 
@@ -778,6 +790,16 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     }
     d.charts = Up(ms, d.hcharts);
     d.g2d = Up(ms, g2d);
+    if (l == 0) {  // raster line of every deep texel, for the FP64 outer kernels
+      std::vector<int> dl(H.numDeep, -1);
+      for (size_t r = 0; r < H.rasterLines.size(); r++) {
+        const Line5& x = H.rasterLines[r];
+        for (int i = 0; i <= x.end - x.start; i++) dl[x.coeff + i] = (int)r;
+      }
+      for (int v : dl)
+        if (v < 0) throw std::runtime_error("deep texel outside every raster line");
+      ms.deepLine = Up(ms, dl);
+    }
     d.type = Up(ms, type);
     // tiles of the sub-grid that contain at least one active texel (any plane)
     std::vector<int> tiles;
@@ -990,26 +1012,26 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
 
 // ------------------------------------------------------------------ flexible conjugate gradients
 // q = A p for per-column operators A_c = w[c][0] M + w[c][1] S (level 0; deep rows by raster line)
-__global__ __launch_bounds__(256) void K64ApplyDeepW(int nLines, const int* __restrict__ lines, const double* __restrict__ coefM,
-                                                     const double* __restrict__ coefS, int D, const double* __restrict__ p, int N, double* q,
-                                                     int K, const double* __restrict__ w) {
-  const int* L = lines + 5 * blockIdx.x;
-  const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
-  for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    double cm[9], cs[9];
+__global__ __launch_bounds__(256) void K64ApplyDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                     const double* __restrict__ coefM, const double* __restrict__ coefS,
+                                                     const double* __restrict__ p, int N, double* q, int K, const double* __restrict__ w) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D) return;
+  int cur, prev, next;
+  DeepRow(deepLine, lines, d, cur, prev, next);
+  double cm[9], cs[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d0 + i], cs[k] = coefS[(long)k * D + d0 + i];
-    for (int ch = 0; ch < K; ch++) {
-      const double* P = p + (long)ch * N;
-      double sm = 0.0, ss = 0.0;
+  for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d], cs[k] = coefS[(long)k * D + d];
+  for (int ch = 0; ch < K; ch++) {
+    const double* P = p + (long)ch * N;
+    double sm = 0.0, ss = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const int base = k < 3 ? prev : (k < 6 ? start : next);
-        const double v = P[base + i + k % 3 - 1];
-        sm += cm[k] * v, ss += cs[k] * v;
-      }
-      q[(long)ch * N + start + i] = w[2 * ch] * sm + w[2 * ch + 1] * ss;
+    for (int k = 0; k < 9; k++) {
+      const int base = k < 3 ? prev : (k < 6 ? cur : next);
+      const double v = P[base + k % 3 - 1];
+      sm += cm[k] * v, ss += cs[k] * v;
     }
+    q[(long)ch * N + cur] = w[2 * ch] * sm + w[2 * ch + 1] * ss;
   }
 }
 
@@ -1124,7 +1146,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   if (weighted) {
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
     if (L0.nRaster > 0)
-      K64ResidualDeepW<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, M.coefRaw, S.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
+      K64ResidualDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
                                                   ms.norms, K, ms.weights, r64);
     if (L0.B > 0)
       K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(
@@ -1133,7 +1155,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
     return;
   }
   if (L0.nRaster > 0)
-    K64ResidualDeep<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, L0.coefRaw, L0.D, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+    K64ResidualDeep<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, L0.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
   if (L0.B > 0)
     K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
                                                                           L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
@@ -1156,7 +1178,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   KFcgDirection<<<NBlk(nk, 256), 256, 0, s>>>(nk, N, ms.z64, ms.beta, ms.p64);
   const DevLevel &M = ds.mass0, &S = ds.stiff0;
   if (L0.nRaster > 0)
-    K64ApplyDeepW<<<L0.nRaster, 256, 0, s>>>(L0.nRaster, L0.raster, M.coefRaw, S.coefRaw, L0.D, ms.p64, N, ms.q64, K, ms.weights);
+    K64ApplyDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, ms.p64, N, ms.q64, K, ms.weights);
   if (L0.B > 0)
     K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
                                                                        M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,

@@ -122,8 +122,8 @@ class SolverServer:
             small = hierarchy.level_info(0)["texels"] <= self.short_texels
             kw["buckets"] = tuple(np.logspace(-4, -2, 5 if small else 3))
         m = ResidentMesh(name, hierarchy, **kw)
-        for s in m.all_systems():
-            s.configure_mixed()  # builds the mixed solver now: first requests should not pay for it
+        for s in m.all_systems():  # builds the mixed solver now: first requests should not pay for it
+            s.configure_mixed(krylov=s is m.heat or s is m.distance)
         mem = m.memory()
         if self.hbm_budget is not None and self.used + mem > self.hbm_budget:
             del m
