@@ -129,7 +129,7 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
 // once rather than once per plane.
 __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                         long Nd, const float* __restrict__ st, const float* __restrict__ e,
-                                                        const float* __restrict__ rhs, float* res, int K) {
+                                                        const float* __restrict__ rhs, float* res, int K, int zeroQ) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
@@ -140,6 +140,10 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
     const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
     const float a4 = st[9 * Nd + idx];
     if (a4 == 0.0f) continue;
+    if (q == zeroQ) {  // relaxed last: its deep rows hold exactly, so the residual is zero
+      for (int ch = 0; ch < K; ch++) res[(long)ch * Nd + idx] = 0.0f;
+      continue;
+    }
     float cf[9];
     int nb[9];
 #pragma unroll
@@ -978,7 +982,8 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     Smooth(ms, l, K, false, s);  // e of this level was zeroed by whoever wrote its rhs
     if (d.nTiles > 0)
-      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K);
+      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K,
+                                                ms.nuPre > 0 ? 3 : -1);
     if (d.B > 0)
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
                                                                             0, d.Nd, K);
