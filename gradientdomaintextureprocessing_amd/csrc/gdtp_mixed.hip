@@ -174,10 +174,11 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
 // ------------------------------------------------------------------ boundary rows
 // Boundary rows are sparse (about 16 entries); a group of kRowLanes lanes works on one row so that
 // the row's entries are read coalesced, once for up to four columns, and summed by shuffles.
-static constexpr int kRowLanes = 16;
-template <class T>
+static constexpr int kRowLanes = 16;    // FP32 boundary rows (measured best of 8 / 16)
+static constexpr int kRowLanes64 = 8;   // FP64 level-0 boundary rows (M and S entries; 52 vs 73 us at 16)
+template <class T, int Lanes = kRowLanes>
 __device__ __forceinline__ T RowSum(T v) {
-  for (int o = kRowLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kRowLanes);
+  for (int o = Lanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, Lanes);
   return v;
 }
 template <class T>
@@ -517,14 +518,14 @@ __global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __
                                                             const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
                                                             float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
                                                             double* r64) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
     double am[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
       auto rows = [&](const int* rp, const int* cl, const double* vl, bool viaBoundary, double (&acc)[4]) {
-        for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes) {
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes64) {
           const double v = vl[k];
           const int g = viaBoundary ? bgi[cl[k]] : cl[k];
 #pragma unroll
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __
       rows(srpd, scold, svald, false, as);
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) am[q] = RowSum(am[q]), as[q] = RowSum(as[q]);
+    for (int q = 0; q < 4; q++) am[q] = RowSum<double, kRowLanes64>(am[q]), as[q] = RowSum<double, kRowLanes64>(as[q]);
     double sq = 0.0;
     if (active && lane < kc) {
       const int ch = c0 + lane, g = bgi[r];
@@ -558,21 +559,21 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
                                                            const double* __restrict__ valb, const double* __restrict__ x,
                                                            const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
                                                            float* e0, long Nd, double* norms, int K) {
-  // kRowLanes lanes per boundary row (no early exit: every thread takes part in the block sums)
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  // kRowLanes64 lanes per boundary row (no early exit: every thread takes part in the block sums)
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
-      for (int k = rpb[r] + lane; k < rpb[r + 1]; k += kRowLanes) {
+      for (int k = rpb[r] + lane; k < rpb[r + 1]; k += kRowLanes64) {
         const double v = valb[k];
         const int g = bgi[colb[k]];
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (q < kc) acc[q] += v * x[(long)(c0 + q) * N + g];
       }
-      for (int k = rpd[r] + lane; k < rpd[r + 1]; k += kRowLanes) {
+      for (int k = rpd[r] + lane; k < rpd[r + 1]; k += kRowLanes64) {
         const double v = vald[k];
         const int g = cold[k];
 #pragma unroll
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[q] = RowSum(acc[q]);
+    for (int q = 0; q < 4; q++) acc[q] = RowSum<double, kRowLanes64>(acc[q]);
     double sq = 0.0;
     if (active && lane < kc) {
       const int g = bgi[r];
@@ -1048,13 +1049,13 @@ __global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __res
                                                          const int* __restrict__ srpb, const int* __restrict__ scolb,
                                                          const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q,
                                                          int K, const double* __restrict__ w) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   if (r >= B) return;  // whole row groups leave together
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
     double am[4] = {0.0, 0.0, 0.0, 0.0}, as[4] = {0.0, 0.0, 0.0, 0.0};
     auto rows = [&](const int* rp, const int* cl, const double* vl, bool viaBoundary, double (&acc)[4]) {
-      for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes) {
+      for (int k = rp[r] + lane; k < rp[r + 1]; k += kRowLanes64) {
         const double v = vl[k];
         const int g = viaBoundary ? bgi[cl[k]] : cl[k];
 #pragma unroll
@@ -1067,7 +1068,7 @@ __global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __res
     rows(srpb, scolb, svalb, true, as);
     rows(srpd, scold, svald, false, as);
 #pragma unroll
-    for (int qq = 0; qq < 4; qq++) am[qq] = RowSum(am[qq]), as[qq] = RowSum(as[qq]);
+    for (int qq = 0; qq < 4; qq++) am[qq] = RowSum<double, kRowLanes64>(am[qq]), as[qq] = RowSum<double, kRowLanes64>(as[qq]);
     if (lane < kc) {
       const int ch = c0 + lane;
       q[(long)ch * N + bgi[r]] = w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane);
@@ -1154,7 +1155,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
       K64ResidualDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
                                                   ms.norms, K, ms.weights, r64);
     if (L0.B > 0)
-      K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(
+      K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(
           L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr,
           S.bb.col, S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64);
     return;
@@ -1162,7 +1163,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   if (L0.nRaster > 0)
     K64ResidualDeep<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, L0.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
   if (L0.B > 0)
-    K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
+    K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
                                                                           L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
 }
 
@@ -1185,7 +1186,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   if (L0.nRaster > 0)
     K64ApplyDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, ms.p64, N, ms.q64, K, ms.weights);
   if (L0.B > 0)
-    K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
+    K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
                                                                        M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
                                                                        S.bb.val, ms.p64, N, ms.q64, K, ms.weights);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.r64, ms.p64, rp);
