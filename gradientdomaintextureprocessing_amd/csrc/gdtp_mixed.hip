@@ -595,11 +595,14 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
   }
 }
 
+// grid-stride over a column (at most kNormBlocks blocks per column: few atomics per slot)
+static constexpr unsigned kNormBlocks = 1024;
 __global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out) {
   const int ch = blockIdx.y;
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double a = i < N ? v[(long)ch * N + i] : 0.0;
-  BlockAddSquares(a * a, out + (long)ch * kSlots);
+  const double* V = v + (long)ch * N;
+  double a = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) a += V[i] * V[i];
+  BlockAddSquares(a, out + (long)ch * kSlots);
 }
 
 __global__ void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
@@ -1227,7 +1230,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   const int N = ms.L[0].N;
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
-  K64SquaredNorm<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, b, bNorms);
+  K64SquaredNorm<<<dim3(std::min(NBlk(N, 256), kNormBlocks), K), 256, 0, s>>>(N, b, bNorms);
   if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
   OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr);
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
