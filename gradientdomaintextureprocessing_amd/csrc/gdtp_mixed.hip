@@ -297,18 +297,33 @@ __global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ 
   }
 }
 
-// Coarse boundary rhs: the boundary restriction matrix (fine dense columns).
-__global__ void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                     const float* __restrict__ vl, const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
-                                     int K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)B * K) return;
-  const int r = t % B, ch = t / B;
-  const float* R = fres + (long)ch * fNd;
-  float s = 0.0f;
-  for (int k = rp[r]; k < rp[r + 1]; k++) s += vl[k] * R[cl[k]];
-  crhs[(long)ch * cNd + bdIdx[r]] = s;
-  ce[(long)ch * cNd + bdIdx[r]] = 0.0f;
+// Coarse boundary rhs: the boundary restriction matrix (fine dense columns); kRowLanes lanes per
+// coarse row read its entries coalesced, once for up to four columns, summed by shuffles.
+__global__ __launch_bounds__(256) void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                            const int* __restrict__ cl, const float* __restrict__ vl,
+                                                            const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
+                                                            int K) {
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+  if (r >= B) return;  // whole row groups leave together
+  const int k0 = rp[r], k1 = rp[r + 1];
+  const long d = bdIdx[r];
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = k0 + lane; k < k1; k += kRowLanes) {
+      const float v = vl[k];
+      const long c = cl[k];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < kc) acc[q] += v * fres[(long)(c0 + q) * fNd + c];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = RowSum(acc[q]);
+    if (lane < kc) {
+      crhs[(long)(c0 + lane) * cNd + d] = Pick(acc, lane);
+      ce[(long)(c0 + lane) * cNd + d] = 0.0f;
+    }
+  }
 }
 
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
@@ -995,8 +1010,8 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
       KMixRestrictDeep<<<c.nTiles, 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
                                                          K);
     if (c.B > 0)
-      KMixRestrictBoundary<<<NBlk((long)c.B * K, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs, c.e, c.Nd,
-                                                                     K);
+      KMixRestrictBoundary<<<NBlk((long)c.B * kRowLanes, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs,
+                                                                            c.e, c.Nd, K);
   }
   {
     DevMixLevel& c = ms.L[nl - 1];
