@@ -23,6 +23,16 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _vec(t, n, K=None, name="vector"):
+    """Device pointer of a [K, n] float64 GPU tensor, checked: the C ABI trusts sizes it is given."""
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.dim() == 2 and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous 2-D float64 GPU tensor [columns, texels]")
+    if t.shape[1] != n or (K is not None and t.shape[0] != K):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}; expected [{K if K is not None else 'columns'}, {n}]")
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def _stream():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -158,7 +168,8 @@ class StiffnessModulation:
             raise ValueError(f"mask must be a float64 vector of {self.num_cells} cells")
         if out is None:
             out = torch.empty(self.columns, self.texels, dtype=torch.float64, device=mask.device)
-        _lib.check(_lib.load().gdtp_modulated_stiffness(self._h, _ptr(mask), _ptr(out), _stream()))
+        po = _vec(out, self.texels, self.columns, "out")
+        _lib.check(_lib.load().gdtp_modulated_stiffness(self._h, _ptr(mask), po, _stream()))
         return out
 
     def array(self, name, dtype=np.float64):
@@ -240,6 +251,13 @@ class MultigridSystem:
 
     def __init__(self, handle, hierarchy, max_cols):
         self._h, self.hierarchy, self.max_cols = handle, hierarchy, max_cols
+        info = [hierarchy.level_info(l) for l in range(hierarchy.levels)]
+        self._texels, self._boundary = [i["texels"] for i in info], [i["boundary"] for i in info]
+
+    def _level(self, level):
+        if not 0 <= level < len(self._texels):
+            raise ValueError(f"level {level} outside [0, {len(self._texels) - 1}]")
+        return self._texels[level]
 
     def update(self, stiffness_weight, screen_weight=1.0):
         _lib.check(_lib.load().gdtp_system_update(self._h, screen_weight, stiffness_weight))
@@ -255,53 +273,75 @@ class MultigridSystem:
         return out
 
     # -- device operations
+    def _pair(self, x, b, n, *more):
+        K = x.shape[0] if hasattr(x, "shape") else None
+        ptrs = [_vec(x, n, None, "x"), _vec(b, n, K, "b")] + [None if m is None else _vec(m, n, K, "output") for m in more]
+        return ptrs, K
+
     def vcycle(self, x, b):
-        _lib.check(_lib.load().gdtp_vcycle(self._h, _ptr(x), _ptr(b), x.shape[0], _stream()))
+        (px, pb), K = self._pair(x, b, self._texels[0])
+        _lib.check(_lib.load().gdtp_vcycle(self._h, px, pb, K, _stream()))
 
     def relax(self, level, x, b, residual=None):
-        _lib.check(_lib.load().gdtp_relax(self._h, level, _ptr(x), _ptr(b), _ptr(residual), x.shape[0], _stream()))
+        (px, pb, pr), K = self._pair(x, b, self._level(level), residual)
+        _lib.check(_lib.load().gdtp_relax(self._h, level, px, pb, pr, K, _stream()))
 
     def band_sweep(self, level, x, b, residual=None):
-        _lib.check(_lib.load().gdtp_band_sweep(self._h, level, _ptr(x), _ptr(b), _ptr(residual), x.shape[0], _stream()))
+        (px, pb, pr), K = self._pair(x, b, self._level(level), residual)
+        _lib.check(_lib.load().gdtp_band_sweep(self._h, level, px, pb, pr, K, _stream()))
 
     def restrict(self, level, fine, coarse):
-        _lib.check(_lib.load().gdtp_restrict(self._h, level, _ptr(fine), _ptr(coarse), fine.shape[0], _stream()))
+        n = self._level(level)
+        pf = _vec(fine, n, None, "fine")
+        pc = _vec(coarse, self._level(level + 1), fine.shape[0], "coarse")
+        _lib.check(_lib.load().gdtp_restrict(self._h, level, pf, pc, fine.shape[0], _stream()))
 
     def prolong_add(self, level, coarse, fine):
-        _lib.check(_lib.load().gdtp_prolong_add(self._h, level, _ptr(coarse), _ptr(fine), fine.shape[0], _stream()))
+        n = self._level(level)
+        pf = _vec(fine, n, None, "fine")
+        pc = _vec(coarse, self._level(level + 1), fine.shape[0], "coarse")
+        _lib.check(_lib.load().gdtp_prolong_add(self._h, level, pc, pf, fine.shape[0], _stream()))
 
     def boundary_solve(self, level, rhs, x):
-        _lib.check(_lib.load().gdtp_boundary_solve(self._h, level, _ptr(rhs), _ptr(x), rhs.shape[0], _stream()))
+        self._level(level)
+        (pr, px), K = self._pair(rhs, x, self._boundary[level])
+        _lib.check(_lib.load().gdtp_boundary_solve(self._h, level, pr, px, K, _stream()))
 
     def coarse_solve(self, rhs, x):
-        _lib.check(_lib.load().gdtp_coarse_solve(self._h, _ptr(rhs), _ptr(x), rhs.shape[0], _stream()))
+        (pr, px), K = self._pair(rhs, x, self._texels[-1])
+        _lib.check(_lib.load().gdtp_coarse_solve(self._h, pr, px, K, _stream()))
 
     def apply(self, which, x, y):
-        _lib.check(_lib.load().gdtp_apply(self._h, which, _ptr(x), _ptr(y), x.shape[0], _stream()))
+        (px, py), K = self._pair(x, y, self._texels[0])
+        _lib.check(_lib.load().gdtp_apply(self._h, which, px, py, K, _stream()))
 
     def residual(self, x, b, r):
-        _lib.check(_lib.load().gdtp_residual(self._h, _ptr(x), _ptr(b), _ptr(r), x.shape[0], _stream()))
+        (px, pb, pr), K = self._pair(x, b, self._texels[0], r)
+        _lib.check(_lib.load().gdtp_residual(self._h, px, pb, pr, K, _stream()))
 
     def geodesic_divergence(self, potential, rhs):
         """rhs = integral < -grad u / |grad u|_g , grad phi_t >_g per column of `potential`."""
-        _lib.check(_lib.load().gdtp_geodesic_divergence(self._h, _ptr(potential), _ptr(rhs), potential.shape[0], _stream()))
+        (pu, pr), K = self._pair(potential, rhs, self._texels[0])
+        _lib.check(_lib.load().gdtp_geodesic_divergence(self._h, pu, pr, K, _stream()))
 
     def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed"):
         """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res)."""
-        rel = np.zeros(x.shape[0])
+        (px, pb), K = self._pair(x, b, self._texels[0])
+        rel = np.zeros(K)
         it = ctypes.c_int(0)
         m = 1 if mode == "mixed" else 0
-        _lib.check(_lib.load().gdtp_solve(self._h, _ptr(x), _ptr(b), x.shape[0], tol, max_iters, m, rel.ctypes.data,
+        _lib.check(_lib.load().gdtp_solve(self._h, px, pb, K, tol, max_iters, m, rel.ctypes.data,
                                           ctypes.byref(it), _stream()))
         return it.value, rel
 
     def solve_weighted(self, x, b, weights, tol=1e-6, max_iters=60):
         """Column c solves (weights[c][0] M + weights[c][1] S) x = b (mixed precision; this system's
         V-cycle preconditions every column).  Returns (iterations, rel_res)."""
-        w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(x.shape[0], 2))
-        rel = np.zeros(x.shape[0])
+        (px, pb), K = self._pair(x, b, self._texels[0])
+        w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(K, 2))
+        rel = np.zeros(K)
         it = ctypes.c_int(0)
-        _lib.check(_lib.load().gdtp_solve_weighted(self._h, _ptr(x), _ptr(b), x.shape[0], w.ctypes.data, tol, max_iters, rel.ctypes.data,
+        _lib.check(_lib.load().gdtp_solve_weighted(self._h, px, pb, K, w.ctypes.data, tol, max_iters, rel.ctypes.data,
                                                    ctypes.byref(it), _stream()))
         return it.value, rel
 
