@@ -1,7 +1,8 @@
 """Texel hierarchy invariants (include/Src/HierarchyConstruction.inl, ProlongationAndRestriction.inl).
 
-The reference builder cannot be compiled in this image, so the hierarchy is pinned
-by the rules it must satisfy, checked from independent recomputations on the exported grids."""
+The reference builder cannot be compiled without stand-ins for headers this image lacks (its
+image reader needs the pre-1.5 libpng layout), so the hierarchy is pinned by the rules it must
+satisfy, checked from independent recomputations on the exported grids."""
 import numpy as np
 import pytest
 
