@@ -74,6 +74,9 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python -m gradientdomaintextureprocessing_amd.build` first")
+    # torch first: its HIP runtime then serves the library too (one runtime per process; a second
+    # copy loaded before torch leaves torch without a visible GPU)
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

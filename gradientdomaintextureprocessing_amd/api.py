@@ -354,6 +354,48 @@ class MultigridSystem:
         _lib.check(lib.gdtp_mixed_cycles(self._h, int(cycles)))
         _lib.check(lib.gdtp_mixed_configure(self._h, nu_pre, nu_post, boundary_sweeps, omega, 1 if use_graph else 0))
 
+    # (pre-smoothing, post-smoothing, Schwarz sweeps, flexible CG, V-cycles per residual)
+    TUNE_CANDIDATES = [(1, 2, 1, False, 2), (1, 2, 1, False, 1), (1, 1, 1, False, 2), (2, 2, 1, False, 2), (2, 1, 1, False, 2),
+                       (1, 2, 2, False, 2), (1, 2, 1, True, 1), (1, 1, 1, True, 1), (2, 2, 1, True, 1)]
+
+    def autotune_mixed(self, columns=None, tol=1e-6, candidates=None, repeats=2, max_iters=60, seed=0):
+        """Time mixed-solver schedules on this system and keep the fastest.  Each candidate
+        (pre, post, Schwarz sweeps, flexible CG, V-cycles per residual) solves seeded right-hand
+        sides b = M f (f uniform [0,1), `columns` columns, default max_cols) from x = 0 to `tol`,
+        `repeats` times after one untimed run (graph capture); candidates that do not converge are
+        skipped.  Returns the measurements, fastest first; the system keeps the fastest schedule."""
+        import time
+        import torch
+        K = columns or self.max_cols
+        N = self._texels[0]
+        f = torch.from_numpy(np.random.default_rng(seed).random((K, N))).cuda()
+        b = torch.empty_like(f)
+        self.apply(OPERATOR_MASS, f, b)
+        x = torch.zeros_like(b)
+        results = []
+        for cand in candidates or self.TUNE_CANDIDATES:
+            pre, post, sweeps, krylov, cycles = cand
+            self.configure_mixed(pre, post, sweeps, 1.0, True, krylov, cycles)
+            try:
+                x.zero_()
+                self.solve(x, b, tol=tol, max_iters=max_iters)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(repeats):
+                    x.zero_()
+                    its, rel = self.solve(x, b, tol=tol, max_iters=max_iters)
+                torch.cuda.synchronize()
+                results.append(dict(schedule=cand, ms=(time.perf_counter() - t0) / repeats * 1e3, iterations=its))
+            except _lib.GdtpError as e:
+                if e.code != 3:
+                    raise
+        if not results:
+            raise RuntimeError("no candidate schedule reached the tolerance")
+        results.sort(key=lambda r: r["ms"])
+        pre, post, sweeps, krylov, cycles = results[0]["schedule"]
+        self.configure_mixed(pre, post, sweeps, 1.0, True, krylov, cycles)
+        return results
+
     def mixed_deep_sweep(self, level, ncols):
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
 
