@@ -183,8 +183,8 @@ def serving_bench(qps, n_requests, tol, seed=0):
             v, t, _ = mesh.arrays()
             ph = np.random.default_rng(p["seed"]).random(3) * 6.28
             field = g.surface_gradient_field(v, t, np.sin(3 * v[:, 0] + ph[0]) + np.cos(2 * v[:, 1] + ph[1]) + np.sin(4 * v[:, 2] + ph[2]))
-            lic = g.LineIntegralConvolution(mesh, W, W, levels, field, max_cols=8)
-        H = g.InitializeHierarchy(mesh, W, W, levels)
+            lic = g.LineIntegralConvolution(mesh, W, W, levels, field, max_cols=8, assembly="device")
+        H = g.InitializeHierarchy(mesh, W, W, levels, assembly="device")
         server.add_mesh(name, H, max_cols=8, lic=lic)
         sizes[name] = H.level_info(0)["texels"]
         log(f"[bench] serving: {name} resident ({sizes[name]} texels, {levels} levels), {time.time() - t0:.1f}s")

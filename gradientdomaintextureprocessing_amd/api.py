@@ -547,11 +547,11 @@ class LineIntegralConvolution:
     s = 100 (sharpnessWeight).  vector_field: per mesh triangle, edge-basis components."""
 
     def __init__(self, mesh, width, height, levels, vector_field, convolution_weight=1e-4, stiffness_weight=1e-4,
-                 sharpening_weight=100.0, max_cols=3):
+                 sharpening_weight=100.0, max_cols=3, assembly="host"):
         v, t, _ = mesh.arrays()
         metric = vector_field_metric(embedding_metric(v, t), vector_field)
-        self.anisotropic = InitializeHierarchy(mesh, width, height, levels, triangle_metric=metric)
-        self.embedding = InitializeHierarchy(mesh, width, height, levels)
+        self.anisotropic = InitializeHierarchy(mesh, width, height, levels, triangle_metric=metric, assembly=assembly)
+        self.embedding = InitializeHierarchy(mesh, width, height, levels, assembly=assembly)
         self.convolution_weight, self.stiffness, self.sharpening = convolution_weight, stiffness_weight, sharpening_weight
         self.lic = UpdateLinearSystem(self.anisotropic, convolution_weight, 1.0, max_cols=max_cols)
         self.sharpen = UpdateLinearSystem(self.embedding, stiffness_weight, 1.0, max_cols=max_cols)
