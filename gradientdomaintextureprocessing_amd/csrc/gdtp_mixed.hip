@@ -328,10 +328,14 @@ __global__ __launch_bounds__(256) void KMixRestrictBoundary(int B, const int* __
 
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
-// e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid), all four
-// fine colour planes of a tile per workgroup (they share their coarse neighbours).  Deep texels of
-// plane `skipQ` are left alone: the Gauss-Seidel sweep that follows overwrites them without
-// reading them (skipQ = -1 when no sweep follows).
+// e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid).  A thread
+// owns one fine texel of each of the four colour planes, (2a + qi, 2b + qj); the coarse neighbours of
+// all four lie in one 2x2 coarse block (columns ci0, ci0 + 1, rows cj0, cj0 + 1), loaded once where an
+// active texel needs it, so every load of a column chunk is in flight together.  Per direction the
+// fine texel with even offset from the coarse grid takes one coarse column (base bx), the odd one the
+// mean of two (bx, bx + 1): bx = 0 for both when ui0 is even, bx = qi when ui0 is odd.  Deep texels of
+// plane `skipQ` are left alone: the Gauss-Seidel sweep that follows overwrites them without reading
+// them (skipQ = -1 when no sweep follows).
 __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ tiles, const MixChart* __restrict__ fch,
                                                       const MixChart* __restrict__ cch, long fplane, long fNd, long cplane, long cNd,
                                                       const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K,
@@ -340,32 +344,58 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
   const MixChart f = fch[T[0]], c = cch[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= f.w2 || b >= f.h2) return;
+  const int ui0 = 2 * a - f.cofx, uj0 = 2 * b - f.cofy;
+  const int ci0 = FloorHalf(ui0) + c.cofx, cj0 = FloorHalf(uj0) + c.cofy;
+  const bool ei = (ui0 & 1) == 0, ej = (uj0 & 1) == 0;
+  int fidx[4];
+  bool act[4];
+  int need = 0;  // bit cx + 2 cy: coarse block entry (ci0 + cx, cj0 + cy) is read
+#pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int fi = 2 * a + (q & 1), fj = 2 * b + (q >> 1);
-    if (fi >= f.w || fj >= f.h) continue;
+    const int qi = q & 1, qj = q >> 1;
     const long idx = (long)q * fplane + f.base2 + (long)b * f.w2 + a;
-    const int t = ftype[idx];
-    if (t < 0 || (q == skipQ && t == 2)) continue;
-    const int ui = fi - f.cofx, uj = fj - f.cofy;
-    const int ci = FloorHalf(ui) + c.cofx, cj = FloorHalf(uj) + c.cofy;
-    const bool oi = (ui & 1) != 0, oj = (uj & 1) != 0;
-    const int n00 = DenseByte(c, cplane, ci, cj);
-    const int n10 = oi ? DenseByte(c, cplane, ci + 1, cj) : n00;
-    const int n01 = oj ? DenseByte(c, cplane, ci, cj + 1) : n00;
-    const int n11 = (oi && oj) ? DenseByte(c, cplane, ci + 1, cj + 1) : (oi ? n10 : n01);
-    ForColumnChunks(K, [&](int c0, auto kc) {
-      constexpr int KC = decltype(kc)::value;
-      float v[KC][4], old[KC];
+    fidx[q] = (int)(4 * idx);
+    const bool in = 2 * a + qi < f.w && 2 * b + qj < f.h;
+    const int t = in ? ftype[idx] : -1;
+    act[q] = t >= 0 && !(q == skipQ && t == 2);
+    const int bx = ei ? 0 : qi, by = ej ? 0 : qj;
+    const bool ox = ei ? qi == 1 : qi == 0, oy = ej ? qj == 1 : qj == 0;
+    if (act[q]) need |= ((1 | (ox ? 2 : 0)) * (1 | (oy ? 4 : 0))) << (bx + 2 * by);
+  }
+  if (!need) return;
+  int nb[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) nb[n] = DenseByte(c, cplane, ci0 + (n & 1), cj0 + (n >> 1));
+  ForColumnChunks(K, [&](int c0, auto kc) {
+    constexpr int KC = decltype(kc)::value;
+    float v[KC][4], old[KC][4];
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) {
+      const float* E = ce + (c0 + ch) * cNd;
+      const float* F = fe + (c0 + ch) * fNd;
+#pragma unroll
+      for (int n = 0; n < 4; n++) v[ch][n] = (need >> n) & 1 ? Ld(E, nb[n]) : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; q++) old[ch][q] = act[q] ? Ld(F, fidx[q]) : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (!act[q]) continue;
+      const int qi = q & 1, qj = q >> 1;
+      const bool ox = ei ? qi == 1 : qi == 0, oy = ej ? qj == 1 : qj == 0;
+      const bool sx = !ei && qi == 1, sy = !ej && qj == 1;  // base column / row is ci0 + 1 / cj0 + 1
 #pragma unroll
       for (int ch = 0; ch < KC; ch++) {
-        const float* E = ce + (c0 + ch) * cNd;
-        v[ch][0] = Ld(E, n00), v[ch][1] = Ld(E, n10), v[ch][2] = Ld(E, n01), v[ch][3] = Ld(E, n11);
-        old[ch] = Ld(fe + (c0 + ch) * fNd, (int)(4 * idx));
+        const float* V = v[ch];
+        const float b0 = sx ? V[1] : V[0], b1 = sx ? V[3] : V[2];  // base column, rows cj0 and cj0 + 1
+        const float v00 = sy ? b1 : b0;
+        const float v10 = ox ? (sy ? V[3] : V[1]) : v00;
+        const float v01 = oy ? b1 : v00;
+        const float v11 = (ox && oy) ? V[3] : (ox ? v10 : v01);
+        fe[(long)(c0 + ch) * fNd + fidx[q] / 4] = old[ch][q] + (v00 + v10 + v01 + v11) * 0.25f;
       }
-#pragma unroll
-      for (int ch = 0; ch < KC; ch++) fe[(long)(c0 + ch) * fNd + idx] = old[ch] + (v[ch][0] + v[ch][1] + v[ch][2] + v[ch][3]) * 0.25f;
-    });
-  }
+    }
+  });
 }
 
 // Coarsest level: e = A^{-1} rhs with the precomputed dense inverse (FP64), rhs / e in the dense
