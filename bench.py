@@ -73,8 +73,9 @@ def solve(S, x, b, tol, max_cycles, mode):
 def mixed_roofline(S, H, K, reps=10):
     """Average duration of one colour-plane launch of the four-colour deep Gauss-Seidel kernel
     (dominant kernel of the mixed path) on level 0, and its algorithmic bytes: per updated deep
-    texel nine FP32 stencil words, rhs read and e written (K columns); plus one read of the other
-    three colour planes of e at active texels."""
+    texel its stencil (the eight off-diagonal ratios in FP16, c4 and their rounding sum in FP32:
+    24 bytes), rhs read and e written (K columns); plus one read of the other three colour planes
+    of e at active texels."""
     import torch
     info = H.level_info(0)
     st = torch.cuda.current_stream()
@@ -88,7 +89,7 @@ def mixed_roofline(S, H, K, reps=10):
     torch.cuda.synchronize()
     launches = 4 * reps
     per_launch_s = e0.elapsed_time(e1) / 1e3 / launches
-    bytes_per_launch = info["deep"] / 4 * (9 * 4 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
+    bytes_per_launch = info["deep"] / 4 * (8 * 2 + 2 * 4 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
     achieved = bytes_per_launch / per_launch_s
     return dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK, traffic=None,
                 kernel="KMixDeepGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)

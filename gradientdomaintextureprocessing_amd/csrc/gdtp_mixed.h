@@ -15,6 +15,8 @@ struct DevMixLevel {
   MixChart* charts = nullptr;      // device
   std::vector<MixChart> hcharts;   // host copy
   float* stencil = nullptr;        // [10][Nd]: c0..c8 reciprocal form, slot 9 = a4; zero off deep texels
+  uint4* stencilH = nullptr;       // [Nd]: the smoother's off-diagonal ratios c0..c3, c5..c8 as FP16 pairs
+  float2* stencilC = nullptr;      // [Nd]: the smoother's c4 and the FP16 rounding sum sum_k (c_k - fp16(c_k))
   int8_t* type = nullptr;          // [Nd]: -1 inactive, 0/1 boundary, 2 deep
   int* g2d = nullptr;              // [N] compact global index -> dense index
   int nTiles = 0;

@@ -65,20 +65,22 @@ __device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
 // Update of deep texel (i, j) of plane q (sub-grid (a, b)): e = c4 rhs - sum_{k!=4} c_k e_k.  The
 // neighbours lie in the other three planes, so every load of a chunk of columns is issued before
 // its stores (few memory round trips per texel: small levels are latency bound).
-__device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float* __restrict__ st,
-                                            float* e, const float* __restrict__ rhs, int K) {
+__device__ __forceinline__ float Half(unsigned bits) { return (float)__builtin_bit_cast(_Float16, (unsigned short)bits); }
+
+__device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float2* __restrict__ stc,
+                                            const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
   const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
   if (i >= c.w || j >= c.h) return;
   const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-  const float c4 = st[4 * Nd + idx];
+  const float2 cd = stc[idx];
+  const float c4 = cd.x, corr = 0.125f * cd.y;
   if (c4 == 0.0f) return;
-  float cf[9];
+  const uint4 w = sth[idx];
+  const float cf[9] = {Half(w.x & 0xffff), Half(w.x >> 16), Half(w.y & 0xffff), Half(w.y >> 16), 0.0f,
+                       Half(w.z & 0xffff), Half(w.z >> 16), Half(w.w & 0xffff), Half(w.w >> 16)};
   int nb[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
-    nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
-  }
+  for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
   ForColumnChunks(K, [&](int c0, auto kc) {
     constexpr int KC = decltype(kc)::value;
     float v[KC][9], r[KC];
@@ -91,22 +93,23 @@ __device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long 
     }
 #pragma unroll
     for (int ch = 0; ch < KC; ch++) {
-      float s = 0.0f;
+      float s = 0.0f, m = 0.0f;
 #pragma unroll
       for (int k = 0; k < 9; k++)
-        if (k != 4) s += cf[k] * v[ch][k];
-      e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s;
+        if (k != 4) s += cf[k] * v[ch][k], m += v[ch][k];
+      e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s - corr * m;
     }
   });
 }
 
 __global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                  long Nd, const float* __restrict__ st, float* e, const float* __restrict__ rhs, int K) {
+                                                  long Nd, const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
+                                                  const float* __restrict__ rhs, int K) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= c.w2 || b >= c.h2) return;
-  DeepGsTexel(c, plane, Nd, q, a, b, st, e, rhs, K);
+  DeepGsTexel(c, plane, Nd, q, a, b, stc, sth, e, rhs, K);
 }
 
 // The same four-colour sweep with one 1024-thread workgroup per chart, colours in order (post:
@@ -114,12 +117,13 @@ __global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__
 // four launches of KMixDeepGs in one, for levels whose charts are small enough that launch
 // latency, not memory, bounds a sweep.  All threads reach every barrier.
 __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __restrict__ charts, long plane, long Nd,
-                                                    const float* __restrict__ st, float* e, const float* __restrict__ rhs, int K) {
+                                                    const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
+                                                    const float* __restrict__ rhs, int K) {
   const MixChart c = charts[blockIdx.x];
   const int aw = (c.w + 1) / 2, n = aw * c.h2;
   for (int qq = 0; qq < 4; qq++) {
     const int q = post ? 3 - qq : qq;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel(c, plane, Nd, q, t % aw, t / aw, st, e, rhs, K);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel(c, plane, Nd, q, t % aw, t / aw, stc, sth, e, rhs, K);
     __syncthreads();
   }
 }
@@ -881,6 +885,28 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         for (int k = 0; k < 10; k++) st[(size_t)k * d.Nd + di] = (float)rc[10 * (size_t)dd + k];
       }
       d.stencil = Up(ms, st);
+      // The smoother's off-diagonal ratios in FP16 (|c_k| <= ~1, relative rounding 2^-11) and, in
+      // FP32, c4 and the rounding sum s = sum_k (c_k - fp16(c_k)).  The sweep subtracts s times the
+      // mean of the eight neighbours, which restores the row sum: the FP16 rounding then only acts
+      // on the local variation of e, not on its smooth part (a row-sum error would inject noise of
+      // 1e-4 |e| per sweep, fatal on nearly singular operators such as 1e-6 M + S).  Residuals keep
+      // the FP32 stencil above.
+      std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
+      std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
+      for (size_t t = 0; t < (size_t)d.Nd; t++) {
+        double round = 0.0;
+        for (int k = 0, o = 0; k < 9; k++) {
+          if (k == 4) continue;
+          const float v = st[(size_t)k * d.Nd + t];
+          if (!(std::fabs(v) < 65504.0f)) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
+          const _Float16 h = (_Float16)v;
+          sh[8 * t + o++] = __builtin_bit_cast(uint16_t, h);
+          round += (double)v - (double)(float)h;
+        }
+        sc[2 * t] = st[4 * (size_t)d.Nd + t], sc[2 * t + 1] = (float)round;
+      }
+      d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
+      d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
       if (l == 0 && H.numDeep > 0) {
         std::vector<double> ratio(H.numDeep);
         for (int dd = 0; dd < H.numDeep; dd++) {
@@ -996,11 +1022,11 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
   for (int it = 0; it < nu; it++) {
     if (!post) BoundarySweeps(ms, d, K, s);
     if (d.nTiles > 0 && d.chartSweeps) {
-      KMixChartGs<<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+      KMixChartGs<<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
     } else if (d.nTiles > 0) {
       for (int qq = 0; qq < 4; qq++) {
         int q = post ? 3 - qq : qq;
-        KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+        KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
       }
     }
     if (post) BoundarySweeps(ms, d, K, s);
@@ -1022,7 +1048,8 @@ static constexpr double kTwoCycleRatio = 2e-6;
 void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
   if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
   DevMixLevel& d = ms.L[l];
-  for (int q = 0; q < 4; q++) KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, K);
+  for (int q = 0; q < 4; q++)
+    KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
 }
 
 void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
