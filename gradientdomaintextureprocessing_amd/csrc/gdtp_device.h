@@ -47,6 +47,8 @@ struct DevLevel {
   int N = 0, D = 0, B = 0;
   double* coef = nullptr;      // reciprocal (levels < L-1) deep stencil, SoA [10][D]
   double* coefRaw = nullptr;   // raw deep stencil A, SoA [10][D]
+  double* coefSym = nullptr;   // level 0: A's deep couplings in symmetric five-word form, SoA [5][N] over global
+                               // indices (slots 0-3: lower neighbours k = 0..3, slot 4: diagonal)
   int* bgi = nullptr;          // boundary -> global
   DevCsr bd, bb, brestr;       // A_BD, A_BB, boundary restriction to level+1
   // band tasks

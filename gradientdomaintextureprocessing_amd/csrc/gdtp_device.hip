@@ -492,6 +492,30 @@ static std::vector<double> SoA10(const std::vector<double>& aos) {
   return s;
 }
 
+// Level-0 deep stencils in symmetric five-word form (the mixed path's FP64 residual): slot k < 4 of
+// texel g holds its coupling to its lower neighbour k (k = 3(dj+1) + (di+1): (-1,-1), (0,-1),
+// (+1,-1), (-1,0)), slot 4 the diagonal, SoA [5][N] over global indices.  A deep row reads its upper
+// coupling k = 5..8 from slot 8-k of the upper neighbour (g+1, next-1, next, next+1); a neighbour
+// that is not deep gets that slot from the deep row's own stencil.  Assembled couplings agree with
+// their mirror to rounding (within 1 ulp), so this is A up to 1 ulp in some off-diagonal entries.
+static std::vector<double> SymStencil(const std::vector<double>& aos, const std::vector<Line5>& lines, int N) {
+  std::vector<double> s(5 * (size_t)N, 0.0);
+  std::vector<char> deep(N, 0);
+  for (const Line5& x : lines)
+    for (int g = x.start; g <= x.end; g++) deep[g] = 1;
+  for (const Line5& x : lines)
+    for (int i = 0; i <= x.end - x.start; i++) {
+      const size_t d = (size_t)x.coeff + i;
+      const int cur = x.start + i, next = x.next + i;
+      for (int k = 0; k < 5; k++) s[k * (size_t)N + cur] = aos[10 * d + k];
+      const int up[4] = {cur + 1, next - 1, next, next + 1};
+      if (cur + 1 >= N || next - 1 < 0 || next + 1 >= N) throw std::runtime_error("deep texel without its upper neighbours");
+      for (int k = 5; k < 9; k++)
+        if (!deep[up[k - 5]]) s[(8 - k) * (size_t)N + up[k - 5]] = aos[10 * d + k];
+    }
+  return s;
+}
+
 // Partition the elimination tree into subtree teams (at most `smax` rows) plus the top part, and
 // group rows by dependency level inside each team and inside the top.
 void UploadLdlt(std::vector<void*>& ds, const Ldlt& F, DevLdlt& d, int kcap) {
@@ -622,6 +646,7 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     d.N = H.numTexels, d.D = H.numDeep, d.B = H.numBoundary;
     d.coef = Upload(ds, SoA10(O.deepRecip));
     d.coefRaw = Upload(ds, SoA10(O.deepRaw));
+    if (l == 0) d.coefSym = Upload(ds, SymStencil(O.deepRaw, H.rasterLines, H.numTexels));
     d.bgi = Upload(ds, H.boundaryGlobalIndex);
     d.bd = UploadCsr(ds, O.bd);
     d.bb = UploadCsr(ds, O.bb);
@@ -661,6 +686,7 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     const DevLevel& l0 = ds.L[0];
     d.N = l0.N, d.D = l0.D, d.B = l0.B, d.bgi = l0.bgi, d.nRaster = l0.nRaster, d.raster = l0.raster;
     d.coefRaw = Upload(ds, SoA10(which == 0 ? op.deepMass : op.deepStiffness));
+    d.coefSym = Upload(ds, SymStencil(which == 0 ? op.deepMass : op.deepStiffness, h.levels[0].rasterLines, l0.N));
     d.bd = UploadCsr(ds, which == 0 ? op.bdMass : op.bdStiffness);
     d.bb = UploadCsr(ds, which == 0 ? op.bbMass : op.bbStiffness);
   }

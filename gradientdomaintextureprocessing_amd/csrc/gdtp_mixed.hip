@@ -464,7 +464,7 @@ __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
   __syncthreads();
 }
 
-// deep rows: r = b - A x with raw FP64 stencils along raster lines, up to four columns per pass
+// deep rows: r = b - A x with the symmetric five-word FP64 stencil along raster lines, up to four columns per pass
 // over the stencil; r -> FP32 dense rhs of the correction, e -> 0, |r|^2 -> norms[ch][slot]
 // One thread per deep texel d; its raster line gives the global indices of the rows above and
 // below (line = deepLine[d]: start, end, prev, next, first deep index).
@@ -475,8 +475,16 @@ __device__ __forceinline__ void DeepRow(const int* __restrict__ deepLine, const 
   cur = L[0] + i, prev = L[2] + i, next = L[3] + i;
 }
 
+// The nine couplings of deep row `cur` from the symmetric five-word stencil ([5][N], global indices):
+// its own lower couplings and diagonal, and each upper coupling k from slot 8-k of that neighbour.
+__device__ __forceinline__ void SymRow(const double* __restrict__ sym, int N, int cur, int next, double cf[9]) {
+#pragma unroll
+  for (int k = 0; k < 5; k++) cf[k] = sym[(long)k * N + cur];
+  cf[5] = sym[3L * N + cur + 1], cf[6] = sym[2L * N + next - 1], cf[7] = sym[(long)N + next], cf[8] = sym[next + 1];
+}
+
 __global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                       const double* __restrict__ coef, const double* __restrict__ x,
+                                                       const double* __restrict__ sym, const double* __restrict__ x,
                                                        const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
                                                        float* e0, long Nd, double* norms, int K) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,8 +496,7 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restr
     double acc[4] = {0, 0, 0, 0};
     if (on) {
       double cf[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) cf[k] = coef[(long)k * D + d];
+      SymRow(sym, N, cur, next, cf);
       const long di = g2d[cur];
 #pragma unroll
       for (int cc = 0; cc < 4; cc++) {
@@ -517,7 +524,7 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restr
 // read once per texel for up to four columns (parametric solves: one refinement loop serves
 // columns with different weights, its FP32 V-cycle built for one nearby pair of weights).
 __global__ __launch_bounds__(256) void K64ResidualDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                        const double* __restrict__ coefM, const double* __restrict__ coefS,
+                                                        const double* __restrict__ symM, const double* __restrict__ symS,
                                                         const double* __restrict__ x, const double* __restrict__ b, int N,
                                                         const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
                                                         const double* __restrict__ w, double* r64) {
@@ -530,8 +537,8 @@ __global__ __launch_bounds__(256) void K64ResidualDeepW(int D, const int* __rest
     double acc[4] = {0, 0, 0, 0};
     if (on) {
       double cm[9], cs[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d], cs[k] = coefS[(long)k * D + d];
+      SymRow(symM, N, cur, next, cm);
+      SymRow(symS, N, cur, next, cs);
       const long di = g2d[cur];
 #pragma unroll
       for (int cc = 0; cc < 4; cc++) {
@@ -1094,15 +1101,15 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
 // ------------------------------------------------------------------ flexible conjugate gradients
 // q = A p for per-column operators A_c = w[c][0] M + w[c][1] S (level 0; deep rows by raster line)
 __global__ __launch_bounds__(256) void K64ApplyDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                     const double* __restrict__ coefM, const double* __restrict__ coefS,
+                                                     const double* __restrict__ symM, const double* __restrict__ symS,
                                                      const double* __restrict__ p, int N, double* q, int K, const double* __restrict__ w) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= D) return;
   int cur, prev, next;
   DeepRow(deepLine, lines, d, cur, prev, next);
   double cm[9], cs[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) cm[k] = coefM[(long)k * D + d], cs[k] = coefS[(long)k * D + d];
+  SymRow(symM, N, cur, next, cm);
+  SymRow(symS, N, cur, next, cs);
   for (int ch = 0; ch < K; ch++) {
     const double* P = p + (long)ch * N;
     double sm = 0.0, ss = 0.0;
@@ -1227,7 +1234,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   if (weighted) {
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
     if (L0.nRaster > 0)
-      K64ResidualDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
+      K64ResidualDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
                                                   ms.norms, K, ms.weights, r64);
     if (L0.B > 0)
       K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(
@@ -1236,7 +1243,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
     return;
   }
   if (L0.nRaster > 0)
-    K64ResidualDeep<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, L0.coefRaw, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+    K64ResidualDeep<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, L0.coefSym, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
   if (L0.B > 0)
     K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
                                                                           L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
@@ -1259,7 +1266,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   KFcgDirection<<<NBlk(nk, 256), 256, 0, s>>>(nk, N, ms.z64, ms.beta, ms.p64);
   const DevLevel &M = ds.mass0, &S = ds.stiff0;
   if (L0.nRaster > 0)
-    K64ApplyDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefRaw, S.coefRaw, ms.p64, N, ms.q64, K, ms.weights);
+    K64ApplyDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, ms.p64, N, ms.q64, K, ms.weights);
   if (L0.B > 0)
     K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
                                                                        M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
