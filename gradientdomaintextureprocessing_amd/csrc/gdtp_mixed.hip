@@ -67,17 +67,29 @@ __device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
 // its stores (few memory round trips per texel: small levels are latency bound).
 __device__ __forceinline__ float Half(unsigned bits) { return (float)__builtin_bit_cast(_Float16, (unsigned short)bits); }
 
-__device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float2* __restrict__ stc,
-                                            const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
+// kHalf: the stencil as FP16 ratios with an FP32 row-sum correction (stc, sth), else the FP32
+// stencil st.
+template <bool kHalf>
+__device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float* __restrict__ st,
+                                            const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
+                                            const float* __restrict__ rhs, int K) {
   const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
   if (i >= c.w || j >= c.h) return;
   const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-  const float2 cd = stc[idx];
-  const float c4 = cd.x, corr = 0.125f * cd.y;
-  if (c4 == 0.0f) return;
-  const uint4 w = sth[idx];
-  const float cf[9] = {Half(w.x & 0xffff), Half(w.x >> 16), Half(w.y & 0xffff), Half(w.y >> 16), 0.0f,
-                       Half(w.z & 0xffff), Half(w.z >> 16), Half(w.w & 0xffff), Half(w.w >> 16)};
+  float c4, corr = 0.0f, cf[9];
+  if constexpr (kHalf) {
+    const float2 cd = stc[idx];
+    c4 = cd.x, corr = 0.125f * cd.y;
+    if (c4 == 0.0f) return;
+    const uint4 w = sth[idx];
+    cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
+    cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
+  } else {
+    c4 = st[4 * Nd + idx];
+    if (c4 == 0.0f) return;
+#pragma unroll
+    for (int k = 0; k < 9; k++) cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+  }
   int nb[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
@@ -93,37 +105,47 @@ __device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long 
     }
 #pragma unroll
     for (int ch = 0; ch < KC; ch++) {
-      float s = 0.0f, m = 0.0f;
+      if constexpr (kHalf) {
+        float s = 0.0f, m = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 9; k++)
-        if (k != 4) s += cf[k] * v[ch][k], m += v[ch][k];
-      e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s - corr * m;
+        for (int k = 0; k < 9; k++)
+          if (k != 4) s += cf[k] * v[ch][k], m += v[ch][k];
+        e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s - corr * m;
+      } else {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+          if (k != 4) s += cf[k] * v[ch][k];
+        e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s;
+      }
     }
   });
 }
 
+template <bool kHalf>
 __global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                  long Nd, const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
-                                                  const float* __restrict__ rhs, int K) {
+                                                  long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+                                                  const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= c.w2 || b >= c.h2) return;
-  DeepGsTexel(c, plane, Nd, q, a, b, stc, sth, e, rhs, K);
+  DeepGsTexel<kHalf>(c, plane, Nd, q, a, b, st, stc, sth, e, rhs, K);
 }
 
 // The same four-colour sweep with one 1024-thread workgroup per chart, colours in order (post:
 // 3..0) separated by barriers: deep stencils never reach across charts, so this is exactly the
 // four launches of KMixDeepGs in one, for levels whose charts are small enough that launch
 // latency, not memory, bounds a sweep.  All threads reach every barrier.
+template <bool kHalf>
 __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __restrict__ charts, long plane, long Nd,
-                                                    const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
-                                                    const float* __restrict__ rhs, int K) {
+                                                    const float* __restrict__ st, const float2* __restrict__ stc,
+                                                    const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
   const MixChart c = charts[blockIdx.x];
   const int aw = (c.w + 1) / 2, n = aw * c.h2;
   for (int qq = 0; qq < 4; qq++) {
     const int q = post ? 3 - qq : qq;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel(c, plane, Nd, q, t % aw, t / aw, stc, sth, e, rhs, K);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel<kHalf>(c, plane, Nd, q, t % aw, t / aw, st, stc, sth, e, rhs, K);
     __syncthreads();
   }
 }
@@ -809,6 +831,12 @@ static void BuildSchwarzBlocks(const Csr& bb, const std::vector<int>& bidx, Schw
   }
 }
 
+// The smoother reads its stencil in FP16 only where the median level-0 row sum / diagonal is at
+// least this: the FP16 rounding left after the row-sum correction still slows the nearly singular
+// geodesic distance system 1e-6 M + S (median ratio ~1e-12) from at most 18 to 28 iterations,
+// while screened systems such as M + 1e-3 S at 4096^2 (~3e-5) converge as with FP32.
+static constexpr double kHalfStencilRatio = 2e-6;
+
 void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& ds, int kcap, MixedSystem& ms) {
   const int nl = (int)h.levels.size();
   if (nl < 2) throw std::runtime_error("the mixed-precision solver needs at least two levels");
@@ -892,28 +920,6 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         for (int k = 0; k < 10; k++) st[(size_t)k * d.Nd + di] = (float)rc[10 * (size_t)dd + k];
       }
       d.stencil = Up(ms, st);
-      // The smoother's off-diagonal ratios in FP16 (|c_k| <= ~1, relative rounding 2^-11) and, in
-      // FP32, c4 and the rounding sum s = sum_k (c_k - fp16(c_k)).  The sweep subtracts s times the
-      // mean of the eight neighbours, which restores the row sum: the FP16 rounding then only acts
-      // on the local variation of e, not on its smooth part (a row-sum error would inject noise of
-      // 1e-4 |e| per sweep, fatal on nearly singular operators such as 1e-6 M + S).  Residuals keep
-      // the FP32 stencil above.
-      std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
-      std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
-      for (size_t t = 0; t < (size_t)d.Nd; t++) {
-        double round = 0.0;
-        for (int k = 0, o = 0; k < 9; k++) {
-          if (k == 4) continue;
-          const float v = st[(size_t)k * d.Nd + t];
-          if (!(std::fabs(v) < 65504.0f)) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
-          const _Float16 h = (_Float16)v;
-          sh[8 * t + o++] = __builtin_bit_cast(uint16_t, h);
-          round += (double)v - (double)(float)h;
-        }
-        sc[2 * t] = st[4 * (size_t)d.Nd + t], sc[2 * t + 1] = (float)round;
-      }
-      d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
-      d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
       if (l == 0 && H.numDeep > 0) {
         std::vector<double> ratio(H.numDeep);
         for (int dd = 0; dd < H.numDeep; dd++) {
@@ -924,6 +930,30 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         }
         std::nth_element(ratio.begin(), ratio.begin() + ratio.size() / 2, ratio.end());
         ms.rowSumRatio = ratio[ratio.size() / 2];
+      }
+      // The smoother's off-diagonal ratios in FP16 (|c_k| <= ~1, relative rounding 2^-11) and, in
+      // FP32, c4 and the rounding sum s = sum_k (c_k - fp16(c_k)).  The sweep subtracts s times the
+      // mean of the eight neighbours, which restores the row sum: the FP16 rounding then only acts
+      // on the local variation of e, not on its smooth part (a row-sum error would inject noise of
+      // 1e-4 |e| per sweep, fatal on nearly singular operators such as 1e-6 M + S).  Residuals keep
+      // the FP32 stencil above.
+      if (ms.rowSumRatio >= kHalfStencilRatio) {
+        std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
+        std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
+        for (size_t t = 0; t < (size_t)d.Nd; t++) {
+          double round = 0.0;
+          for (int k = 0, o = 0; k < 9; k++) {
+            if (k == 4) continue;
+            const float v = st[(size_t)k * d.Nd + t];
+            if (!(std::fabs(v) < 65504.0f)) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
+            const _Float16 h = (_Float16)v;
+            sh[8 * t + o++] = __builtin_bit_cast(uint16_t, h);
+            round += (double)v - (double)(float)h;
+          }
+          sc[2 * t] = st[4 * (size_t)d.Nd + t], sc[2 * t + 1] = (float)round;
+        }
+        d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
+        d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
       }
       // boundary rows: merged A_BB (off-diagonal) and A_BD with dense columns
       const Csr &bb = oh.levels[l].bb, &bd = oh.levels[l].bd;
@@ -1011,6 +1041,13 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
   (void)ds;
 }
 
+// One colour of the deep sweep: FP16 stencil where the level holds one (well-screened operators),
+// else FP32.
+static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
+  if (d.stencilH) KMixDeepGs<true><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+  else KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+}
+
 static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s) {
   if (d.B == 0 || d.nBlocks == 0) return;
   for (int sw = 0; sw < ms.boundarySweeps; sw++) {
@@ -1029,11 +1066,12 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
   for (int it = 0; it < nu; it++) {
     if (!post) BoundarySweeps(ms, d, K, s);
     if (d.nTiles > 0 && d.chartSweeps) {
-      KMixChartGs<<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
+      if (d.stencilH) KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+      else KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
     } else if (d.nTiles > 0) {
       for (int qq = 0; qq < 4; qq++) {
         int q = post ? 3 - qq : qq;
-        KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
+        DeepGsLaunch(d, q, K, s);
       }
     }
     if (post) BoundarySweeps(ms, d, K, s);
@@ -1056,7 +1094,7 @@ void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
   if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
   DevMixLevel& d = ms.L[l];
   for (int q = 0; q < 4; q++)
-    KMixDeepGs<<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencilC, d.stencilH, d.e, d.rhs, K);
+    DeepGsLaunch(d, q, K, s);
 }
 
 void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
