@@ -501,18 +501,31 @@ static std::vector<double> SoA10(const std::vector<double>& aos) {
 static std::vector<double> SymStencil(const std::vector<double>& aos, const std::vector<Line5>& lines, int N) {
   std::vector<double> s(5 * (size_t)N, 0.0);
   std::vector<char> deep(N, 0);
-  for (const Line5& x : lines)
-    for (int g = x.start; g <= x.end; g++) deep[g] = 1;
-  for (const Line5& x : lines)
-    for (int i = 0; i <= x.end - x.start; i++) {
+  const long nl = (long)lines.size();
+  bool bad = false;
+#pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
+  for (long r = 0; r < nl; r++) {
+    const Line5& x = lines[r];
+    if (x.start < 0 || x.end >= N) bad = true;
+    else
+      for (int g = x.start; g <= x.end; g++) deep[g] = 1;
+  }
+  // every (slot, texel) has one writer: a deep texel its own slots, a non-deep texel's slot 8-k the
+  // deep row below it in direction k
+#pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
+  for (long r = 0; r < nl; r++) {
+    const Line5& x = lines[r];
+    for (int i = 0; i <= x.end - x.start && !bad; i++) {
       const size_t d = (size_t)x.coeff + i;
       const int cur = x.start + i, next = x.next + i;
+      if (cur + 1 >= N || next - 1 < 0 || next + 1 >= N) { bad = true; break; }
       for (int k = 0; k < 5; k++) s[k * (size_t)N + cur] = aos[10 * d + k];
       const int up[4] = {cur + 1, next - 1, next, next + 1};
-      if (cur + 1 >= N || next - 1 < 0 || next + 1 >= N) throw std::runtime_error("deep texel without its upper neighbours");
       for (int k = 5; k < 9; k++)
         if (!deep[up[k - 5]]) s[(8 - k) * (size_t)N + up[k - 5]] = aos[10 * d + k];
     }
+  }
+  if (bad) throw std::runtime_error("deep raster line outside the level or without its upper neighbours");
   return s;
 }
 

@@ -940,18 +940,21 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       if (ms.rowSumRatio >= kHalfStencilRatio) {
         std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
         std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
-        for (size_t t = 0; t < (size_t)d.Nd; t++) {
+        bool range = true;
+#pragma omp parallel for schedule(static) reduction(&& : range)
+        for (long t = 0; t < d.Nd; t++) {
           double round = 0.0;
           for (int k = 0, o = 0; k < 9; k++) {
             if (k == 4) continue;
             const float v = st[(size_t)k * d.Nd + t];
-            if (!(std::fabs(v) < 65504.0f)) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
+            if (!(std::fabs(v) < 65504.0f)) range = false;
             const _Float16 h = (_Float16)v;
             sh[8 * t + o++] = __builtin_bit_cast(uint16_t, h);
             round += (double)v - (double)(float)h;
           }
           sc[2 * t] = st[4 * (size_t)d.Nd + t], sc[2 * t + 1] = (float)round;
         }
+        if (!range) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
         d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
         d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
       }
