@@ -431,6 +431,8 @@ def main():
         for _ in range(args.warmup):
             cycles, rel = solve(S, x, b, args.tol, max_cycles, mode)
             log(f"[bench] {mode} warmup: {cycles} iterations, rel res {rel:.2e}")
+        ref_bytes, mix_bytes = S.memory()
+        log(f"[bench] {mode} device memory: reference system {ref_bytes / 1e9:.2f} GB, mixed system {mix_bytes / 1e9:.2f} GB")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         cyc, lat = [], []
