@@ -14,9 +14,11 @@ struct DevMixLevel {
   long plane = 0, Nd = 0;          // dense colour-plane size and total dense size (4 planes)
   MixChart* charts = nullptr;      // device
   std::vector<MixChart> hcharts;   // host copy
-  float* stencil = nullptr;        // [10][Nd]: c0..c8 reciprocal form, slot 9 = a4; zero off deep texels
+  float* stencil = nullptr;        // [10][Nd]: c0..c8 reciprocal form, slot 9 = a4; zero off deep texels (FP32 smoother only)
   uint4* stencilH = nullptr;       // [Nd]: the smoother's off-diagonal ratios c0..c3, c5..c8 as FP16 pairs
   float2* stencilC = nullptr;      // [Nd]: the smoother's c4 and the FP16 rounding sum sum_k (c_k - fp16(c_k))
+  float* stencilSym = nullptr;     // [5][Nd]: raw A in symmetric five-word form for residuals (slots 0-3: lower
+                                   // neighbours (-1,-1) (0,-1) (+1,-1) (-1,0); slot 4: a4, zero off deep texels)
   int8_t* type = nullptr;          // [Nd]: -1 inactive, 0/1 boundary, 2 deep
   int* g2d = nullptr;              // [N] compact global index -> dense index
   int nTiles = 0;

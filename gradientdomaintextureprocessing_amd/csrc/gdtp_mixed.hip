@@ -154,7 +154,7 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
 // covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
 // once rather than once per plane.
 __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                        long Nd, const float* __restrict__ st, const float* __restrict__ e,
+                                                        long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                         const float* __restrict__ rhs, float* res, int K, int zeroQ) {
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
@@ -164,19 +164,18 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
     const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
     if (i >= c.w || j >= c.h) continue;
     const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-    const float a4 = st[9 * Nd + idx];
+    const float a4 = sym[4 * Nd + idx];
     if (a4 == 0.0f) continue;
     if (q == zeroQ) {  // relaxed last: its deep rows hold exactly, so the residual is zero
       for (int ch = 0; ch < K; ch++) res[(long)ch * Nd + idx] = 0.0f;
       continue;
     }
-    float cf[9];
     int nb[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      cf[k] = k == 4 ? 1.0f : st[k * Nd + idx];
-      nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
-    }
+    for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
+    // own lower couplings and a4; upper coupling k from slot 8-k of that neighbour
+    const float cf[9] = {sym[idx], sym[Nd + idx], sym[2 * Nd + idx], sym[3 * Nd + idx], a4,
+                         Ld(sym + 3 * Nd, nb[5]), Ld(sym + 2 * Nd, nb[6]), Ld(sym + Nd, nb[7]), Ld(sym, nb[8])};
     ForColumnChunks(K, [&](int c0, auto kc) {
       constexpr int KC = decltype(kc)::value;
       float v[KC][9], r[KC];
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ 
         float s = 0.0f;
 #pragma unroll
         for (int k = 0; k < 9; k++) s += cf[k] * v[ch][k];
-        res[(long)(c0 + ch) * Nd + idx] = r[ch] - s * a4;
+        res[(long)(c0 + ch) * Nd + idx] = r[ch] - s;
       }
     });
   }
@@ -919,7 +918,6 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         long di = g2dl[H.deepGlobalIndex[dd]];
         for (int k = 0; k < 10; k++) st[(size_t)k * d.Nd + di] = (float)rc[10 * (size_t)dd + k];
       }
-      d.stencil = Up(ms, st);
       if (l == 0 && H.numDeep > 0) {
         std::vector<double> ratio(H.numDeep);
         for (int dd = 0; dd < H.numDeep; dd++) {
@@ -931,13 +929,40 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         std::nth_element(ratio.begin(), ratio.begin() + ratio.size() / 2, ratio.end());
         ms.rowSumRatio = ratio[ratio.size() / 2];
       }
+      const bool half = ms.rowSumRatio >= kHalfStencilRatio;
+      if (!half) d.stencil = Up(ms, st);
+      // residuals: raw A in symmetric five-word form.  A deep texel reads its upper coupling k = 5..8
+      // from slot 8-k of that neighbour; a neighbour that is not deep gets that slot from the deep
+      // texel's own stencil (one writer per slot).
+      {
+        const std::vector<double>& raw = oh.levels[l].deepRaw;
+        std::vector<float> sy(5 * (size_t)d.Nd, 0.0f);
+        bool inside = true;
+#pragma omp parallel for schedule(static) reduction(&& : inside)
+        for (int dd = 0; dd < H.numDeep; dd++) {
+          const NodeInfo& ni = H.nodeInfo[H.deepGlobalIndex[dd]];
+          const MixChart& m = d.hcharts[ni.chart];
+          if (ni.ci < 1 || ni.cj < 1 || ni.ci + 1 >= m.w || ni.cj + 1 >= m.h) {
+            inside = false;
+            continue;
+          }
+          const long di = dense(ni.chart, ni.ci, ni.cj);
+          for (int k = 0; k < 5; k++) sy[k * (size_t)d.Nd + di] = (float)raw[10 * (size_t)dd + k];
+          for (int k = 5; k < 9; k++) {
+            const long qn = dense(ni.chart, ni.ci + k % 3 - 1, ni.cj + k / 3 - 1);
+            if (type[qn] != 2) sy[(8 - k) * (size_t)d.Nd + qn] = (float)raw[10 * (size_t)dd + k];
+          }
+        }
+        if (!inside) throw std::runtime_error("deep texel on the edge of its chart grid");
+        d.stencilSym = Up(ms, sy);
+      }
       // The smoother's off-diagonal ratios in FP16 (|c_k| <= ~1, relative rounding 2^-11) and, in
       // FP32, c4 and the rounding sum s = sum_k (c_k - fp16(c_k)).  The sweep subtracts s times the
       // mean of the eight neighbours, which restores the row sum: the FP16 rounding then only acts
       // on the local variation of e, not on its smooth part (a row-sum error would inject noise of
       // 1e-4 |e| per sweep, fatal on nearly singular operators such as 1e-6 M + S).  Residuals keep
       // the FP32 stencil above.
-      if (ms.rowSumRatio >= kHalfStencilRatio) {
+      if (half) {
         std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
         std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
         bool range = true;
@@ -1106,7 +1131,7 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     Smooth(ms, l, K, false, s);  // e of this level was zeroed by whoever wrote its rhs
     if (d.nTiles > 0)
-      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.e, d.rhs, d.res, K,
+      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
                                                 ms.nuPre > 0 ? 3 : -1);
     if (d.B > 0)
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
