@@ -101,8 +101,9 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
 // Drop the captured refinement step (after the smoothing configuration changes).
 void InvalidateGraph(MixedSystem& ms);
 
-// One FP32 V-cycle of the correction equation on the current level-0 residual (exposed for profiling).
-void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s);
+// One FP32 V-cycle of the correction equation on the current level-0 residual; zeroE0: the level-0
+// correction is zero on entry (the first cycle after a residual was written).
+void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool zeroE0);
 // One four-colour deep Gauss-Seidel sweep of `level` on the solver's internal vectors.
 void MixedDeepSweep(MixedSystem& ms, int level, int K, hipStream_t s);
 

@@ -250,7 +250,8 @@ static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
 __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
                                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
                                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
-                                                                  const float* __restrict__ resB, int B, float* e, long Nd, int K,
+                                                                  const float* __restrict__ resB, const int* __restrict__ bdIdx,
+                                                                  const float* __restrict__ rhs, int B, float* e, long Nd, int K,
                                                                   float omega) {
   __shared__ float R[4 * kBlockExtMax];
   __shared__ float P[kSchwarzWaves][4][kBlockCore];
@@ -260,7 +261,8 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
     for (int j = threadIdx.x; j < nx; j += 64 * kSchwarzWaves)
-      for (int q = 0; q < kc; q++) R[q * kBlockExtMax + j] = resB[(long)(ch0 + q) * B + ext[x0 + j]];
+      for (int q = 0; q < kc; q++)  // resB null: e is zero on this level, the residual is the rhs
+        R[q * kBlockExtMax + j] = resB ? resB[(long)(ch0 + q) * B + ext[x0 + j]] : rhs[(long)(ch0 + q) * Nd + bdIdx[ext[x0 + j]]];
     __syncthreads();
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (i < nc)
@@ -1076,23 +1078,28 @@ static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
   else KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
 }
 
-static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s) {
+// zeroE: e is zero on the whole level (start of pre-smoothing), so the first sweep takes the rhs
+// as its residual without a residual pass.
+static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s, bool zeroE) {
   if (d.B == 0 || d.nBlocks == 0) return;
   for (int sw = 0; sw < ms.boundarySweeps; sw++) {
-    KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.tmpB,
-                                                                          1, d.Nd, K);
-    KMixSchwarz<<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, d.tmpB, d.B, d.e, d.Nd, K,
-                                         ms.omega);
+    const bool fromRhs = zeroE && sw == 0;
+    if (!fromRhs)
+      KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
+                                                                            d.tmpB, 1, d.Nd, K);
+    KMixSchwarz<<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+                                                         fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
   }
 }
 
 // Pre-smoothing: (boundary Schwarz, deep colours 0..3) x nuPre; post-smoothing is its adjoint
 // order: (deep colours 3..0, boundary Schwarz) x nuPost.
-static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
+// zeroE: e is zero on the whole level when pre-smoothing starts.
+static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool zeroE = false) {
   DevMixLevel& d = ms.L[l];
   const int nu = post ? ms.nuPost : ms.nuPre;
   for (int it = 0; it < nu; it++) {
-    if (!post) BoundarySweeps(ms, d, K, s);
+    if (!post) BoundarySweeps(ms, d, K, s, zeroE && it == 0);
     if (d.nTiles > 0 && d.chartSweeps) {
       if (d.stencilH) KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
       else KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
@@ -1102,7 +1109,7 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s) {
         DeepGsLaunch(d, q, K, s);
       }
     }
-    if (post) BoundarySweeps(ms, d, K, s);
+    if (post) BoundarySweeps(ms, d, K, s, false);
   }
 }
 
@@ -1125,11 +1132,13 @@ void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
     DeepGsLaunch(d, q, K, s);
 }
 
-void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s) {
+// zeroE0: the level-0 correction is zero on entry (first V-cycle after a residual was written);
+// coarser levels always start from zero (the restriction zeroes e where it writes the rhs).
+void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool zeroE0) {
   const int nl = ms.levels;
   for (int l = 0; l + 1 < nl; l++) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
-    Smooth(ms, l, K, false, s);  // e of this level was zeroed by whoever wrote its rhs
+    Smooth(ms, l, K, false, s, l > 0 || zeroE0);
     if (d.nTiles > 0)
       KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
                                                 ms.nuPre > 0 ? 3 : -1);
@@ -1402,7 +1411,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   int cycles = ms.cycles > 0 ? ms.cycles : (!krylov && ms.rowSumRatio >= kTwoCycleRatio ? 2 : 1);
   // one refinement step = V-cycles on the current residual, update, new residual
   auto step = [&](hipStream_t st, int cyc) {
-    for (int c = 0; c < cyc; c++) MixedVCycle(ds, ms, K, st);
+    for (int c = 0; c < cyc; c++) MixedVCycle(ds, ms, K, st, c == 0);  // the residual step zeroed e
     if (krylov) {
       FcgStep(ds, ms, x, K, st);
     } else {
