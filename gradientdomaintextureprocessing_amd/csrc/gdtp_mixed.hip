@@ -153,10 +153,10 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
 // res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4).  A workgroup
 // covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
 // once rather than once per plane.
-__global__ __launch_bounds__(256) void KMixResidualDeep(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                        long Nd, const float* __restrict__ sym, const float* __restrict__ e,
-                                                        const float* __restrict__ rhs, float* res, int K, int zeroQ) {
-  const int* T = tiles + 3 * blockIdx.x;
+__device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                 long Nd, const float* __restrict__ sym, const float* __restrict__ e,
+                                                 const float* __restrict__ rhs, float* res, int K, int zeroQ) {
+  const int* T = tiles + 3 * vb;
   const MixChart c = charts[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= c.w2 || b >= c.h2) return;
@@ -213,11 +213,11 @@ __device__ __forceinline__ T Pick(const T (&a)[4], int q) {
 
 // Boundary-row residual rhs - A e (FP32).  out: compact [K][B] when `compact`, else the dense
 // vector [K][Nd] at the row's texel.
-__global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                            const int* __restrict__ cl, const float* __restrict__ vl,
-                                                            const float* __restrict__ dg, const float* __restrict__ e,
-                                                            const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+__device__ __forceinline__ void BoundaryResidualRows(int vb, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                     const int* __restrict__ cl, const float* __restrict__ vl,
+                                                     const float* __restrict__ dg, const float* __restrict__ e,
+                                                     const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
+  const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
   if (r >= B) return;  // whole row groups leave together
   const int k0 = rp[r], k1 = rp[r + 1];
   const long d = bdIdx[r];
@@ -240,6 +240,24 @@ __global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __
       else out[o] = rv;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                            const int* __restrict__ cl, const float* __restrict__ vl,
+                                                            const float* __restrict__ dg, const float* __restrict__ e,
+                                                            const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
+  BoundaryResidualRows(blockIdx.x, B, bdIdx, rp, cl, vl, dg, e, rhs, out, compact, Nd, K);
+}
+
+// The residual of a level after pre-smoothing in one launch: blocks [0, nTiles) take the deep tiles,
+// the others the boundary rows (disjoint outputs, so the two parts overlap on the device).
+__global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                    long Nd, const float* __restrict__ sym, const float* __restrict__ e,
+                                                    const float* __restrict__ rhs, float* res, int K, int zeroQ, int B,
+                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                                    const float* __restrict__ vl, const float* __restrict__ dg) {
+  if ((int)blockIdx.x < nTiles) ResidualDeepTile(blockIdx.x, tiles, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
+  else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
 }
 
 // Restricted additive Schwarz over the boundary rows: one workgroup of kSchwarzWaves waves per
@@ -290,11 +308,11 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
 // ------------------------------------------------------------------ transfers
 // Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c (all four
 // coarse colour planes of a tile per workgroup: they read overlapping fine neighbourhoods).
-__global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ tiles, const MixChart* __restrict__ cch,
-                                                        const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
-                                                        const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
-                                                        int K) {
-  const int* T = tiles + 3 * blockIdx.x;
+__device__ __forceinline__ void RestrictDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ cch,
+                                                 const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
+                                                 const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
+                                                 int K) {
+  const int* T = tiles + 3 * vb;
   const MixChart c = cch[T[0]], f = fch[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= c.w2 || b >= c.h2) return;
@@ -326,11 +344,11 @@ __global__ __launch_bounds__(256) void KMixRestrictDeep(const int* __restrict__ 
 
 // Coarse boundary rhs: the boundary restriction matrix (fine dense columns); kRowLanes lanes per
 // coarse row read its entries coalesced, once for up to four columns, summed by shuffles.
-__global__ __launch_bounds__(256) void KMixRestrictBoundary(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                            const int* __restrict__ cl, const float* __restrict__ vl,
-                                                            const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
-                                                            int K) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
+__device__ __forceinline__ void RestrictBoundaryRows(int vb, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                     const int* __restrict__ cl, const float* __restrict__ vl,
+                                                     const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
+                                                     int K) {
+  const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
   if (r >= B) return;  // whole row groups leave together
   const int k0 = rp[r], k1 = rp[r + 1];
   const long d = bdIdx[r];
@@ -351,6 +369,17 @@ __global__ __launch_bounds__(256) void KMixRestrictBoundary(int B, const int* __
       ce[(long)(c0 + lane) * cNd + d] = 0.0f;
     }
   }
+}
+
+// Restriction into a level in one launch: blocks [0, nTiles) take the coarse deep tiles, the others
+// the coarse boundary rows (disjoint outputs).
+__global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __restrict__ tiles, const MixChart* __restrict__ cch,
+                                                    const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
+                                                    const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
+                                                    int K, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
+                                                    const int* __restrict__ cl, const float* __restrict__ vl) {
+  if ((int)blockIdx.x < nTiles) RestrictDeepTile(blockIdx.x, tiles, cch, fch, cplane, cNd, fplane, fNd, ctype, fres, crhs, ce, K);
+  else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, fres, fNd, crhs, ce, cNd, K);
 }
 
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
@@ -1139,18 +1168,13 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
   for (int l = 0; l + 1 < nl; l++) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     Smooth(ms, l, K, false, s, l > 0 || zeroE0);
-    if (d.nTiles > 0)
-      KMixResidualDeep<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
-                                                ms.nuPre > 0 ? 3 : -1);
-    if (d.B > 0)
-      KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs, d.res,
-                                                                            0, d.Nd, K);
-    if (c.nTiles > 0)
-      KMixRestrictDeep<<<c.nTiles, 256, 0, s>>>(c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs, c.e,
-                                                         K);
-    if (c.B > 0)
-      KMixRestrictBoundary<<<NBlk((long)c.B * kRowLanes, 256), 256, 0, s>>>(c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal, d.res, d.Nd, c.rhs,
-                                                                            c.e, c.Nd, K);
+    const long rb = d.B > 0 ? NBlk((long)d.B * kRowLanes, 256) : 0, cb = c.B > 0 ? NBlk((long)c.B * kRowLanes, 256) : 0;
+    if (d.nTiles + rb > 0)
+      KMixResidual<<<d.nTiles + rb, 256, 0, s>>>(d.nTiles, d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
+                                                 ms.nuPre > 0 ? 3 : -1, d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
+    if (c.nTiles + cb > 0)
+      KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs,
+                                                 c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);
   }
   {
     DevMixLevel& c = ms.L[nl - 1];
