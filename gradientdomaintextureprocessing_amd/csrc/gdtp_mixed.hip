@@ -535,11 +535,11 @@ __device__ __forceinline__ void SymRow(const double* __restrict__ sym, int N, in
   cf[5] = sym[3L * N + cur + 1], cf[6] = sym[2L * N + next - 1], cf[7] = sym[(long)N + next], cf[8] = sym[next + 1];
 }
 
-__global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                       const double* __restrict__ sym, const double* __restrict__ x,
-                                                       const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                       float* e0, long Nd, double* norms, int K) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void K64ResidualDeepRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                    const double* __restrict__ sym, const double* __restrict__ x,
+                                                    const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                    float* e0, long Nd, double* norms, int K) {
+  const int d = vb * blockDim.x + threadIdx.x;
   const bool on = d < D;
   int cur = 0, prev = 0, next = 0;
   if (on) DeepRow(deepLine, lines, d, cur, prev, next);
@@ -575,12 +575,12 @@ __global__ __launch_bounds__(256) void K64ResidualDeep(int D, const int* __restr
 // The same residual for per-column operators A_c = w[c][0] M + w[c][1] S: both level-0 stencils are
 // read once per texel for up to four columns (parametric solves: one refinement loop serves
 // columns with different weights, its FP32 V-cycle built for one nearby pair of weights).
-__global__ __launch_bounds__(256) void K64ResidualDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                        const double* __restrict__ symM, const double* __restrict__ symS,
-                                                        const double* __restrict__ x, const double* __restrict__ b, int N,
-                                                        const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
-                                                        const double* __restrict__ w, double* r64) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void K64ResidualDeepWRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                     const double* __restrict__ symM, const double* __restrict__ symS,
+                                                     const double* __restrict__ x, const double* __restrict__ b, int N,
+                                                     const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
+                                                     const double* __restrict__ w, double* r64) {
+  const int d = vb * blockDim.x + threadIdx.x;
   const bool on = d < D;
   int cur = 0, prev = 0, next = 0;
   if (on) DeepRow(deepLine, lines, d, cur, prev, next);
@@ -616,17 +616,17 @@ __global__ __launch_bounds__(256) void K64ResidualDeepW(int D, const int* __rest
 }
 
 // Boundary rows of the per-column-weights residual: rows of M (A_BB, A_BD) then of S.
-__global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
-                                                            const int* __restrict__ mcold, const double* __restrict__ mvald,
-                                                            const int* __restrict__ mrpb, const int* __restrict__ mcolb,
-                                                            const double* __restrict__ mvalb, const int* __restrict__ srpd,
-                                                            const int* __restrict__ scold, const double* __restrict__ svald,
-                                                            const int* __restrict__ srpb, const int* __restrict__ scolb,
-                                                            const double* __restrict__ svalb, const double* __restrict__ x,
-                                                            const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                            float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
-                                                            double* r64) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
+__device__ __forceinline__ void K64ResidualBoundaryWRows(int vb, int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
+                                                         const int* __restrict__ mcold, const double* __restrict__ mvald,
+                                                         const int* __restrict__ mrpb, const int* __restrict__ mcolb,
+                                                         const double* __restrict__ mvalb, const int* __restrict__ srpd,
+                                                         const int* __restrict__ scold, const double* __restrict__ svald,
+                                                         const int* __restrict__ srpb, const int* __restrict__ scolb,
+                                                         const double* __restrict__ svalb, const double* __restrict__ x,
+                                                         const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                         float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
+                                                         double* r64) {
+  const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
@@ -661,14 +661,14 @@ __global__ __launch_bounds__(256) void K64ResidualBoundaryW(int B, const int* __
   }
 }
 
-__global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
-                                                           const int* __restrict__ cold, const double* __restrict__ vald,
-                                                           const int* __restrict__ rpb, const int* __restrict__ colb,
-                                                           const double* __restrict__ valb, const double* __restrict__ x,
-                                                           const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                           float* e0, long Nd, double* norms, int K) {
+__device__ __forceinline__ void K64ResidualBoundaryRows(int vb, int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
+                                                        const int* __restrict__ cold, const double* __restrict__ vald,
+                                                        const int* __restrict__ rpb, const int* __restrict__ colb,
+                                                        const double* __restrict__ valb, const double* __restrict__ x,
+                                                        const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
+                                                        float* e0, long Nd, double* norms, int K) {
   // kRowLanes64 lanes per boundary row (no early exit: every thread takes part in the block sums)
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
+  const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
@@ -701,6 +701,37 @@ __global__ __launch_bounds__(256) void K64ResidualBoundary(int B, const int* __r
     }
     for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
   }
+}
+
+// The level-0 FP64 residual in one launch: blocks [0, deepBlocks) take the deep rows, the others
+// the boundary rows (disjoint rows; every block takes part in its own norm sums).
+__global__ __launch_bounds__(256) void K64Residual(int deepBlocks, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                   const double* __restrict__ sym, int B, const int* __restrict__ bgi,
+                                                   const int* __restrict__ rpd, const int* __restrict__ cold, const double* __restrict__ vald,
+                                                   const int* __restrict__ rpb, const int* __restrict__ colb, const double* __restrict__ valb,
+                                                   const double* __restrict__ x, const double* __restrict__ b, int N,
+                                                   const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K) {
+  if ((int)blockIdx.x < deepBlocks) K64ResidualDeepRows(blockIdx.x, D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K);
+  else K64ResidualBoundaryRows(blockIdx.x - deepBlocks, B, bgi, rpd, cold, vald, rpb, colb, valb, x, b, N, g2d, rhs0, e0, Nd, norms, K);
+}
+
+// The same for per-column weights (M and S rows of both parts).
+__global__ __launch_bounds__(256) void K64ResidualW(int deepBlocks, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                    const double* __restrict__ symM, const double* __restrict__ symS, int B,
+                                                    const int* __restrict__ bgi, const int* __restrict__ mrpd, const int* __restrict__ mcold,
+                                                    const double* __restrict__ mvald, const int* __restrict__ mrpb,
+                                                    const int* __restrict__ mcolb, const double* __restrict__ mvalb,
+                                                    const int* __restrict__ srpd, const int* __restrict__ scold,
+                                                    const double* __restrict__ svald, const int* __restrict__ srpb,
+                                                    const int* __restrict__ scolb, const double* __restrict__ svalb,
+                                                    const double* __restrict__ x, const double* __restrict__ b, int N,
+                                                    const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
+                                                    const double* __restrict__ w, double* r64) {
+  if ((int)blockIdx.x < deepBlocks)
+    K64ResidualDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64);
+  else
+    K64ResidualBoundaryWRows(blockIdx.x - deepBlocks, B, bgi, mrpd, mcold, mvald, mrpb, mcolb, mvalb, srpd, scold, svald, srpb, scolb,
+                             svalb, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64);
 }
 
 // grid-stride over a column (at most kNormBlocks blocks per column: few atomics per slot)
@@ -1199,10 +1230,10 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
 
 // ------------------------------------------------------------------ flexible conjugate gradients
 // q = A p for per-column operators A_c = w[c][0] M + w[c][1] S (level 0; deep rows by raster line)
-__global__ __launch_bounds__(256) void K64ApplyDeepW(int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
-                                                     const double* __restrict__ symM, const double* __restrict__ symS,
-                                                     const double* __restrict__ p, int N, double* q, int K, const double* __restrict__ w) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void K64ApplyDeepWRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                  const double* __restrict__ symM, const double* __restrict__ symS,
+                                                  const double* __restrict__ p, int N, double* q, int K, const double* __restrict__ w) {
+  const int d = vb * blockDim.x + threadIdx.x;
   if (d >= D) return;
   int cur, prev, next;
   DeepRow(deepLine, lines, d, cur, prev, next);
@@ -1222,15 +1253,15 @@ __global__ __launch_bounds__(256) void K64ApplyDeepW(int D, const int* __restric
   }
 }
 
-__global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
-                                                         const int* __restrict__ mcold, const double* __restrict__ mvald,
-                                                         const int* __restrict__ mrpb, const int* __restrict__ mcolb,
-                                                         const double* __restrict__ mvalb, const int* __restrict__ srpd,
-                                                         const int* __restrict__ scold, const double* __restrict__ svald,
-                                                         const int* __restrict__ srpb, const int* __restrict__ scolb,
-                                                         const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q,
-                                                         int K, const double* __restrict__ w) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
+__device__ __forceinline__ void K64ApplyBoundaryWRows(int vb, int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
+                                                      const int* __restrict__ mcold, const double* __restrict__ mvald,
+                                                      const int* __restrict__ mrpb, const int* __restrict__ mcolb,
+                                                      const double* __restrict__ mvalb, const int* __restrict__ srpd,
+                                                      const int* __restrict__ scold, const double* __restrict__ svald,
+                                                      const int* __restrict__ srpb, const int* __restrict__ scolb,
+                                                      const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q,
+                                                      int K, const double* __restrict__ w) {
+  const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   if (r >= B) return;  // whole row groups leave together
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
@@ -1255,6 +1286,21 @@ __global__ __launch_bounds__(256) void K64ApplyBoundaryW(int B, const int* __res
       q[(long)ch * N + bgi[r]] = w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane);
     }
   }
+}
+
+// q = A p for per-column operators in one launch: deep rows, then boundary rows.
+__global__ __launch_bounds__(256) void K64ApplyW(int deepBlocks, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
+                                                 const double* __restrict__ symM, const double* __restrict__ symS, int B,
+                                                 const int* __restrict__ bgi, const int* __restrict__ mrpd, const int* __restrict__ mcold,
+                                                 const double* __restrict__ mvald, const int* __restrict__ mrpb, const int* __restrict__ mcolb,
+                                                 const double* __restrict__ mvalb, const int* __restrict__ srpd, const int* __restrict__ scold,
+                                                 const double* __restrict__ svald, const int* __restrict__ srpb, const int* __restrict__ scolb,
+                                                 const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q, int K,
+                                                 const double* __restrict__ w) {
+  if ((int)blockIdx.x < deepBlocks) K64ApplyDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, p, N, q, K, w);
+  else
+    K64ApplyBoundaryWRows(blockIdx.x - deepBlocks, B, bgi, mrpd, mcold, mvald, mrpb, mcolb, mvalb, srpd, scold, svald, srpb, scolb, svalb, p,
+                          N, q, K, w);
 }
 
 // column dot products a_c . b_c: one partial sum per block (out[c][blockIdx.x]), no atomics, so
@@ -1330,22 +1376,18 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  const int db = L0.nRaster > 0 ? (int)NBlk(L0.D, 256) : 0;
+  const int bb = L0.B > 0 ? (int)NBlk((long)L0.B * kRowLanes64, 256) : 0;
+  if (db + bb == 0) return;
   if (weighted) {
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
-    if (L0.nRaster > 0)
-      K64ResidualDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd,
-                                                  ms.norms, K, ms.weights, r64);
-    if (L0.B > 0)
-      K64ResidualBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(
-          L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr,
-          S.bb.col, S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64);
+    K64ResidualW<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, L0.B, L0.bgi, M.bd.rowptr, M.bd.col,
+                                         M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
+                                         S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64);
     return;
   }
-  if (L0.nRaster > 0)
-    K64ResidualDeep<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, L0.coefSym, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
-  if (L0.B > 0)
-    K64ResidualBoundary<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val, L0.bb.rowptr,
-                                                                          L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+  K64Residual<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, L0.coefSym, L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val,
+                                      L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
 }
 
 // One flexible-CG step after the V-cycle left z = V(r) in the level-0 FP32 correction:
@@ -1364,12 +1406,12 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   KFcgBeta<<<1, cb, 0, s>>>(K, (int)gb, zq, ms.pqPrev, ms.first, ms.beta);
   KFcgDirection<<<NBlk(nk, 256), 256, 0, s>>>(nk, N, ms.z64, ms.beta, ms.p64);
   const DevLevel &M = ds.mass0, &S = ds.stiff0;
-  if (L0.nRaster > 0)
-    K64ApplyDeepW<<<NBlk(L0.D, 256), 256, 0, s>>>(L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, ms.p64, N, ms.q64, K, ms.weights);
-  if (L0.B > 0)
-    K64ApplyBoundaryW<<<NBlk((long)L0.B * kRowLanes64, 256), 256, 0, s>>>(L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val, M.bb.rowptr, M.bb.col,
-                                                                       M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
-                                                                       S.bb.val, ms.p64, N, ms.q64, K, ms.weights);
+  const int db = L0.nRaster > 0 ? (int)NBlk(L0.D, 256) : 0;
+  const int bb = L0.B > 0 ? (int)NBlk((long)L0.B * kRowLanes64, 256) : 0;
+  if (db + bb > 0)
+    K64ApplyW<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, L0.B, L0.bgi, M.bd.rowptr, M.bd.col, M.bd.val,
+                                      M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col, S.bb.val,
+                                      ms.p64, N, ms.q64, K, ms.weights);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.r64, ms.p64, rp);
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.p64, ms.q64, pq);
   KFcgAlpha<<<1, cb, 0, s>>>(K, (int)gb, rp, pq, ms.alpha, ms.pqPrev, ms.first);
