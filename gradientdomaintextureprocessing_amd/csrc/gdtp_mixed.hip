@@ -538,14 +538,14 @@ __device__ __forceinline__ void SymRow(const double* __restrict__ sym, int N, in
 __device__ __forceinline__ void K64ResidualDeepRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
                                                     const double* __restrict__ sym, const double* __restrict__ x,
                                                     const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                    float* e0, long Nd, double* norms, int K) {
+                                                    float* e0, long Nd, double* norms, int K, double* bnorms) {
   const int d = vb * blockDim.x + threadIdx.x;
   const bool on = d < D;
   int cur = 0, prev = 0, next = 0;
   if (on) DeepRow(deepLine, lines, d, cur, prev, next);
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
-    double acc[4] = {0, 0, 0, 0};
+    double acc[4] = {0, 0, 0, 0}, bacc[4] = {0, 0, 0, 0};
     if (on) {
       double cf[9];
       SymRow(sym, N, cur, next, cf);
@@ -561,13 +561,15 @@ __device__ __forceinline__ void K64ResidualDeepRows(int vb, int D, const int* __
           const int base = k < 3 ? prev : (k < 6 ? cur : next);
           s += cf[k] * X[base + k % 3 - 1];
         }
-        const double r = b[(long)ch * N + cur] - s;
+        const double bv = b[(long)ch * N + cur], r = bv - s;
         rhs0[(long)ch * Nd + di] = (float)r;
         e0[(long)ch * Nd + di] = 0.0f;
-        acc[cc] += r * r;
+        acc[cc] += r * r, bacc[cc] += bv * bv;
       }
     }
     for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
+    if (bnorms)  // the same for every thread: |b|^2 on the first residual of a solve
+      for (int cc = 0; cc < kc; cc++) BlockAddSquares(bacc[cc], bnorms + (long)(c0 + cc) * kSlots);
   }
 }
 
@@ -579,14 +581,14 @@ __device__ __forceinline__ void K64ResidualDeepWRows(int vb, int D, const int* _
                                                      const double* __restrict__ symM, const double* __restrict__ symS,
                                                      const double* __restrict__ x, const double* __restrict__ b, int N,
                                                      const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
-                                                     const double* __restrict__ w, double* r64) {
+                                                     const double* __restrict__ w, double* r64, double* bnorms) {
   const int d = vb * blockDim.x + threadIdx.x;
   const bool on = d < D;
   int cur = 0, prev = 0, next = 0;
   if (on) DeepRow(deepLine, lines, d, cur, prev, next);
   for (int c0 = 0; c0 < K; c0 += 4) {
     const int kc = min(4, K - c0);
-    double acc[4] = {0, 0, 0, 0};
+    double acc[4] = {0, 0, 0, 0}, bacc[4] = {0, 0, 0, 0};
     if (on) {
       double cm[9], cs[9];
       SymRow(symM, N, cur, next, cm);
@@ -604,14 +606,16 @@ __device__ __forceinline__ void K64ResidualDeepWRows(int vb, int D, const int* _
           const double xv = X[base + k % 3 - 1];
           sm += cm[k] * xv, ss += cs[k] * xv;
         }
-        const double r = b[(long)ch * N + cur] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
+        const double bv = b[(long)ch * N + cur], r = bv - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
         if (r64) r64[(long)ch * N + cur] = r;
         rhs0[(long)ch * Nd + di] = (float)r;
         e0[(long)ch * Nd + di] = 0.0f;
-        acc[cc] += r * r;
+        acc[cc] += r * r, bacc[cc] += bv * bv;
       }
     }
     for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
+    if (bnorms)
+      for (int cc = 0; cc < kc; cc++) BlockAddSquares(bacc[cc], bnorms + (long)(c0 + cc) * kSlots);
   }
 }
 
@@ -625,7 +629,7 @@ __device__ __forceinline__ void K64ResidualBoundaryWRows(int vb, int B, const in
                                                          const double* __restrict__ svalb, const double* __restrict__ x,
                                                          const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
                                                          float* e0, long Nd, double* norms, int K, const double* __restrict__ w,
-                                                         double* r64) {
+                                                         double* r64, double* bnorms) {
   const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
   for (int c0 = 0; c0 < K; c0 += 4) {
@@ -648,16 +652,18 @@ __device__ __forceinline__ void K64ResidualBoundaryWRows(int vb, int B, const in
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) am[q] = RowSum<double, kRowLanes64>(am[q]), as[q] = RowSum<double, kRowLanes64>(as[q]);
-    double sq = 0.0;
+    double sq = 0.0, bq = 0.0;
     if (active && lane < kc) {
       const int ch = c0 + lane, g = bgi[r];
-      const double rv = b[(long)ch * N + g] - (w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane));
+      const double bv = b[(long)ch * N + g], rv = bv - (w[2 * ch] * Pick(am, lane) + w[2 * ch + 1] * Pick(as, lane));
       if (r64) r64[(long)ch * N + g] = rv;
       rhs0[(long)ch * Nd + g2d[g]] = (float)rv;
       e0[(long)ch * Nd + g2d[g]] = 0.0f;
-      sq = rv * rv;
+      sq = rv * rv, bq = bv * bv;
     }
     for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
+    if (bnorms)
+      for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? bq : 0.0, bnorms + (long)(c0 + q) * kSlots);
   }
 }
 
@@ -666,7 +672,7 @@ __device__ __forceinline__ void K64ResidualBoundaryRows(int vb, int B, const int
                                                         const int* __restrict__ rpb, const int* __restrict__ colb,
                                                         const double* __restrict__ valb, const double* __restrict__ x,
                                                         const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
-                                                        float* e0, long Nd, double* norms, int K) {
+                                                        float* e0, long Nd, double* norms, int K, double* bnorms) {
   // kRowLanes64 lanes per boundary row (no early exit: every thread takes part in the block sums)
   const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes64), lane = threadIdx.x % kRowLanes64;
   const bool active = r < B;
@@ -691,15 +697,17 @@ __device__ __forceinline__ void K64ResidualBoundaryRows(int vb, int B, const int
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) acc[q] = RowSum<double, kRowLanes64>(acc[q]);
-    double sq = 0.0;
+    double sq = 0.0, bq = 0.0;
     if (active && lane < kc) {
       const int g = bgi[r];
-      const double rv = b[(long)(c0 + lane) * N + g] - Pick(acc, lane);
+      const double bv = b[(long)(c0 + lane) * N + g], rv = bv - Pick(acc, lane);
       rhs0[(long)(c0 + lane) * Nd + g2d[g]] = (float)rv;
       e0[(long)(c0 + lane) * Nd + g2d[g]] = 0.0f;
-      sq = rv * rv;
+      sq = rv * rv, bq = bv * bv;
     }
     for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
+    if (bnorms)
+      for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? bq : 0.0, bnorms + (long)(c0 + q) * kSlots);
   }
 }
 
@@ -710,9 +718,11 @@ __global__ __launch_bounds__(256) void K64Residual(int deepBlocks, int D, const 
                                                    const int* __restrict__ rpd, const int* __restrict__ cold, const double* __restrict__ vald,
                                                    const int* __restrict__ rpb, const int* __restrict__ colb, const double* __restrict__ valb,
                                                    const double* __restrict__ x, const double* __restrict__ b, int N,
-                                                   const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K) {
-  if ((int)blockIdx.x < deepBlocks) K64ResidualDeepRows(blockIdx.x, D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K);
-  else K64ResidualBoundaryRows(blockIdx.x - deepBlocks, B, bgi, rpd, cold, vald, rpb, colb, valb, x, b, N, g2d, rhs0, e0, Nd, norms, K);
+                                                   const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
+                                                   double* bnorms) {
+  if ((int)blockIdx.x < deepBlocks) K64ResidualDeepRows(blockIdx.x, D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
+  else
+    K64ResidualBoundaryRows(blockIdx.x - deepBlocks, B, bgi, rpd, cold, vald, rpb, colb, valb, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
 }
 
 // The same for per-column weights (M and S rows of both parts).
@@ -726,29 +736,21 @@ __global__ __launch_bounds__(256) void K64ResidualW(int deepBlocks, int D, const
                                                     const int* __restrict__ scolb, const double* __restrict__ svalb,
                                                     const double* __restrict__ x, const double* __restrict__ b, int N,
                                                     const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
-                                                    const double* __restrict__ w, double* r64) {
+                                                    const double* __restrict__ w, double* r64, double* bnorms) {
   if ((int)blockIdx.x < deepBlocks)
-    K64ResidualDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64);
+    K64ResidualDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64, bnorms);
   else
     K64ResidualBoundaryWRows(blockIdx.x - deepBlocks, B, bgi, mrpd, mcold, mvald, mrpb, mcolb, mvalb, srpd, scold, svald, srpb, scolb,
-                             svalb, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64);
+                             svalb, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64, bnorms);
 }
 
-// grid-stride over a column (at most kNormBlocks blocks per column: few atomics per slot)
-static constexpr unsigned kNormBlocks = 1024;
-__global__ __launch_bounds__(256) void K64SquaredNorm(int N, const double* __restrict__ v, double* out) {
-  const int ch = blockIdx.y;
-  const double* V = v + (long)ch * N;
-  double a = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) a += V[i] * V[i];
-  BlockAddSquares(a, out + (long)ch * kSlots);
-}
-
-__global__ void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)N * K) return;
-  const int g = t % N, ch = t / N;
-  x[(long)ch * N + g] += (double)e0[(long)ch * Nd + g2d[g]];
+// x += e at every texel: one thread per texel for all columns (its dense index read once)
+__global__ __launch_bounds__(256) void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const long di = g2d[g];
+#pragma unroll 4
+  for (int ch = 0; ch < K; ch++) x[(long)ch * N + g] += (double)e0[(long)ch * Nd + di];
 }
 
 // ------------------------------------------------------------------ host
@@ -1371,8 +1373,9 @@ __global__ __launch_bounds__(256) void KFcgUpdate(int N, const int* __restrict__
 }
 
 // residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
+// bnorms (first residual of a solve): |b|^2 per column into bnorms[ch][slot] as well.
 static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s,
-                          double* r64 = nullptr) {
+                          double* r64 = nullptr, double* bnorms = nullptr) {
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
@@ -1383,11 +1386,11 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
     K64ResidualW<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, L0.B, L0.bgi, M.bd.rowptr, M.bd.col,
                                          M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
-                                         S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64);
+                                         S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64, bnorms);
     return;
   }
   K64Residual<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, L0.coefSym, L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val,
-                                      L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K);
+                                      L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, bnorms);
 }
 
 // One flexible-CG step after the V-cycle left z = V(r) in the level-0 FP32 correction:
@@ -1421,8 +1424,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
 
 static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
   DevMixLevel& d = ms.L[0];
-  const long nk = (long)d.N * K;
-  K64Update<<<NBlk(nk, 256), 256, 0, s>>>(d.N, d.g2d, d.e, d.Nd, x, K);
+  K64Update<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, d.e, d.Nd, x, K);
   (void)ds;
 }
 
@@ -1447,12 +1449,10 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
       if (!(weights[2 * k] > 0)) throw std::runtime_error("the mass weight of every column must be positive");
     GDTP_HIP(hipMemcpyAsync(ms.weights, weights, 2 * (size_t)K * sizeof(double), hipMemcpyHostToDevice, s));
   }
-  const int N = ms.L[0].N;
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
-  K64SquaredNorm<<<dim3(std::min(NBlk(N, 256), kNormBlocks), K), 256, 0, s>>>(N, b, bNorms);
   if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
-  OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr);
+  OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr, bNorms);  // |b|^2 in the same pass
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
   auto slotSum = [&](const double* p) {
