@@ -346,9 +346,11 @@ class MultigridSystem:
         return it.value, rel
 
     def configure_mixed(self, nu_pre=1, nu_post=2, boundary_sweeps=1, omega=1.0, use_graph=True, krylov=False, cycles=0):
-        """Smoothing schedule of the FP32 V-cycle, graph capture, the outer iteration (flexible CG
-        with krylov=True, else plain iterative refinement) and the V-cycles per outer iteration (0:
-        automatic, two unless the operator is nearly singular in FP32)."""
+        """Smoothing schedule of the FP32 V-cycle (nu_pre steps of boundary Schwarz then the four
+        deep colours; nu_post sweeps of the deep colours in reverse order, then one boundary Schwarz
+        step; boundary_sweeps Schwarz sweeps per Schwarz step), graph capture, the outer iteration
+        (flexible CG with krylov=True, else plain iterative refinement) and the V-cycles per outer
+        iteration (0: automatic, two unless the operator is nearly singular in FP32)."""
         lib = _lib.load()
         _lib.check(lib.gdtp_mixed_acceleration(self._h, 1 if krylov else 0))
         _lib.check(lib.gdtp_mixed_cycles(self._h, int(cycles)))

@@ -45,7 +45,7 @@ struct DevMixLevel {
 
 struct MixedSystem {
   int levels = 0, kcap = 0;
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per smoothing step
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per Schwarz step
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level

@@ -158,9 +158,11 @@ int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol
  * batched requests with different weights. */
 int gdtp_solve_weighted(gdtp_system* s, double* x, const double* b, int ncols, const double* weights, double tol, int max_iters,
                         double* rel_res, int* iterations, void* stream);
-/* Mixed-solver smoothing: pre / post smoothing steps, boundary Schwarz sweeps per step and
- * the damping of their block update, hipGraph capture of the refinement step (1) or plain
- * launches (0).  Builds the mixed solver's device data if it does not exist yet. */
+/* Mixed-solver smoothing: pre / post smoothing steps (a pre step: boundary Schwarz, then the four
+ * deep colours; the post steps: the deep colours in reverse order, then one boundary Schwarz step
+ * after the last), boundary Schwarz sweeps per Schwarz step and the damping of their block update,
+ * hipGraph capture of the refinement step (1) or plain launches (0).  Builds the mixed solver's
+ * device data if it does not exist yet. */
 int gdtp_mixed_configure(gdtp_system* s, int nu_pre, int nu_post, int boundary_sweeps, double omega, int use_graph);
 /* Outer iteration of the mixed solver: plain iterative refinement x += V(b - A x) (0, the
  * default: fewest passes over the operator per iteration) or flexible conjugate gradients
