@@ -1043,9 +1043,12 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
           }
           sc[2 * t] = st[4 * (size_t)d.Nd + t], sc[2 * t + 1] = (float)round;
         }
-        if (!range) throw std::runtime_error("mixed solver: stencil ratio outside the FP16 range on level " + std::to_string(l));
-        d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
-        d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
+        if (range) {
+          d.stencilH = reinterpret_cast<uint4*>(Up(ms, sh));
+          d.stencilC = reinterpret_cast<float2*>(Up(ms, sc));
+        } else {  // a ratio beyond the FP16 range: this level keeps the FP32 smoother
+          d.stencil = Up(ms, st);
+        }
       }
       // boundary rows: merged A_BB (off-diagonal) and A_BD with dense columns
       const Csr &bb = oh.levels[l].bb, &bd = oh.levels[l].bd;
