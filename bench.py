@@ -89,10 +89,16 @@ def mixed_roofline(S, H, K, reps=10):
     torch.cuda.synchronize()
     launches = 4 * reps
     per_launch_s = e0.elapsed_time(e1) / 1e3 / launches
-    bytes_per_launch = info["deep"] / 4 * (8 * 2 + 2 * 4 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
+    # B_relax = s*q + 3*K*p per updated deep texel (stencil once, e read, rhs read, e written),
+    # with the declared FP16-ratio stencil s*q = 8*2 + 2*4 = 24 bytes and p = 4; one launch
+    # updates one colour plane, a quarter of the deep texels.
+    bytes_per_launch = info["deep"] / 4 * (24 + 3 * K * 4)
+    # what the kernel itself must touch: the other three colour planes of e read at each launch
+    kernel_bytes = info["deep"] / 4 * (24 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
     achieved = bytes_per_launch / per_launch_s
     return dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK, traffic=None,
-                kernel="KMixDeepGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)
+                kernel="KMixDeepGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch,
+                kernel_bytes_per_launch=kernel_bytes)
 
 
 def band_roofline(S, H, x, b, r, reps=5):

@@ -1,9 +1,23 @@
 """ctypes binding of libgdtp.so (the C ABI declared in include/gdtp.h)."""
 import ctypes
+import hashlib
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgdtp.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+
+def source_hash():
+    """SHA-256 (hex, first 16 digits) over the names and contents of the library's sources; the
+    build embeds it (gdtp_build_id) so that a library older than its sources is refused."""
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(CSRC)):
+        if name.endswith((".h", ".cpp", ".hip")):
+            h.update(name.encode())
+            with open(os.path.join(CSRC, name), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
 
 c_int, c_double, c_void_p, c_char_p, c_size_t, c_ull = (ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p,
                                                          ctypes.c_size_t, ctypes.c_ulonglong)
@@ -12,6 +26,7 @@ P = ctypes.POINTER
 # name -> (restype, argtypes)
 SIGNATURES = {
     "gdtp_last_error": (c_char_p, []),
+    "gdtp_build_id": (c_char_p, []),
     "gdtp_device_count": (c_int, [P(c_int)]),
     "gdtp_mesh_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, P(c_void_p)]),
     "gdtp_mesh_read_ply": (c_int, [c_char_p, P(c_void_p)]),
@@ -67,8 +82,20 @@ class GdtpError(RuntimeError):
         self.code = code
 
 
+def is_current():
+    """True when libgdtp.so exists and was built from the sources in csrc/ as they are now."""
+    if not os.path.exists(LIB_PATH):
+        return False
+    lib = ctypes.CDLL(LIB_PATH)
+    if not hasattr(lib, "gdtp_build_id"):
+        return False
+    lib.gdtp_build_id.restype = c_char_p
+    return lib.gdtp_build_id().decode() == source_hash()
+
+
 def load():
-    """Load libgdtp.so (fails loudly if the HIP library has not been built)."""
+    """Load libgdtp.so (fails loudly if the HIP library has not been built, or was built from
+    other sources than those in csrc/)."""
     global _lib
     if _lib is not None:
         return _lib
@@ -78,6 +105,11 @@ def load():
     # copy loaded before torch leaves torch without a visible GPU)
     import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
+    lib.gdtp_build_id.restype = c_char_p
+    built = lib.gdtp_build_id().decode()
+    if built != source_hash():
+        raise ImportError(f"{LIB_PATH} was built from other sources ({built}, sources now {source_hash()}): "
+                          "run `python -m gradientdomaintextureprocessing_amd.build` again")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

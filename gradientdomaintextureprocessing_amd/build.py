@@ -39,10 +39,14 @@ def build(verbose=False):
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         jobs.append([clang] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+    sys.path.insert(0, os.path.dirname(HERE))
+    from gradientdomaintextureprocessing_amd._lib import source_hash
+    build_id = ['-DGDTP_BUILD_ID="' + source_hash() + '"']
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        jobs.append([hipcc] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        extra = build_id if src == "gdtp_capi.cpp" else []
+        jobs.append([hipcc] + HIP_FLAGS + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:

@@ -81,6 +81,11 @@ static int Fail(int code, const std::string& msg) {
 
 const char* gdtp_last_error(void) { return g_error.c_str(); }
 
+#ifndef GDTP_BUILD_ID
+#define GDTP_BUILD_ID "unknown"
+#endif
+const char* gdtp_build_id(void) { return GDTP_BUILD_ID; }
+
 int gdtp_device_count(int* count) {
   if (!count) return Fail(GDTP_ERROR_INVALID, "null count");
   int n = 0;

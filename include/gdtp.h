@@ -38,6 +38,10 @@ typedef struct gdtp_system gdtp_system;
 typedef struct gdtp_modulation gdtp_modulation;
 
 const char* gdtp_last_error(void);
+/* Identifier of the sources this library was built from (a hash of the package's csrc/ files);
+ * the Python binding refuses a library whose identifier differs from the sources beside it.  No
+ * counterpart in the reference (header-only, compiled into each app). */
+const char* gdtp_build_id(void);
 int gdtp_device_count(int* count);
 
 /* ---- meshes with per-corner texture coordinates (reference: TexturedMesh,
@@ -143,7 +147,8 @@ void gdtp_modulation_destroy(gdtp_modulation* m);
 /* ---- solves to a relative residual (every column: ||b - A x|| <= tol ||b||)
  * mode GDTP_SOLVE_FP64_REFERENCE: reference V-cycles with an FP64 residual check per cycle.
  * mode GDTP_SOLVE_MIXED: FP64 iterative refinement around FP32 V-cycles (four-colour deep
- *   Gauss-Seidel on dense chart planes, damped-Jacobi boundary rows, exact FP64 coarsest solve),
+ *   Gauss-Seidel on dense chart planes, restricted additive Schwarz on the boundary rows, exact FP64
+ *   coarsest solve),
  *   on the system's own hierarchy; no counterpart in the reference.
  * x holds the initial guess and receives the solution; rel_res (host, ncols) and iterations
  * (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached first. */

@@ -14,9 +14,10 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session", autouse=True)
 def _built_libraries():
-    """Build libgdtp.so / liboracle.so only when they are missing (the tree ships them prebuilt)."""
+    """Build libgdtp.so / liboracle.so when they are missing or libgdtp.so was built from other
+    sources than those in the tree (the tree normally ships them prebuilt and current)."""
     from gradientdomaintextureprocessing_amd import _lib
-    if not os.path.exists(_lib.LIB_PATH):
+    if not _lib.is_current():
         from gradientdomaintextureprocessing_amd import build
         build.build()
     import oracle

@@ -114,3 +114,31 @@ def test_vcycle_converges_to_direct_solution(case):
     e = x - xd
     en = np.sqrt(np.einsum("ij,ij->", e, (A @ e.T).T) / np.einsum("ij,ij->", xd, (A @ xd.T).T))
     assert en < 1e-8
+
+
+def test_apply_is_full_matrix_product(case):
+    """oracle.apply (MultiplyBySystemMatrix_NoReciprocals, IterativeSolvers.inl:881-950) equals the
+    product with the sparse matrix assembled row by row from the same stencils and boundary rows,
+    for A on every level and for M and S on level 0."""
+    from helpers import mass_matrix, stiffness_matrix
+    H, S, L = case
+    rng = np.random.default_rng(11)
+    for l in range(H.levels):
+        x = rng.standard_normal((3, L[l].N))
+        want = (oracle.full_matrix(L[l]) @ x.T).T
+        got = oracle.apply(L[l], L[l].coef_raw, L[l].bd, L[l].bb, x)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    x = rng.standard_normal((2, L[0].N))
+    for name, build in (("mass", mass_matrix), ("stiffness", stiffness_matrix)):
+        bd = oracle.Csr(H.array(0, name + ".bd.rowptr"), H.array(0, name + ".bd.col"), H.array(0, name + ".bd.val", np.float64), L[0].N)
+        bb = oracle.Csr(H.array(0, name + ".bb.rowptr"), H.array(0, name + ".bb.col"), H.array(0, name + ".bb.val", np.float64), L[0].B)
+        got = oracle.apply(L[0], H.array(0, name + ".deep", np.float64), bd, bb, x)
+        want = (build(H, L[0]) @ x.T).T
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    # a planted mistake (a corner and an edge slot exchanged) must fail the comparison
+    bad = L[0].coef_raw.reshape(-1, 10).copy()
+    bad[:, [0, 1]] = bad[:, [1, 0]]
+    x = rng.standard_normal((1, L[0].N))
+    got = oracle.apply(L[0], bad.ravel(), L[0].bd, L[0].bb, x)
+    want = (oracle.full_matrix(L[0]) @ x.T).T
+    assert np.abs(got - want).max() > 1e-6 * np.abs(want).max()
