@@ -71,13 +71,15 @@ def solve(S, x, b, tol, max_cycles, mode):
 
 
 def mixed_roofline(S, H, K, reps=10):
-    """Average duration of one colour-plane launch of the four-colour deep Gauss-Seidel kernel
-    (dominant kernel of the mixed path) on level 0, and its algorithmic bytes: per updated deep
-    texel its stencil (the eight off-diagonal ratios in FP16, c4 and their rounding sum in FP32:
-    24 bytes), rhs read and e written (K columns); plus one read of the other three colour planes
-    of e at active texels."""
+    """Average duration of one launch of the deep Gauss-Seidel sweep (dominant kernel of the mixed
+    path) on level 0, and its algorithmic bytes.  With a fused tile shape one launch is a whole
+    four-colour sweep (KMixDeepGsFused, for K <= 4); with shape -1 it is one colour plane
+    (KMixDeepGs).  B_relax = s*q + 3*K*p per updated deep texel (stencil once, e read, rhs read,
+    e written) with the declared FP16-ratio stencil s*q = 8*2 + 2*4 = 24 bytes and p = 4."""
     import torch
     info = H.level_info(0)
+    shape = S.tile_shape()
+    fused = shape >= 0
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     S.mixed_deep_sweep(0, K)
@@ -87,18 +89,14 @@ def mixed_roofline(S, H, K, reps=10):
         S.mixed_deep_sweep(0, K)
     e1.record(st)
     torch.cuda.synchronize()
-    launches = 4 * reps
+    launches = (K + 3) // 4 * reps if fused else 4 * reps
     per_launch_s = e0.elapsed_time(e1) / 1e3 / launches
-    # B_relax = s*q + 3*K*p per updated deep texel (stencil once, e read, rhs read, e written),
-    # with the declared FP16-ratio stencil s*q = 8*2 + 2*4 = 24 bytes and p = 4; one launch
-    # updates one colour plane, a quarter of the deep texels.
-    bytes_per_launch = info["deep"] / 4 * (24 + 3 * K * 4)
-    # what the kernel itself must touch: the other three colour planes of e read at each launch
-    kernel_bytes = info["deep"] / 4 * (24 + 2 * K * 4) + 0.75 * info["texels"] * K * 4
+    updated = info["deep"] if fused else info["deep"] / 4  # deep texels one launch updates
+    bytes_per_launch = updated * (24 + 3 * K * 4)
     achieved = bytes_per_launch / per_launch_s
     return dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK, traffic=None,
-                kernel="KMixDeepGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch,
-                kernel_bytes_per_launch=kernel_bytes)
+                kernel="KMixDeepGsFused" if fused else "KMixDeepGs", tile_shape=S.TILE_SHAPES[shape],
+                avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)
 
 
 def band_roofline(S, H, x, b, r, reps=5):
@@ -122,6 +120,26 @@ def band_roofline(S, H, x, b, r, reps=5):
     achieved = bytes_per_launch / per_launch_s
     return dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK, traffic=None,
                 kernel="KBandGs", avg_launch_us=per_launch_s * 1e6, algorithmic_bytes_per_launch=bytes_per_launch)
+
+
+def error_vs_direct(S, x, b):
+    """Relative energy-norm error of the solution x against the FP64 direct-equivalent solution of
+    the same system (gdtp_direct_solve: at this size FP64 reference-schedule V-cycles from zero to
+    the FP64 residual floor; a sparse factorisation of the 12 M-texel level does not fit the run);
+    both relative residuals recomputed in FP64 (gdtp_residual_norms)."""
+    import torch
+    import gradientdomaintextureprocessing_amd as g
+    ref = torch.empty_like(b)
+    rel_ref, method = S.direct_solve(ref, b)
+    rel_x, _, _ = S.residual_norms(x, b)
+    e = x - ref
+    ae, aref = torch.empty_like(e), torch.empty_like(ref)
+    S.apply(g.OPERATOR_SYSTEM, e, ae)
+    S.apply(g.OPERATOR_SYSTEM, ref, aref)
+    err = torch.sqrt((e * ae).sum(dim=1) / (ref * aref).sum(dim=1)).cpu().numpy()
+    return dict(energy_rel_max=float(err.max()), energy_rel=[float(v) for v in err], rel_res=[float(v) for v in rel_x],
+                reference=("LDL^T of the full level-0 matrix" if method == 0 else "FP64 reference-schedule V-cycles to the FP64 floor")
+                + f", rel-res {float(rel_ref.max()):.2e}")
 
 
 COPY_BYTES = 1 << 28
@@ -259,15 +277,37 @@ def load_traffic(kernel, workload_key):
     return rec["bytes_per_launch"]
 
 
-def cpu_baseline(H, S, K, tol, budget_s=60.0):
-    """Oracle (plain-C restatement of the reference V-cycle + exact sparse solves) on one core,
-    one solve of the same workload from x = 0, bounded by budget_s."""
+def host_cpu():
+    """Host CPU model, sockets, cores and threads (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return dict(model=info.get("Model name"), sockets=info.get("Socket(s)"), cores_per_socket=info.get("Core(s) per socket"),
+                threads_per_core=info.get("Thread(s) per core"), cpus=info.get("CPU(s)"))
+
+
+def cpu_baseline(H, S, K, tol, runs=5):
+    """Oracle (plain-C restatement of the reference V-cycle, OpenMP over same-colour band tasks and
+    rows, exact sparse solves per column in parallel) on the host's cores: one solve of the same
+    workload from x = 0 to `tol` counts the V-cycles n (and warms up), then `runs` single V-cycles
+    are timed with the wall clock; value = K / (n x median V-cycle time)."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    import scipy.sparse.linalg as spla  # noqa: F401
+    oracle.set_threads(threads)
     L = [oracle.level_from_product(H, S, l) for l in range(H.levels)]
+    t0 = time.perf_counter()
     solv = oracle.Solvers(L, oracle.full_matrix(L[-1]))
     A = oracle.full_matrix(L[0])
+    setup = time.perf_counter() - t0
     tex = np.random.default_rng(1).random((H.height, H.width, K), dtype=np.float32)
     ci, cj, _ = H.texel_coordinates(0)
     f = np.ascontiguousarray(tex[cj, ci, :].T.astype(np.float64))
@@ -276,22 +316,22 @@ def cpu_baseline(H, S, K, tol, budget_s=60.0):
     b = oracle.apply(L[0], H.array(0, "mass.deep", np.float64), bd, bb, f)
     x = np.zeros_like(b)
     bn = np.linalg.norm(b, axis=1)
-    t0 = time.time()
     cycles, rel = 0, 1.0
-    while time.time() - t0 < budget_s:
+    while rel > tol and cycles < 40:
         oracle.vcycle(L, solv, x, b)
         cycles += 1
         rel = float((np.linalg.norm(b - (A @ x.T).T, axis=1) / bn).max())
-        if rel <= tol:
-            break
-    el = time.time() - t0
-    if rel > tol:  # extrapolate the remaining cycles from the observed reduction rate
-        per = el / cycles
-        rate = rel ** (1.0 / cycles)
-        extra = np.log(tol / rel) / np.log(rate)
-        el = el + per * extra
-    return dict(value=K / el, unit="solves/s", cores=1, kind="port",
-                sample=f"one {K}-column solve of the same system to rel-res {tol:g} ({cycles} oracle V-cycles, single thread)")
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.vcycle(L, solv, x, b)
+        times.append(time.perf_counter() - t0)
+    per = float(np.median(times))
+    return dict(value=K / (cycles * per), unit="solves/s", cores=threads, kind="port",
+                sample=f"oracle V-cycle on the same {K}-column system: {cycles} cycles reach rel-res {rel:.1e}; "
+                       f"median of {runs} timed V-cycles {per:.3f} s (all {threads} threads, OMP_PROC_BIND="
+                       f"{os.environ['OMP_PROC_BIND']}, OMP_PLACES={os.environ['OMP_PLACES']}); setup {setup:.1f} s not counted",
+                vcycle_s=[float(t) for t in times], host=host_cpu())
 
 
 def geodesics_bench(steps, warmup, tol, sources=64, seed=7):
@@ -454,6 +494,8 @@ def main():
         log(f"[bench] {mode}: {args.steps} steps in {el:.3f}s ({res['value']:.1f} solves/s), iterations/step {cyc}, "
             f"final rel res {rel:.2e}")
         if mode == "mixed":
+            res["error_vs_direct"] = error_vs_direct(S, x, b)
+            log(f"[bench] mixed error vs FP64 direct-equivalent: {res['error_vs_direct']}")
             res["roofline"] = mixed_roofline(S, H, K)
         else:
             res["roofline"] = band_roofline(S, H, x, b, r)
@@ -474,7 +516,7 @@ def main():
         "metric": "solves/s at rel-residual 1e-6 (4096² atlas) vs FP64 GPU; p50/p99 req latency",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 relax / f64 refine" if main_mode == "mixed" else "f64", "data": "synthetic",
+        "dtype": "f32 relax (f16 stencil ratios) / f64 refine" if main_mode == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {CONFIGS[args.config][0]} {CONFIGS[args.config][2]}^2 atlas, "
                                f"{info['texels']} texels, 3 RHS, (M+1e-3 S)x=Mf from x=0 to rel-res {args.tol:g}",
                    "mode": main_mode, "levels": H.levels, "iterations_per_step": cyc},
@@ -482,6 +524,7 @@ def main():
         "fp64_reference_gpu": {k: v for k, v in fp.items() if k in ("value", "ms_per_step", "iterations", "levels")} if fp else None,
         "speedup_vs_fp64_gpu": value / fp["value"] if fp else None,
         "roofline": roof, "cpu_baseline": cpu,
+        "error_vs_direct": res.get("error_vs_direct"),
     }
     print(json.dumps(out))
 

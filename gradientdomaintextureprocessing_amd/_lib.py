@@ -56,6 +56,8 @@ SIGNATURES = {
     "gdtp_coarse_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "gdtp_residual_norms": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "gdtp_direct_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_solve_weighted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_geodesic_divergence": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
@@ -70,6 +72,7 @@ SIGNATURES = {
     "gdtp_modulation_destroy": (None, [c_void_p]),
     "gdtp_mixed_acceleration": (c_int, [c_void_p, c_int]),
     "gdtp_mixed_deep_sweep": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "gdtp_mixed_tile_shape": (c_int, [c_void_p, c_int, P(c_int)]),
     "gdtp_system_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),
 }
 

@@ -319,6 +319,24 @@ class MultigridSystem:
         (px, pb, pr), K = self._pair(x, b, self._texels[0], r)
         _lib.check(_lib.load().gdtp_residual(self._h, px, pb, pr, K, _stream()))
 
+    def residual_norms(self, x, b):
+        """(rel_res, residual norms, rhs norms) per column: ||b - A x|| / ||b|| in FP64 on level 0
+        (ComputeSystemResdiual, IterativeSolvers.inl:953-1007), without writing the residual."""
+        (px, pb), K = self._pair(x, b, self._texels[0])
+        rel, norms = np.zeros(K), np.zeros(2 * K)
+        _lib.check(_lib.load().gdtp_residual_norms(self._h, px, pb, K, rel.ctypes.data, norms.ctypes.data, _stream()))
+        return rel, norms[0::2].copy(), norms[1::2].copy()
+
+    def direct_solve(self, x, b):
+        """Exact FP64 solve of A x = b on level 0 (the apps' --useDirectSolver path): LDL^T of the
+        full matrix up to 2^20 texels, else FP64 V-cycles to the FP64 residual floor.  Returns
+        (rel_res per column, method: 0 factorisation / 1 V-cycles)."""
+        (px, pb), K = self._pair(x, b, self._texels[0])
+        rel = np.zeros(K)
+        m = ctypes.c_int(-1)
+        _lib.check(_lib.load().gdtp_direct_solve(self._h, px, pb, K, rel.ctypes.data, ctypes.byref(m), _stream()))
+        return rel, m.value
+
     def geodesic_divergence(self, potential, rhs):
         """rhs = integral < -grad u / |grad u|_g , grad phi_t >_g per column of `potential`."""
         (pu, pr), K = self._pair(potential, rhs, self._texels[0])
@@ -400,6 +418,37 @@ class MultigridSystem:
 
     def mixed_deep_sweep(self, level, ncols):
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
+
+    # fused deep-sweep tile shapes (sub-grid texels): index -> (width, height); -1 = one launch per colour
+    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), -1: (64, 4)}
+
+    def tile_shape(self, shape=None):
+        """Set (0..3 fused, -1 one launch per colour) and/or return the deep-sweep tile shape."""
+        cur = ctypes.c_int(0)
+        _lib.check(_lib.load().gdtp_mixed_tile_shape(self._h, -2 if shape is None else int(shape), ctypes.byref(cur)))
+        return cur.value
+
+    def autotune_tiles(self, level=0, columns=None, sweeps=20, shapes=(0, 1, 2, 3, -1)):
+        """Time `sweeps` deep sweeps of `level` for each tile shape and keep the fastest (every shape
+        gives the same iterate).  Returns {shape: microseconds per sweep}, fastest first."""
+        import torch
+        K = columns or self.max_cols
+        st = torch.cuda.current_stream()
+        times = {}
+        for sh in shapes:
+            self.tile_shape(sh)
+            self.mixed_deep_sweep(level, K)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(sweeps):
+                self.mixed_deep_sweep(level, K)
+            e1.record(st)
+            torch.cuda.synchronize()
+            times[sh] = e0.elapsed_time(e1) * 1e3 / sweeps
+        best = min(times, key=times.get)
+        self.tile_shape(best)
+        return dict(sorted(times.items(), key=lambda kv: kv[1]))
 
     def memory(self):
         r, m = ctypes.c_size_t(), ctypes.c_size_t()

@@ -43,6 +43,17 @@ struct gdtp_modulation {
   int N = 0, D = 0, B = 0;
 };
 
+// LDL^T factor of the full level-0 matrix for gdtp_direct_solve (host factor, GPU triangular solves).
+struct DirectFactor {
+  Ldlt F;
+  DevLdlt d;
+  double* y = nullptr;  // [kcap][N] solve scratch
+  std::vector<void*> allocations;
+  ~DirectFactor() {
+    for (void* p : allocations) (void)hipFree(p);
+  }
+};
+
 struct gdtp_system {
   const gdtp_hierarchy* H = nullptr;
   double screen = 1, stiff = 0;
@@ -51,12 +62,15 @@ struct gdtp_system {
   std::unique_ptr<DeviceSystem> ds;
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0;
+  std::unique_ptr<DirectFactor> direct;  // built by the first gdtp_direct_solve after create / update
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = 0;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
+  double* dNormParts = nullptr;  // residual-norm partial sums, NormPartsSize(kcap) + 2 kcap doubles
   ~gdtp_system() {
     if (dNorms) (void)hipFree(dNorms);
+    if (dNormParts) (void)hipFree(dNormParts);
     if (hNorms) (void)hipHostFree(hNorms);
   }
 };
@@ -263,8 +277,6 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
   else if (name == "tasks.blockPadEnd") v = L.tasks.blockPadEnd;
   else if (name == "tasks.linePtr") v = L.tasks.linePtr;
   else if (name == "tasks.segments") { for (auto& s : L.tasks.segments) v.insert(v.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart}); }
-  else if (name == "boundaryDeepIndices") { for (auto& x : L.boundaryDeepIndices) v.insert(v.end(), {x.boundary, x.deepGlobal, x.deep, x.offset}); }
-  else if (name == "boundaryBoundaryIndices") { for (auto& x : L.boundaryBoundaryIndices) v.insert(v.end(), {x.principal, x.secondary, x.fineDeep, x.offset}); }
   else if (name == "charts") {
     for (auto& c : L.charts)
       v.insert(v.end(), {c.width, c.height, c.cornerCoords[0], c.cornerCoords[1], c.centerOffset[0], c.centerOffset[1], c.texelOffset, c.deepOffset, c.boundaryOffset});
@@ -284,8 +296,7 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
         }
     }
   }
-  else if (CsrI32(L.boundaryRestriction, name, "boundaryRestriction", v) || CsrI32(L.boundaryCoarseFineProlongation, name, "boundaryCoarseFineProlongation", v) ||
-           CsrI32(L.boundaryFineCoarseRestriction, name, "boundaryFineCoarseRestriction", v)) {
+  else if (CsrI32(L.boundaryRestriction, name, "boundaryRestriction", v)) {
   } else if (level == 0 && name == "vf.cellCorners") v = Samples(h).cellCorners;
   else if (level == 0 && name == "vf.genPtr") v = Samples(h).genPtr;
   else if (level == 0 && name == "vf.genTexel") v = Samples(h).genTexel;
@@ -308,9 +319,6 @@ int gdtp_hierarchy_array_f64(const gdtp_hierarchy* h, int level, const char* cna
   const std::string name(cname);
   std::vector<double> v;
   if (name == "boundaryRestriction.val") v = L.boundaryRestriction.val;
-  else if (name == "boundaryCoarseFineProlongation.val") v = L.boundaryCoarseFineProlongation.val;
-  else if (name == "boundaryFineCoarseRestriction.val") v = L.boundaryFineCoarseRestriction.val;
-  else if (name == "boundaryBoundaryWeights") { for (auto& x : L.boundaryBoundaryIndices) v.push_back(x.weight); }
   else if (name == "chartsGeom") { for (auto& c : L.charts) v.insert(v.end(), {c.corner.x, c.corner.y, c.cellW, c.cellH}); }
   else if (level == 0 && name.rfind("vf.", 0) == 0) {
     const VectorFieldSamples& vs = Samples(h);
@@ -345,6 +353,7 @@ static int BuildSystem(gdtp_system* s, double screen, double stiff) {
     UploadSystem(s->H->h, oh, s->H->op, s->kcap, *ds);
   }
   s->mixed.reset();
+  s->direct.reset();
   s->oh = std::move(oh);
   s->ds = std::move(ds);
   s->screen = screen, s->stiff = stiff;
@@ -459,6 +468,81 @@ int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, i
   GDTP_DEVICE_OP(x && b && r, ResidualLevel(ds, 0, x, b, r, ncols, ds.L[0].N, st))
 }
 
+// Host side of the residual norms: squared sums (device, 2 K doubles) -> norms and relative residuals.
+static void FinishNorms(const double* sq, int K, double* relRes, double* norms) {
+  for (int c = 0; c < K; c++) {
+    const double r = std::sqrt(sq[2 * c]), bn = std::sqrt(sq[2 * c + 1]);
+    if (relRes) relRes[c] = r / std::max(bn, 1e-300);
+    if (norms) norms[2 * c] = r, norms[2 * c + 1] = bn;
+  }
+}
+
+// Squared residual / rhs norms of K level-0 columns into the host vector sq (2 K entries), through
+// the system's norm scratch (allocated on first use).
+static void SystemNorms(gdtp_system* s, const double* x, const double* b, int K, std::vector<double>& sq, hipStream_t st) {
+  DeviceSystem& ds = *s->ds;
+  if (!s->dNormParts) GDTP_HIP(hipMalloc(&s->dNormParts, (NormPartsSize(s->kcap) + 2 * (size_t)s->kcap) * sizeof(double)));
+  double* out = s->dNormParts + NormPartsSize(s->kcap);
+  ResidualNorms(ds, x, b, K, ds.L[0].N, s->dNormParts, out, st);
+  sq.assign(2 * (size_t)K, 0.0);
+  GDTP_HIP(hipMemcpyAsync(sq.data(), out, sq.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  GDTP_HIP(hipStreamSynchronize(st));
+}
+
+int gdtp_residual_norms(gdtp_system* s, const double* x, const double* b, int ncols, double* rel_res, double* norms, void* stream) {
+  GDTP_DEVICE_OP(x && b, {
+    std::vector<double> sq;
+    SystemNorms(s, x, b, ncols, sq, st);
+    FinishNorms(sq.data(), ncols, rel_res, norms);
+  })
+}
+
+// Largest level 0 that gdtp_direct_solve factorises (the host factorisation grows as N^1.5).
+static constexpr int kDirectMaxTexels = 1 << 20;
+
+int gdtp_direct_solve(gdtp_system* s, double* x, const double* b, int ncols, double* rel_res, int* method, void* stream) {
+  GDTP_DEVICE_OP(x && b, {
+    const int N = ds.L[0].N;
+    std::vector<double> sq;
+    if (N <= kDirectMaxTexels) {
+      if (!s->direct) {
+        auto f = std::make_unique<DirectFactor>();
+        const LevelOperator& op = s->oh.levels[0];
+        const Csr A = FullLevelMatrix(s->H->h.levels[0], op.deepRaw, op.bb, op.bd);
+        std::string err;
+        if (!FactorLdlt(A, f->F, err)) return Fail(GDTP_ERROR_NUMERIC, "direct factorisation: " + err);
+        UploadLdlt(f->allocations, f->F, f->d, s->kcap);
+        GDTP_HIP(hipMalloc(&f->y, (size_t)s->kcap * N * sizeof(double)));
+        f->allocations.push_back(f->y);
+        s->direct = std::move(f);
+      }
+      LdltSolve(s->direct->d, b, N, x, N, nullptr, s->direct->y, ncols, st);
+      SystemNorms(s, x, b, ncols, sq, st);
+      if (method) *method = 0;
+    } else {
+      // FP64 direct-equivalent: reference V-cycles from zero until the worst column's relative
+      // residual reaches 1e-14 or has not fallen below 0.9 of its best for three cycles
+      GDTP_HIP(hipMemsetAsync(x, 0, (size_t)ncols * N * sizeof(double), st));
+      double best = 1e300;
+      for (int it = 0, stall = 0; it < 300 && stall < 3; it++) {
+        VCycleReference(ds, x, b, ncols, N, st);
+        SystemNorms(s, x, b, ncols, sq, st);
+        double worst = 0.0;
+        for (int c = 0; c < ncols; c++) worst = std::max(worst, std::sqrt(sq[2 * c]) / std::max(std::sqrt(sq[2 * c + 1]), 1e-300));
+        if (worst <= 1e-14) break;
+        if (worst < 0.9 * best) {
+          best = worst;
+          stall = 0;
+        } else {
+          stall++;
+        }
+      }
+      if (method) *method = 1;
+    }
+    FinishNorms(sq.data(), ncols, rel_res, nullptr);
+  })
+}
+
 static MixedSystem& Mixed(gdtp_system* s) {
   if (!s->mixed) {
     std::unique_ptr<MixedSystem> m(new MixedSystem());
@@ -467,8 +551,21 @@ static MixedSystem& Mixed(gdtp_system* s) {
   }
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
-  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles;
+  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles, m.tileShape = s->tileShape;
   return m;
+}
+
+int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
+  if (!s || shape < -2 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-2, 4)");
+  if (shape >= -1 && shape != s->tileShape) {
+    s->tileShape = shape;
+    if (s->mixed) {
+      s->mixed->tileShape = shape;
+      InvalidateGraph(*s->mixed);
+    }
+  }
+  if (current) *current = s->tileShape;
+  return GDTP_OK;
 }
 
 int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySweeps, double omega, int useGraph) {

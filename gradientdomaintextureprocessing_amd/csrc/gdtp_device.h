@@ -98,6 +98,12 @@ void ApplyOperator(DeviceSystem& ds, DevLevel& op, const double* x, double* y, i
 // Squared 2-norms of K columns (stride ld) accumulated into dNorms[0..K) (zeroed first).
 void SquaredNorms(const double* v, int N, int K, int ld, double* dNorms, hipStream_t s);
 void ResidualLevel(DeviceSystem& ds, int level, const double* x, const double* b, double* r, int K, int ld, hipStream_t s);
+// Squared norms of the level-0 residual b - A x and of b for K columns: out[2c] = ||b_c - A x_c||^2,
+// out[2c + 1] = ||b_c||^2 (device, 2K doubles); `parts` is a device scratch of NormPartsSize(K)
+// doubles, every entry of which the call overwrites.
+constexpr int kNormParts = 512;
+constexpr size_t NormPartsSize(int K) { return (size_t)2 * K * kNormParts; }
+void ResidualNorms(DeviceSystem& ds, const double* x, const double* b, int K, int ld, double* parts, double* out, hipStream_t s);
 
 // Normalised-gradient integration samples on the device (cell samples SoA [13][n]).
 struct DevVectorField {

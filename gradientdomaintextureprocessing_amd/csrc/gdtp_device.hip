@@ -921,4 +921,94 @@ void ModulatedStiffness(const DevModulation& m, const double* mask, double* out,
                                                                      m.K);
 }
 
+// ------------------------------------------------------------------ residual norms
+// ||b - A x||^2 and ||b||^2 per column on level 0 without storing the residual: a fixed grid of
+// kNormParts blocks strides over the raster lines (deep rows, raw stencil) and the boundary rows,
+// each block writes one partial per (column, quantity), and a second pass adds the partials of a
+// column in a fixed order, so repeated calls give bitwise identical norms.
+template <int KC>
+__global__ __launch_bounds__(256) void KResidualNormParts(int nLines, const int* __restrict__ lines, const double* __restrict__ coef, int D,
+                                                          int B, const int* __restrict__ bgi, const int* __restrict__ rpd,
+                                                          const int* __restrict__ cold, const double* __restrict__ vald,
+                                                          const int* __restrict__ rpb, const int* __restrict__ colb,
+                                                          const double* __restrict__ valb, const double* __restrict__ x,
+                                                          const double* __restrict__ b, int ld, int c0, double* parts) {
+  double rr[KC], bn[KC];
+#pragma unroll
+  for (int c = 0; c < KC; c++) rr[c] = 0.0, bn[c] = 0.0;
+  for (int l = blockIdx.x; l < nLines; l += gridDim.x) {
+    const int* L = lines + 5 * l;
+    const int start = L[0], len = L[1] - L[0] + 1, prev = L[2], next = L[3], d0 = L[4];
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      double cf[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) cf[k] = coef[(size_t)k * D + d0 + i];
+#pragma unroll
+      for (int c = 0; c < KC; c++) {
+        const double* X = x + (size_t)(c0 + c) * ld;
+        const double s = X[prev + i - 1] * cf[0] + X[prev + i] * cf[1] + X[prev + i + 1] * cf[2] + X[start + i - 1] * cf[3] +
+                         X[start + i] * cf[4] + X[start + i + 1] * cf[5] + X[next + i - 1] * cf[6] + X[next + i] * cf[7] +
+                         X[next + i + 1] * cf[8];
+        const double t = b[(size_t)(c0 + c) * ld + start + i], r = t - s;
+        rr[c] += r * r, bn[c] += t * t;
+      }
+    }
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < B; i += gridDim.x * blockDim.x) {
+    const int g = bgi[i];
+#pragma unroll
+    for (int c = 0; c < KC; c++) {
+      const double* X = x + (size_t)(c0 + c) * ld;
+      double t = 0.0, s = 0.0;
+      for (int k = rpb[i]; k < rpb[i + 1]; k++) t += X[bgi[colb[k]]] * valb[k];
+      for (int k = rpd[i]; k < rpd[i + 1]; k++) s += X[cold[k]] * vald[k];
+      const double bv = b[(size_t)(c0 + c) * ld + g], r = bv - (t + s);
+      rr[c] += r * r, bn[c] += bv * bv;
+    }
+  }
+  __shared__ double part[2 * KC][4];
+#pragma unroll
+  for (int c = 0; c < KC; c++) {
+    double a = rr[c], e = bn[c];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64), e += __shfl_xor(e, o, 64);
+    if (threadIdx.x % 64 == 0) part[2 * c][threadIdx.x / 64] = a, part[2 * c + 1][threadIdx.x / 64] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * KC) {
+    const int c = threadIdx.x / 2, w = threadIdx.x % 2;
+    const double* p = part[threadIdx.x];
+    parts[((size_t)(c0 + c) * 2 + w) * gridDim.x + blockIdx.x] = (p[0] + p[1]) + (p[2] + p[3]);
+  }
+}
+
+// out[q] = sum of the nParts partials of row q (q = 2 c + w), in a fixed order.
+__global__ __launch_bounds__(256) void KNormPartsSum(const double* __restrict__ parts, int nParts, double* out) {
+  const int q = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nParts; i += blockDim.x) a += parts[(size_t)q * nParts + i];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  __shared__ double w[4];
+  if (threadIdx.x % 64 == 0) w[threadIdx.x / 64] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) out[q] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+void ResidualNorms(DeviceSystem& ds, const double* x, const double* b, int K, int ld, double* parts, double* out, hipStream_t s) {
+  DevLevel& d = ds.L[0];
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = std::min(4, K - c0);
+#define GDTP_NORM_PARTS(KC)                                                                                                         \
+  KResidualNormParts<KC><<<kNormParts, 256, 0, s>>>(d.nRaster, d.raster, d.coefRaw, d.D, d.B, d.bgi, d.bd.rowptr, d.bd.col, d.bd.val, \
+                                                    d.bb.rowptr, d.bb.col, d.bb.val, x, b, ld, c0, parts)
+    switch (kc) {
+      case 4: GDTP_NORM_PARTS(4); break;
+      case 3: GDTP_NORM_PARTS(3); break;
+      case 2: GDTP_NORM_PARTS(2); break;
+      default: GDTP_NORM_PARTS(1); break;
+    }
+#undef GDTP_NORM_PARTS
+  }
+  KNormPartsSum<<<2 * K, 256, 0, s>>>(parts, kNormParts, out);
+}
+
 }  // namespace gdtp

@@ -44,6 +44,83 @@ void RestrictDeepCoefficients(const std::vector<double>& fine, std::vector<doubl
   }
 }
 
+// Coarse parents of fine texel f with their bilinear weights (row f of the prolongation): per
+// axis, the coarse texel at floor(u / 2) and, for odd u, also the next one, each with weight 1/2,
+// u being f's chart-grid offset from the chart centre (the level grids are aligned at chart
+// centres, ProlongationAndRestriction.inl:99-172).  Returns the number of parents.
+static int ProlongRow(const Level& F, const Level& C, int f, int (&col)[4], double (&w)[4]) {
+  const NodeInfo& ni = F.nodeInfo[f];
+  const GridChart &fc = F.charts[ni.chart], &cc = C.charts[ni.chart];
+  const int ui = ni.ci - fc.centerOffset[0], uj = ni.cj - fc.centerOffset[1];
+  auto floorHalf = [](int u) { return u >= 0 ? u / 2 : -((1 - u) / 2); };
+  const int ci = floorHalf(ui) + cc.centerOffset[0], cj = floorHalf(uj) + cc.centerOffset[1];
+  const double wt = ((ui & 1) ? 0.5 : 1.0) * ((uj & 1) ? 0.5 : 1.0);
+  int n = 0;
+  for (int dj = 0; dj <= (uj & 1); dj++)
+    for (int di = 0; di <= (ui & 1); di++) col[n] = cc.texel(ci + di, cj + dj), w[n] = wt, n++;
+  return n;
+}
+
+// Calls visit(g, a) for the entries a = A[f][g] of row f of a level's operator: the nine-point
+// stencil of a deep texel, or the A_BD and A_BB rows of a boundary texel.
+template <class Visit>
+static void OperatorRow(const Level& L, const LevelOperator& o, int f, Visit&& visit) {
+  const int bd = L.boundaryAndDeepIndex[f];
+  if (bd < 0) {
+    const int d = -bd - 1;
+    const NodeInfo& ni = L.nodeInfo[f];
+    const GridChart& c = L.charts[ni.chart];
+    for (int k = 0; k < 9; k++) visit(c.texel(ni.ci + k % 3 - 1, ni.cj + k / 3 - 1), o.deepRaw[10 * (size_t)d + k]);
+  } else {
+    const int b = bd - 1;
+    for (int k = o.bd.rowptr[b]; k < o.bd.rowptr[b + 1]; k++) visit(o.bd.col[k], o.bd.val[k]);
+    for (int k = o.bb.rowptr[b]; k < o.bb.rowptr[b + 1]; k++) visit(L.boundaryGlobalIndex[o.bb.col[k]], o.bb.val[k]);
+  }
+}
+
+// Boundary rows of the coarse Galerkin operator P^T A P (UpdateCoefficients.inl:89-135 forms the
+// same rows from index lists): row i of the coarse boundary is (R_B A) P with R_B = rows of P^T at
+// the coarse boundary texels (the fine level's boundaryRestriction), evaluated row by row over
+// the few fine texels each row touches.  Coarse columns that are deep go to A_BD (global
+// columns), boundary ones to A_BB (boundary columns).
+static void CoarseBoundaryRows(const Level& F, const LevelOperator& fo, const Level& C, Csr& bd, Csr& bb) {
+  const int B = C.numBoundary;
+  std::vector<std::vector<std::pair<int, double>>> rows(B);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < B; i++) {
+    std::vector<std::pair<int, double>>& acc = rows[i];
+    auto add = [&](int c, double v) {
+      for (auto& e : acc)
+        if (e.first == c) {
+          e.second += v;
+          return;
+        }
+      acc.push_back({c, v});
+    };
+    const Csr& R = F.boundaryRestriction;
+    for (int k = R.rowptr[i]; k < R.rowptr[i + 1]; k++) {
+      const int f = R.col[k];
+      const double rw = R.val[k];
+      OperatorRow(F, fo, f, [&](int g, double a) {
+        int pc[4];
+        double pw[4];
+        const int n = ProlongRow(F, C, g, pc, pw);
+        for (int q = 0; q < n; q++) add(pc[q], rw * a * pw[q]);
+      });
+    }
+    std::sort(acc.begin(), acc.end());
+  }
+  std::vector<Triplet> td, tb;
+  for (int i = 0; i < B; i++)
+    for (const auto& e : rows[i]) {
+      const int x = C.boundaryAndDeepIndex[e.first];
+      if (x < 0) td.push_back({i, e.first, e.second});
+      else tb.push_back({i, x - 1, e.second});
+    }
+  bd = Csr::FromTriplets(B, C.numTexels, td);
+  bb = Csr::FromTriplets(B, B, tb);
+}
+
 Csr FullLevelMatrix(const Level& L, const std::vector<double>& deepRaw, const Csr& bb, const Csr& bd) {
   std::vector<Triplet> t;
   t.reserve((size_t)L.numDeep * 9 + bb.col.size() + bd.col.size() + L.numTexels);
@@ -88,19 +165,7 @@ int BuildOperatorHierarchy(const Hierarchy& h, const FineOperator& op, double sc
     }
     co.deepRaw.assign(10 * (size_t)C.numDeep, 0.0);
     RestrictDeepCoefficients(fo.deepRaw, co.deepRaw, C.deepLines);
-    // boundary-deep rows from the coarse deep stencils by symmetry (UpdateCoefficients.inl:89-101)
-    std::vector<Triplet> t;
-    t.reserve(C.boundaryDeepIndices.size());
-    for (const auto& x : C.boundaryDeepIndices) t.push_back({x.boundary, x.deepGlobal, co.deepRaw[10 * (size_t)x.deep + x.offset]});
-    co.bd = Csr::FromTriplets(C.numBoundary, C.numTexels, t);
-    // boundary-boundary: R A_bb P + couplings through fine deep texels (UpdateCoefficients.inl:104-135)
-    Csr partial = h.levels[l - 1].boundaryFineCoarseRestriction.Multiply(fo.bb.Multiply(C.boundaryCoarseFineProlongation));
-    t.clear();
-    t.reserve(C.boundaryBoundaryIndices.size());
-    for (const auto& x : C.boundaryBoundaryIndices)
-      t.push_back({x.principal, x.secondary, fo.deepRaw[10 * (size_t)x.fineDeep + x.offset] * x.weight});
-    Csr T = Csr::FromTriplets(C.numBoundary, C.numBoundary, t);
-    co.bb = T.Combine(1.0, partial, 1.0);
+    CoarseBoundaryRows(h.levels[l - 1], fo, C, co.bd, co.bb);
   }
   oh.coarseMatrix = FullLevelMatrix(h.levels[nl - 1], oh.levels[nl - 1].deepRaw, oh.levels[nl - 1].bb, oh.levels[nl - 1].bd);
   // reciprocal form of the smoothed levels (UpdateCoefficients.inl:374-397)

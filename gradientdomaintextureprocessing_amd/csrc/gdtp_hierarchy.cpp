@@ -553,71 +553,6 @@ static int BuildProlongation(Level& F, const Level& C, std::string& err) {
   return 1;
 }
 
-// InitializeAtlasHierachicalBoundaryCoefficients (HierarchyConstruction.inl:568-706)
-static int BuildBoundaryCoefficients(const Level& F, Level& C, std::string& err) {
-  std::vector<Triplet> trip;
-  C.boundaryDeepIndices.clear();
-  C.boundaryBoundaryIndices.clear();
-  for (int i = 0; i < C.numBoundary; i++) {
-    const NodeInfo& nd = C.nodeInfo[C.boundaryGlobalIndex[i]];
-    const GridChart &cc = C.charts[nd.chart], &fc = F.charts[nd.chart];
-    int nb = 0, bid[9], boi[9], boj[9];
-    for (int li = -1; li < 2; li++)
-      for (int lj = -1; lj < 2; lj++) {
-        int pi = nd.ci + li, pj = nd.cj + lj;
-        if (pi < 0 || pi >= cc.width || pj < 0 || pj >= cc.height) continue;
-        int cg = cc.texel(pi, pj);
-        if (cg == -1) continue;
-        int bd = C.boundaryAndDeepIndex[cg];
-        if (bd < 0) C.boundaryDeepIndices.push_back({i, cg, -bd - 1, (1 - lj) * 3 + (1 - li)});
-        else bid[nb] = bd - 1, boi[nb] = 2 * li, boj[nb] = 2 * lj, nb++;
-      }
-    int fi = fc.centerOffset[0] + 2 * (nd.ci - cc.centerOffset[0]);
-    int fj = fc.centerOffset[1] + 2 * (nd.cj - cc.centerOffset[1]);
-    for (int di = -1; di < 2; di++)
-      for (int dj = -1; dj < 2; dj++) {
-        int pi = fi + di, pj = fj + dj;
-        if (pi < 0 || pi >= fc.width || pj < 0 || pj >= fc.height) continue;
-        int fg = fc.texel(pi, pj);
-        if (fg == -1) continue;
-        int fb = F.boundaryAndDeepIndex[fg];
-        double wt = (1.0 / (1.0 + std::fabs(double(di)))) * (1.0 / (1.0 + std::fabs(double(dj))));
-        if (fb > 0) {
-          trip.push_back({fb - 1, i, wt});
-          for (int ki = -1; ki < 2; ki++)
-            for (int kj = -1; kj < 2; kj++) {
-              int qi = pi + ki, qj = pj + kj;
-              if (qi < 0 || qi >= fc.width || qj < 0 || qj >= fc.height) continue;
-              int ng = fc.texel(qi, qj);
-              if (ng == -1) continue;
-              int nbd = F.boundaryAndDeepIndex[ng];
-              if (nbd >= 0) continue;
-              int oi = di + ki, oj = dj + kj;
-              for (int n = 0; n < nb; n++) {
-                double dI = std::fabs(double(oi - boi[n])), dJ = std::fabs(double(oj - boj[n]));
-                if (dI < 1.5 && dJ < 1.5)
-                  C.boundaryBoundaryIndices.push_back({i, bid[n], -nbd - 1, (1 - kj) * 3 + (1 - ki), wt * (1.0 / (1.0 + dI)) * (1.0 / (1.0 + dJ))});
-              }
-            }
-        } else {
-          int fd = -fb - 1;
-          for (int ki = -1; ki < 2; ki++)
-            for (int kj = -1; kj < 2; kj++) {
-              int oi = di + ki, oj = dj + kj;
-              for (int n = 0; n < nb; n++) {
-                double dI = std::fabs(double(oi - boi[n])), dJ = std::fabs(double(oj - boj[n]));
-                if (dI < 1.5 && dJ < 1.5)
-                  C.boundaryBoundaryIndices.push_back({i, bid[n], fd, (1 + kj) * 3 + (1 + ki), wt * (1.0 / (1.0 + dI)) * (1.0 / (1.0 + dJ))});
-              }
-            }
-        }
-      }
-  }
-  C.boundaryCoarseFineProlongation = Csr::FromTriplets(F.numBoundary, C.numBoundary, trip);
-  (void)err;
-  return 1;
-}
-
 int BuildHierarchy(const TexturedMesh& mesh, int W, int H, int numLevels, const BlockInfo& bi, Hierarchy& h, std::string& err) {
   if (numLevels < 1) { err = "need at least one level"; return 0; }
   h.width = W, h.height = H;
@@ -630,10 +565,6 @@ int BuildHierarchy(const TexturedMesh& mesh, int W, int H, int numLevels, const 
   for (int l = 0; l + 1 < numLevels; l++) {
     if (!BuildRestriction(h.levels[l], h.levels[l + 1], h.levels[l].boundaryRestriction, err)) return 0;
     if (!BuildProlongation(h.levels[l], h.levels[l + 1], err)) return 0;
-  }
-  for (int l = 1; l < numLevels; l++) {
-    if (!BuildBoundaryCoefficients(h.levels[l - 1], h.levels[l], err)) return 0;
-    h.levels[l - 1].boundaryFineCoarseRestriction = h.levels[l].boundaryCoarseFineProlongation.Transpose();
   }
   return 1;
 }

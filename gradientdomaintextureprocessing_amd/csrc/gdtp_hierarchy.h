@@ -75,8 +75,6 @@ struct NodeInfo { int chart, ci, cj, type; };
 struct Line5 { int start, end, prev, next, coeff; };
 struct DeepLine { int coarseStart, coarseEnd, finePrev, fineCurrent, fineNext; };
 struct ProlongationLine { int start, length, center, prev, next, aligned; };
-struct BoundaryDeepIndex { int boundary, deepGlobal, deep, offset; };
-struct BoundaryBoundaryIndex { int principal, secondary, fineDeep, offset; double weight; };
 
 // Flattened band tasks: task t owns blocks [taskPtr[t], taskPtr[t+1]); block b owns deep rows
 // [lineBegin[b], deepEnd[b]) then padded rows [deepEnd[b], padEnd[b]); row r owns segments
@@ -97,10 +95,6 @@ struct Level {
   std::vector<DeepLine> deepLines;          // coarse levels: Galerkin deep coefficient rows
   std::vector<ProlongationLine> prolongationLines;  // fine levels: all active texels by row runs
   Csr boundaryRestriction;                  // fine levels: (coarse boundary) x (fine texels)
-  std::vector<BoundaryDeepIndex> boundaryDeepIndices;            // coarse levels
-  std::vector<BoundaryBoundaryIndex> boundaryBoundaryIndices;    // coarse levels
-  Csr boundaryCoarseFineProlongation;       // coarse levels: (fine boundary) x (coarse boundary)
-  Csr boundaryFineCoarseRestriction;        // fine levels: transpose of the above of level+1
 };
 
 struct BlockInfo { int blockWidth = 128, blockHeight = 16, paddingWidth = 2, paddingHeight = 0; };

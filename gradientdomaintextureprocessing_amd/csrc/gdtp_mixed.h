@@ -9,6 +9,11 @@ struct MixChart {  // per chart geometry of one level
   int w, h, w2, h2, base2, cofx, cofy;
 };
 
+// Tile shapes (sub-grid texels, width x height) of the fused four-colour deep sweep; the system
+// uses one of them (or, with shape -1, one launch per colour on 64 x 4 tiles).
+constexpr int kFusedShapes = 4;
+constexpr int kFusedShape[kFusedShapes][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}};
+
 struct DevMixLevel {
   int N = 0, B = 0, nCharts = 0;
   long plane = 0, Nd = 0;          // dense colour-plane size and total dense size (4 planes)
@@ -24,6 +29,8 @@ struct DevMixLevel {
   int nTiles = 0;
   bool chartSweeps = false;         // small charts: one workgroup per chart runs all four colours of a sweep
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
+  int* ftiles[kFusedShapes] = {};  // the same for each fused-sweep tile shape
+  int nFTiles[kFusedShapes] = {};
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;
   int *bRowptr = nullptr, *bCol = nullptr;
@@ -39,13 +46,15 @@ struct DevMixLevel {
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
   int *rRowptr = nullptr, *rCol = nullptr;
   float* rVal = nullptr;
-  // vectors [K][Nd] and boundary residual scratch [K][B]
-  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr;
+  // vectors [K][Nd] and boundary residual scratch [K][B]; the fused sweep reads e and writes e2
+  // (levels swept by tiles), after which the two pointers swap
+  float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr, *e2 = nullptr;
 };
 
 struct MixedSystem {
   int levels = 0, kcap = 0;
   int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per Schwarz step
+  int tileShape = 0;                              // fused deep-sweep tile shape (kFusedShape), -1: one launch per colour
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level
@@ -104,7 +113,8 @@ void InvalidateGraph(MixedSystem& ms);
 // One FP32 V-cycle of the correction equation on the current level-0 residual; zeroE0: the level-0
 // correction is zero on entry (the first cycle after a residual was written).
 void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool zeroE0);
-// One four-colour deep Gauss-Seidel sweep of `level` on the solver's internal vectors.
-void MixedDeepSweep(MixedSystem& ms, int level, int K, hipStream_t s);
+// One four-colour deep Gauss-Seidel sweep of `level` on the solver's internal vectors (pre order);
+// returns the number of kernel launches it made.
+int MixedDeepSweep(MixedSystem& ms, int level, int K, hipStream_t s);
 
 }  // namespace gdtp

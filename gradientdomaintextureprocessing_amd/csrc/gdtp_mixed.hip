@@ -16,6 +16,7 @@
 #include "gdtp_mixed.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <stdexcept>
 #include <type_traits>
@@ -67,6 +68,46 @@ __device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
 // its stores (few memory round trips per texel: small levels are latency bound).
 __device__ __forceinline__ float Half(unsigned bits) { return (float)__builtin_bit_cast(_Float16, (unsigned short)bits); }
 
+// The Gauss-Seidel value of one texel and column from its eight neighbours v (slot 4 unused):
+// c4 rhs - sum_{k!=4} c_k v_k, with the FP16 stencil's row-sum correction - corr * sum_k v_k.  The
+// fused-multiply-adds are explicit so that every kernel that calls this evaluates it identically.
+template <bool kHalf>
+__device__ __forceinline__ float GsValue(const float (&cf)[9], const float (&v)[9], float c4, float corr, float r) {
+  float sum = 0.0f, m = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    if (k != 4) {
+      sum = __builtin_fmaf(cf[k], v[k], sum);
+      if constexpr (kHalf) m += v[k];
+    }
+  const float e = __builtin_fmaf(c4, r, -sum);
+  if constexpr (kHalf) return __builtin_fmaf(-corr, m, e);
+  return e;
+}
+
+// The smoother's stencil of dense texel idx: c4 (0 off deep texels), the FP16 stencil's row-sum
+// correction and the eight off-diagonal ratios.
+template <bool kHalf>
+__device__ __forceinline__ float GsStencil(long idx, long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+                                           const uint4* __restrict__ sth, float& corr, float (&cf)[9]) {
+  corr = 0.0f;
+  if constexpr (kHalf) {
+    const float2 cd = stc[idx];
+    if (cd.x == 0.0f) return 0.0f;
+    corr = 0.125f * cd.y;
+    const uint4 w = sth[idx];
+    cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
+    cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
+    return cd.x;
+  } else {
+    const float c4 = st[4 * Nd + idx];
+    if (c4 == 0.0f) return 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+    return c4;
+  }
+}
+
 // kHalf: the stencil as FP16 ratios with an FP32 row-sum correction (stc, sth), else the FP32
 // stencil st.
 template <bool kHalf>
@@ -76,20 +117,9 @@ __device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long 
   const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
   if (i >= c.w || j >= c.h) return;
   const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-  float c4, corr = 0.0f, cf[9];
-  if constexpr (kHalf) {
-    const float2 cd = stc[idx];
-    c4 = cd.x, corr = 0.125f * cd.y;
-    if (c4 == 0.0f) return;
-    const uint4 w = sth[idx];
-    cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
-    cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
-  } else {
-    c4 = st[4 * Nd + idx];
-    if (c4 == 0.0f) return;
-#pragma unroll
-    for (int k = 0; k < 9; k++) cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
-  }
+  float corr, cf[9];
+  const float c4 = GsStencil<kHalf>(idx, Nd, st, stc, sth, corr, cf);
+  if (c4 == 0.0f) return;
   int nb[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
@@ -104,21 +134,7 @@ __device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long 
       r[ch] = Ld(rhs + (c0 + ch) * Nd, nb[4]);
     }
 #pragma unroll
-    for (int ch = 0; ch < KC; ch++) {
-      if constexpr (kHalf) {
-        float s = 0.0f, m = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-          if (k != 4) s += cf[k] * v[ch][k], m += v[ch][k];
-        e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s - corr * m;
-      } else {
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-          if (k != 4) s += cf[k] * v[ch][k];
-        e[(long)(c0 + ch) * Nd + idx] = c4 * r[ch] - s;
-      }
-    }
+    for (int ch = 0; ch < KC; ch++) e[(long)(c0 + ch) * Nd + idx] = GsValue<kHalf>(cf, v[ch], c4, corr, r[ch]);
   });
 }
 
@@ -150,6 +166,131 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
   }
 }
 
+// ------------------------------------------------------------------ fused four-colour sweep
+// All four colours of one sweep in one launch.  A workgroup owns a TA x TB tile of the sub-grid
+// (all four planes) and stages the window [-2, TA+2) x [-1, TB+1) around it in LDS: the old
+// values of the planes coloured after the first, then colour by colour the new values.  A colour
+// is computed on its tile and, redundantly, on the ring of neighbouring-tile texels that a later
+// colour of this tile reads (FusedRegion), so no workgroup waits for another.  The sweep reads e
+// from eIn and writes every active texel of its tile to eOut (deep texels: new values; boundary
+// texels: copied): neighbouring workgroups still read the old values from eIn, and the result is
+// bitwise that of the four single-colour launches (same arithmetic, GsValue).  The caller swaps
+// the two buffers afterwards.
+struct FusedBox {
+  int lx, hx, ly, hy;  // region [lx, TA + hx) x [ly, TB + hy) relative to the tile
+};
+// Region of plane q for the colour order 0,1,2,3 (pre-smoothing) or 3,2,1,0 (post): the plane's
+// texels whose new value this tile's later colours read, closed over every later colour.
+__host__ __device__ constexpr FusedBox FusedRegion(bool post, int q) {
+  return post ? (q == 3 ? FusedBox{-2, 1, -1, 0} : q == 2 ? FusedBox{-1, 1, -1, 0} : q == 1 ? FusedBox{-1, 0, 0, 0} : FusedBox{0, 0, 0, 0})
+              : (q == 0 ? FusedBox{-1, 2, 0, 1} : q == 1 ? FusedBox{-1, 1, 0, 1} : q == 2 ? FusedBox{0, 1, 0, 0} : FusedBox{0, 0, 0, 0});
+}
+
+template <bool kHalf, bool kFirst, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY>
+__device__ __forceinline__ void FusedColour(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
+                                            const float2* __restrict__ stc, const uint4* __restrict__ sth,
+                                            const int8_t* __restrict__ type, const float* __restrict__ eIn, float* __restrict__ eOut,
+                                            const float* __restrict__ rhs, float* sm) {
+  constexpr int WX = TA + 4, WN = WX * (TB + 2);
+  constexpr int RW = TA + HX - LX, RN = RW * (TB + HY - LY);
+  for (int t = threadIdx.x; t < RN; t += blockDim.x) {
+    const int x = LX + t % RW, y = LY + t / RW;
+    const int a = a0 + x, b = b0 + y, i = 2 * a + (Q & 1), j = 2 * b + (Q >> 1);
+    const int w = (y + 1) * WX + (x + 2);
+    if (a < 0 || b < 0 || i >= c.w || j >= c.h) {
+      if constexpr (kFirst) {
+#pragma unroll
+        for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = 0.0f;
+      }
+      continue;
+    }
+    const bool own = x >= 0 && x < TA && y >= 0 && y < TB;
+    const long idx = (long)Q * plane + c.base2 + (long)b * c.w2 + a;
+    float corr, cf[9];
+    const float c4 = GsStencil<kHalf>(idx, Nd, st, stc, sth, corr, cf);
+    if (c4 != 0.0f) {
+      float v[KC][9], r[KC];
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        r[ch] = rhs[(long)ch * Nd + idx];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          if (k == 4) {
+            v[ch][k] = 0.0f;
+            continue;
+          }
+          // neighbour (i + di, j + dj): plane p, sub-grid offset (da, db) (floor division by 2)
+          const int ni = (Q & 1) + k % 3 - 1, nj = (Q >> 1) + k / 3 - 1;
+          const int p = (ni & 1) | ((nj & 1) << 1), da = ni >> 1, db = nj >> 1;
+          v[ch][k] = sm[(ch * 4 + p) * WN + w + db * WX + da];
+        }
+      }
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        const float e = GsValue<kHalf>(cf, v[ch], c4, corr, r[ch]);
+        sm[(ch * 4 + Q) * WN + w] = e;
+        if (own) eOut[(long)ch * Nd + idx] = e;
+      }
+    } else {
+      if constexpr (kFirst) {
+#pragma unroll
+        for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = eIn[(long)ch * Nd + idx];
+      }
+      if (own && type[idx] >= 0) {
+#pragma unroll
+        for (int ch = 0; ch < KC; ch++) eOut[(long)ch * Nd + idx] = sm[(ch * 4 + Q) * WN + w];
+      }
+    }
+  }
+}
+
+template <bool kHalf, bool kPost, int TA, int TB, int KC, int S>
+__device__ __forceinline__ void FusedStep(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
+                                          const float2* __restrict__ stc, const uint4* __restrict__ sth, const int8_t* __restrict__ type,
+                                          const float* __restrict__ eIn, float* __restrict__ eOut, const float* __restrict__ rhs,
+                                          float* sm) {
+  constexpr int Q = kPost ? 3 - S : S;
+  constexpr FusedBox R = FusedRegion(kPost, Q);
+  FusedColour<kHalf, S == 0, TA, TB, KC, Q, R.lx, R.hx, R.ly, R.hy>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+}
+
+template <bool kHalf, bool kPost, int TA, int TB, int KC>
+__global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                       long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+                                                       const uint4* __restrict__ sth, const int8_t* __restrict__ type,
+                                                       const float* __restrict__ eIn, float* __restrict__ eOut,
+                                                       const float* __restrict__ rhs) {
+  constexpr int WX = TA + 4, WN = WX * (TB + 2);
+  __shared__ float sm[KC * 4 * WN];
+  const int* T = tiles + 3 * blockIdx.x;
+  const MixChart c = charts[T[0]];
+  const int a0 = T[1], b0 = T[2];
+  // old values of the three planes coloured after the first (pre: 1, 2, 3; post: 2, 1, 0)
+  for (int t = threadIdx.x; t < 3 * WN; t += blockDim.x) {
+    const int q = kPost ? 2 - t / WN : 1 + t / WN, w = t % WN;
+    const int a = a0 + w % WX - 2, b = b0 + w / WX - 1, i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+    float v[KC];
+    if (a >= 0 && b >= 0 && i < c.w && j < c.h) {
+      const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) v[ch] = eIn[(long)ch * Nd + idx];
+    } else {
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) v[ch] = 0.0f;
+    }
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + q) * WN + w] = v[ch];
+  }
+  __syncthreads();
+  FusedStep<kHalf, kPost, TA, TB, KC, 0>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+  __syncthreads();
+  FusedStep<kHalf, kPost, TA, TB, KC, 1>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+  __syncthreads();
+  FusedStep<kHalf, kPost, TA, TB, KC, 2>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+  __syncthreads();
+  FusedStep<kHalf, kPost, TA, TB, KC, 3>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+}
+
 // res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4).  A workgroup
 // covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
 // once rather than once per plane.
@@ -166,7 +307,7 @@ __device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__
     const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
     const float a4 = sym[4 * Nd + idx];
     if (a4 == 0.0f) continue;
-    if (q == zeroQ) {  // relaxed last: its deep rows hold exactly, so the residual is zero
+    if (q == zeroQ) {  // relaxed last by the FP32 smoother: its deep rows hold exactly, so the residual is zero
       for (int ch = 0; ch < K; ch++) res[(long)ch * Nd + idx] = 0.0f;
       continue;
     }
@@ -957,23 +1098,35 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     }
     d.type = Up(ms, type);
     // tiles of the sub-grid that contain at least one active texel (any plane)
-    std::vector<int> tiles;
-    for (int c = 0; c < d.nCharts; c++) {
-      const MixChart& m = d.hcharts[c];
-      for (int b0 = 0; b0 < m.h2; b0 += kTB)
-        for (int a0 = 0; a0 < m.w2; a0 += kTA) {
-          bool any = false;
-          for (int q = 0; q < 4 && !any; q++)
-            for (int bb = b0; bb < std::min(b0 + kTB, m.h2) && !any; bb++)
-              for (int aa = a0; aa < std::min(a0 + kTA, m.w2) && !any; aa++) {
-                int i = 2 * aa + (q & 1), j = 2 * bb + (q >> 1);
-                if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)bb * m.w2 + aa] >= 0) any = true;
-              }
-          if (any) tiles.insert(tiles.end(), {c, a0, b0});
-        }
+    auto buildTiles = [&](int ta, int tb) {
+      std::vector<int> tiles;
+      for (int c = 0; c < d.nCharts; c++) {
+        const MixChart& m = d.hcharts[c];
+        for (int b0 = 0; b0 < m.h2; b0 += tb)
+          for (int a0 = 0; a0 < m.w2; a0 += ta) {
+            bool any = false;
+            for (int q = 0; q < 4 && !any; q++)
+              for (int bb = b0; bb < std::min(b0 + tb, m.h2) && !any; bb++)
+                for (int aa = a0; aa < std::min(a0 + ta, m.w2) && !any; aa++) {
+                  int i = 2 * aa + (q & 1), j = 2 * bb + (q >> 1);
+                  if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)bb * m.w2 + aa] >= 0) any = true;
+                }
+            if (any) tiles.insert(tiles.end(), {c, a0, b0});
+          }
+      }
+      return tiles;
+    };
+    {
+      const std::vector<int> tiles = buildTiles(kTA, kTB);
+      d.nTiles = (int)tiles.size() / 3;
+      d.tiles = Up(ms, tiles);
     }
-    d.nTiles = (int)tiles.size() / 3;
-    d.tiles = Up(ms, tiles);
+    if (l + 1 < nl && !d.chartSweeps)
+      for (int sh = 0; sh < kFusedShapes; sh++) {
+        const std::vector<int> tiles = buildTiles(kFusedShape[sh][0], kFusedShape[sh][1]);
+        d.nFTiles[sh] = (int)tiles.size() / 3;
+        d.ftiles[sh] = Up(ms, tiles);
+      }
     // deep stencils (reciprocal form) on smoothed levels
     if (l + 1 < nl) {
       std::vector<float> st(10 * (size_t)d.Nd, 0.0f);
@@ -1030,13 +1183,17 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         std::vector<uint16_t> sh(8 * (size_t)d.Nd, 0);
         std::vector<float> sc(2 * (size_t)d.Nd, 0.0f);
         bool range = true;
+        // largest |ratio| kept in FP16: the format's range; GDTP_FP16_RATIO_LIMIT lowers it (tests
+        // use it to force the FP32 fallback on ordinary operators)
+        float limit = 65504.0f;
+        if (const char* env = std::getenv("GDTP_FP16_RATIO_LIMIT")) limit = std::min(limit, (float)std::atof(env));
 #pragma omp parallel for schedule(static) reduction(&& : range)
         for (long t = 0; t < d.Nd; t++) {
           double round = 0.0;
           for (int k = 0, o = 0; k < 9; k++) {
             if (k == 4) continue;
             const float v = st[(size_t)k * d.Nd + t];
-            if (!(std::fabs(v) < 65504.0f)) range = false;
+            if (!(std::fabs(v) < limit)) range = false;
             const _Float16 h = (_Float16)v;
             sh[8 * t + o++] = __builtin_bit_cast(uint16_t, h);
             round += (double)v - (double)(float)h;
@@ -1101,6 +1258,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       }
     }
     d.e = Zeros<float>(ms, (size_t)d.Nd * kcap);
+    if (d.ftiles[0]) d.e2 = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.rhs = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.res = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.tmpB = Zeros<float>(ms, (size_t)std::max(d.B, 1) * kcap);
@@ -1143,6 +1301,51 @@ static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
   else KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
 }
 
+// One fused four-colour sweep (KMixDeepGsFused) over the level's tiles of shape `shape`, columns in
+// chunks of up to four; e and e2 swap afterwards.
+template <bool kHalf, bool kPost, int TA, int TB>
+static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s) {
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const long off = (long)c0 * d.Nd;
+#define GDTP_FUSED(KC)                                                                                                              \
+  KMixDeepGsFused<kHalf, kPost, TA, TB, KC><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, d.Nd, d.stencil,     \
+                                                                             d.stencilC, d.stencilH, d.type, d.e + off, d.e2 + off, \
+                                                                             d.rhs + off)
+    switch (std::min(4, K - c0)) {
+      case 4: GDTP_FUSED(4); break;
+      case 3: GDTP_FUSED(3); break;
+      case 2: GDTP_FUSED(2); break;
+      default: GDTP_FUSED(1); break;
+    }
+#undef GDTP_FUSED
+  }
+  std::swap(d.e, d.e2);
+}
+
+template <bool kHalf, bool kPost>
+static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s) {
+  static_assert(kFusedShapes == 4, "one case per fused tile shape");
+  switch (shape) {
+    case 0: FusedLaunch<kHalf, kPost, kFusedShape[0][0], kFusedShape[0][1]>(d, 0, K, s); break;
+    case 1: FusedLaunch<kHalf, kPost, kFusedShape[1][0], kFusedShape[1][1]>(d, 1, K, s); break;
+    case 2: FusedLaunch<kHalf, kPost, kFusedShape[2][0], kFusedShape[2][1]>(d, 2, K, s); break;
+    default: FusedLaunch<kHalf, kPost, kFusedShape[3][0], kFusedShape[3][1]>(d, 3, K, s); break;
+  }
+}
+
+// One deep four-colour sweep of a tiled level (colours 0..3, post: 3..0): fused when the system
+// uses a fused tile shape, else one launch per colour.  Returns the launches made.
+static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStream_t s) {
+  const int shape = ms.tileShape;
+  if (shape >= 0 && d.nFTiles[shape] > 0) {
+    if (d.stencilH) post ? FusedShapeLaunch<true, true>(d, shape, K, s) : FusedShapeLaunch<true, false>(d, shape, K, s);
+    else post ? FusedShapeLaunch<false, true>(d, shape, K, s) : FusedShapeLaunch<false, false>(d, shape, K, s);
+    return (K + 3) / 4;
+  }
+  for (int qq = 0; qq < 4; qq++) DeepGsLaunch(d, post ? 3 - qq : qq, K, s);
+  return 4;
+}
+
 // zeroE: e is zero on the whole level (start of pre-smoothing), so the first sweep takes the rhs
 // as its residual without a residual pass.
 static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s, bool zeroE) {
@@ -1170,10 +1373,7 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool
       if (d.stencilH) KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
       else KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
     } else if (d.nTiles > 0) {
-      for (int qq = 0; qq < 4; qq++) {
-        int q = post ? 3 - qq : qq;
-        DeepGsLaunch(d, q, K, s);
-      }
+      DeepSweep(ms, d, post, K, s);
     }
     if (post && it == nu - 1) BoundarySweeps(ms, d, K, s, false);
   }
@@ -1191,11 +1391,11 @@ static constexpr size_t kMaxGraphs = 8;
 // automatic V-cycles per iteration: 2 when the level-0 row-sum ratio is at least this, else 1
 static constexpr double kTwoCycleRatio = 2e-6;
 
-void MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
+int MixedDeepSweep(MixedSystem& ms, int l, int K, hipStream_t s) {
   if (l < 0 || l + 1 >= ms.levels) throw std::runtime_error("no smoother on this level");
   DevMixLevel& d = ms.L[l];
-  for (int q = 0; q < 4; q++)
-    DeepGsLaunch(d, q, K, s);
+  if (d.chartSweeps) throw std::runtime_error("level swept one chart per workgroup, not by tiles");
+  return DeepSweep(ms, d, false, K, s);
 }
 
 // zeroE0: the level-0 correction is zero on entry (first V-cycle after a residual was written);
@@ -1208,7 +1408,7 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
     const long rb = d.B > 0 ? NBlk((long)d.B * kRowLanes, 256) : 0, cb = c.B > 0 ? NBlk((long)c.B * kRowLanes, 256) : 0;
     if (d.nTiles + rb > 0)
       KMixResidual<<<d.nTiles + rb, 256, 0, s>>>(d.nTiles, d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
-                                                 ms.nuPre > 0 ? 3 : -1, d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
+                                                 ms.nuPre > 0 && !d.stencilH ? 3 : -1, d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
     if (c.nTiles + cb > 0)
       KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs,
                                                  c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);

@@ -115,6 +115,20 @@ int gdtp_coarse_solve(gdtp_system* s, const double* rhs, double* x, int ncols, v
 int gdtp_apply(gdtp_system* s, int which, const double* x, double* y, int ncols, void* stream);
 /* r = b - A x on level 0 (ComputeSystemResdiual, :953-1007). */
 int gdtp_residual(gdtp_system* s, const double* x, const double* b, double* r, int ncols, void* stream);
+/* Per column ||b - A x|| / ||b|| on level 0 in FP64 (host rel_res[ncols], may be NULL) and the two
+ * norms (host norms[2 ncols] = ||b_c - A x_c||, ||b_c||, may be NULL), without writing the
+ * residual; synchronises `stream`.  Bitwise repeatable (fixed-order reduction).  Replaces
+ * ComputeSystemResdiual followed by the apps' residual-norm printout (IterativeSolvers.inl:953-1007,
+ * TextureFiltering.cpp:540-558). */
+int gdtp_residual_norms(gdtp_system* s, const double* x, const double* b, int ncols, double* rel_res, double* norms, void* stream);
+/* Exact FP64 solve of A x = b on level 0: the apps' --useDirectSolver path (FullMatrixConstruction
+ * and solve(directSolver, ...), UpdateCoefficients.inl:137-219, TextureFiltering.cpp:509-558,
+ * Geodesics.cpp:259-301).  Up to 2^20 texels: LDL^T of the full level-0 matrix (nested dissection,
+ * factorised on the host by the first call after create / update and kept; triangular solves on
+ * the GPU).  Larger levels: the FP64 direct-equivalent, reference-schedule V-cycles from x = 0
+ * until the relative residual reaches 1e-14 or stops falling (the FP64 floor).  x is overwritten;
+ * rel_res (host, ncols) and method (host: 0 factorisation, 1 V-cycles) may be NULL. */
+int gdtp_direct_solve(gdtp_system* s, double* x, const double* b, int ncols, double* rel_res, int* method, void* stream);
 
 /* Normalised-gradient divergence of the heat-method geodesics: for K potentials u (level-0
  * columns), rhs[:, t] = integral < -grad u / |grad u|_g , grad phi_t >_g over the surface, one
@@ -182,6 +196,12 @@ int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg);
  * residual grows continues with 1.  No counterpart in the reference, whose solve loop is VCycle
  * repeated (TextureFiltering.cpp:562-579). */
 int gdtp_mixed_cycles(gdtp_system* s, int cycles);
+/* Tile shape of the mixed solver's deep sweep on levels swept by tiles: 0..3 = one fused launch per
+ * sweep (all four colours, the tile and its halo staged in LDS) on tiles of 64x8, 64x4, 128x4 or
+ * 32x16 sub-grid texels (the default 0); -1 = one launch per colour on 64x4 tiles.  Every shape gives
+ * bitwise the same iterate.  shape -2 only reports; `current` (may be NULL) receives the shape in
+ * use.  No counterpart in the reference. */
+int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */
 int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream);

@@ -1,8 +1,14 @@
 /* ORACLE — test infrastructure only (see oracle.h).  Each function states the reference lines
- * it restates; loops run in the reference's sequential order. */
+ * it restates; loops run in the reference's sequential order, or in parallel (OpenMP) over
+ * iterations that write disjoint outputs, which gives the sequential result bit for bit. */
 #include "oracle.h"
 
+#include <omp.h>
 #include <stddef.h>
+
+/* Threads of the parallel loops (tests compare one thread with many: the results are identical). */
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 /* UpdateRow (IterativeSolvers.inl:438-465): left-to-right Gauss-Seidel over the row's segments. */
 static void update_row(const oracle_tasks* t, int row, const double* coef, double* x, const double* b, int N, int K) {
@@ -58,10 +64,13 @@ static void update_block(const oracle_tasks* t, int blk, const double* coef, dou
 }
 
 void oracle_band_relax(const oracle_tasks* t, const double* coef, double* x, const double* b, double* r, int N, int K) {
-  /* even tasks, then odd tasks (IterativeSolvers.inl:511-521) */
-  for (int color = 0; color < 2; color++)
+  /* even tasks, then odd tasks (IterativeSolvers.inl:511-521); the tasks of one colour touch
+   * disjoint rows, so they run in parallel as in the reference, with the sequential result */
+  for (int color = 0; color < 2; color++) {
+#pragma omp parallel for schedule(dynamic, 1)
     for (int task = color; task < t->n_tasks; task += 2)
       for (int blk = t->task_ptr[task]; blk < t->task_ptr[task + 1]; blk++) update_block(t, blk, coef, x, b, r, N, K);
+  }
 }
 
 static double row_dot(const oracle_csr* m, int i, const double* x) {
@@ -72,6 +81,7 @@ static double row_dot(const oracle_csr* m, int i, const double* x) {
 
 void oracle_boundary_rhs(int B, const int* bgi, const oracle_csr* bd, const double* x, const double* b, double* vb, int N, int K) {
   for (int k = 0; k < K; k++)
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < B; i++) vb[(size_t)k * B + i] = b[(size_t)k * N + bgi[i]] - row_dot(bd, i, x + (size_t)k * N);
 }
 
@@ -79,6 +89,7 @@ void oracle_boundary_residual(int B, const int* bgi, const oracle_csr* bd, const
                               double* r, int N, int K) {
   for (int k = 0; k < K; k++) {
     const double* X = x + (size_t)k * N;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < B; i++) {
       double v = b[(size_t)k * N + bgi[i]] - row_dot(bd, i, X);
       double s = 0;
@@ -93,7 +104,9 @@ void oracle_restrict(int Bc, const int* bgi_c, const oracle_csr* restr, int n_li
   for (int k = 0; k < K; k++) {
     const double* in = fine + (size_t)k * Nf;
     double* out = coarse + (size_t)k * Nc;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < Bc; i++) out[bgi_c[i]] = row_dot(restr, i, in);
+#pragma omp parallel for schedule(static)
     for (int l = 0; l < n_lines; l++) {
       const int* L = lines + 5 * l;
       const double *cu = in + L[0], *pv = in + L[2], *nx = in + L[3];
@@ -109,6 +122,7 @@ void oracle_prolong_add(int n_lines, const int* lines, const double* coarse, int
   for (int k = 0; k < K; k++) {
     const double* in = coarse + (size_t)k * Nc;
     double* out = fine + (size_t)k * Nf;
+#pragma omp parallel for schedule(static)
     for (int l = 0; l < n_lines; l++) {
       const int* L = lines + 6 * l;
       int off = L[5] ? 0 : 1;
@@ -126,11 +140,13 @@ void oracle_apply(int n_raster, const int* raster, const double* coef_raw, int B
   for (int k = 0; k < K; k++) {
     const double* X = x + (size_t)k * N;
     double* Y = y + (size_t)k * N;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < B; i++) {
       double s = 0;
       for (int q = bb->rowptr[i]; q < bb->rowptr[i + 1]; q++) s += X[bgi[bb->col[q]]] * bb->val[q];
       Y[bgi[i]] = s + row_dot(bd, i, X);
     }
+#pragma omp parallel for schedule(static)
     for (int l = 0; l < n_raster; l++) {
       const int* L = raster + 5 * l;
       const double *cu = X + L[0], *pv = X + L[2], *nx = X + L[3];

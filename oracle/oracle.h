@@ -44,4 +44,9 @@ void oracle_prolong_add(int n_lines, const int* lines, const double* coarse, int
 void oracle_apply(int n_raster, const int* raster, const double* coef_raw, int B, const int* bgi, const oracle_csr* bd,
                   const oracle_csr* bb, const double* x, double* y, int N, int K);
 
+/* OpenMP threads of the parallel loops (iterations with disjoint outputs: any count gives the
+ * sequential result bit for bit). */
+void oracle_set_threads(int n);
+int oracle_max_threads(void);
+
 #endif

@@ -209,8 +209,27 @@ class Solvers:
         self.coarse = spla.splu(sp.csc_matrix(coarse_matrix))
 
 
+_pool = None
+
+
+def set_threads(n):
+    """Threads of the C kernels' parallel loops and of the per-column exact solves."""
+    global _pool
+    load().oracle_set_threads(int(n))
+    _pool = None
+
+
 def _solve(lu, rhs):
-    return np.ascontiguousarray(np.stack([lu.solve(rhs[k]) for k in range(rhs.shape[0])]))
+    """Exact solve per column, the columns in parallel (the reference solves its three channels in
+    an OpenMP loop, Solver.h:197-198)."""
+    global _pool
+    K = rhs.shape[0]
+    if K == 1:
+        return np.ascontiguousarray(lu.solve(rhs[0])[None])
+    if _pool is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _pool = ThreadPoolExecutor(max(1, min(16, load().oracle_max_threads())))
+    return np.ascontiguousarray(np.stack(list(_pool.map(lambda k: lu.solve(rhs[k]), range(K)))))
 
 
 def relaxation(L, lu, x, b, r=None):

@@ -1,0 +1,88 @@
+"""Fused four-colour deep sweep (one launch per sweep, tile and halo staged in LDS, ping-pong
+correction buffers) against one launch per colour: every tile shape gives bitwise the same iterate
+of the mixed solver, on screened operators (FP16 smoother stencil) and on a nearly singular one
+(FP32 smoother stencil), for column counts that fill one and two column chunks.  The FP32 fallback
+for FP16-range stencils (forced by lowering the FP16 ratio limit) solves to the direct solution."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle
+from helpers import noise_texture_rhs, oracle_levels
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+import gradientdomaintextureprocessing_amd as g  # noqa: E402
+
+
+def iterate(S, b, shape, iters, use_graph):
+    S.tile_shape(shape)
+    S.configure_mixed(use_graph=use_graph)
+    x = torch.zeros_like(b)
+    try:
+        S.solve(x, b, tol=1e-30, max_iters=iters, mode="mixed")
+    except g._lib.GdtpError as e:  # the tolerance is out of reach on purpose: a fixed number of iterations
+        assert e.code == 3
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["icosphere512", "cube256-singular"])
+@pytest.mark.parametrize("K", [3, 6])
+def test_fused_sweep_is_bitwise_four_launches(case, K):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if case == "icosphere512":
+        mesh = g.TexturedMesh.icosphere(4, 512, 512, 0.05, 3)
+        H = g.InitializeHierarchy(mesh, 512, 512, 5)
+        S = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=K)
+    else:  # the geodesic distance operator 1e-6 M + S keeps the FP32 smoother stencil
+        mesh = g.TexturedMesh.cube(16, 256, 256)
+        H = g.InitializeHierarchy(mesh, 256, 256, 4)
+        S = g.UpdateLinearSystem(H, 1.0, 1e-6, max_cols=K)
+    N = H.level_info(0)["texels"]
+    b = torch.from_numpy(np.random.default_rng(K).standard_normal((K, N))).cuda()
+    print(f"[fused {case} K={K}] per-colour launches", flush=True)
+    want = iterate(S, b, -1, 2, False)
+    assert np.all(np.isfinite(want)) and np.abs(want).max() > 0
+    for shape in (0, 1, 2, 3):
+        for use_graph in (False, True):
+            print(f"[fused {case} K={K}] shape {shape} graph {use_graph}", flush=True)
+            got = iterate(S, b, shape, 2, use_graph)
+            assert np.array_equal(got, want), f"shape {shape}, graph {use_graph}: max diff {np.abs(got - want).max()}"
+    # a single deep sweep through the C ABI reports the fused launch (no error on a tiled level)
+    S.tile_shape(0)
+    S.mixed_deep_sweep(0, K)
+    torch.cuda.synchronize()
+
+
+def test_fp32_smoother_fallback_solves_to_direct():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mesh = g.TexturedMesh.icosphere(4, 256, 256, 0.05, 3)
+    H = g.InitializeHierarchy(mesh, 256, 256, 4)
+    S16 = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=3)
+    S16.configure_mixed()
+    os.environ["GDTP_FP16_RATIO_LIMIT"] = "0.1"  # every stencil ratio of 1e-3 S + M exceeds it somewhere
+    try:
+        S32 = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=3)
+        S32.configure_mixed()
+    finally:
+        del os.environ["GDTP_FP16_RATIO_LIMIT"]
+    # FP32 smoother stencils: 40 bytes per dense texel instead of 24
+    assert S32.memory()[1] > S16.memory()[1]
+    L = oracle_levels(H, S32)
+    A = oracle.full_matrix(L[0]).tocsc()
+    _, b = noise_texture_rhs(H, L[0], K=3, seed=8)
+    x = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+    its, rel = S32.solve(x, torch.from_numpy(b).cuda(), tol=1e-9, max_iters=40, mode="mixed")
+    xs = x.cpu().numpy()
+    lu = spla.splu(A)
+    xd = np.stack([lu.solve(b[k]) for k in range(3)])
+    e = xs - xd
+    en = np.sqrt(np.einsum("ij,ij->i", e, (A @ e.T).T) / np.einsum("ij,ij->i", xd, (A @ xd.T).T))
+    print(f"[fp32 fallback] {its} iterations, energy error {en}", flush=True)
+    assert np.all(en < 1e-7)

@@ -142,3 +142,19 @@ def test_apply_is_full_matrix_product(case):
     got = oracle.apply(L[0], bad.ravel(), L[0].bd, L[0].bb, x)
     want = (oracle.full_matrix(L[0]) @ x.T).T
     assert np.abs(got - want).max() > 1e-6 * np.abs(want).max()
+
+
+def test_parallel_oracle_equals_one_thread(case):
+    """The oracle's OpenMP loops (same-colour band tasks, rows with disjoint outputs) and the
+    per-column exact solves give the one-thread V-cycle bit for bit."""
+    H, S, L = case
+    x0, b = noise_texture_rhs(H, L[0], K=3)
+    solv = oracle.Solvers(L, oracle.full_matrix(L[-1]))
+    out = []
+    for threads in (1, 8):
+        oracle.set_threads(threads)
+        x = x0.copy()
+        oracle.vcycle(L, solv, x, b)
+        out.append(x)
+    oracle.set_threads(8)
+    assert np.array_equal(out[0], out[1])
