@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 namespace gdtp {
 
@@ -26,6 +28,18 @@ const double kQW[6] = {0.109951743655322, 0.109951743655322, 0.109951743655322,
                        0.223381589678011, 0.223381589678011, 0.223381589678011};
 
 using Weights = NodeWeights;
+
+// Seam pieces are fanned from their centroid; with GDTP_SEAM_FAN=vertex in the environment, from
+// their first ring node instead.  Both are valid quadratic triangulations of the piece: tests
+// compare the two to bound how much the solution depends on the choice (the reference's Triangle
+// triangulation is a third one).
+bool VertexFan() {
+  static const bool v = [] {
+    const char* e = std::getenv("GDTP_SEAM_FAN");
+    return e && std::string(e) == "vertex";
+  }();
+  return v;
+}
 
 V2 ChartVertex(const Hierarchy& h, int c, int lv) { return h.atlas.charts[c].vertices[lv] - h.levels[0].charts[c].corner; }
 
@@ -271,6 +285,28 @@ int SeamPieceElements(const SeamContext& ctx, int c, int t, int i, int j, const 
   std::vector<Weights> rw(m);
   for (int q = 0; q < m; q++)
     if (!weightsOf(ring[q], rw[q])) return -1;
+  if (VertexFan()) {  // alternative triangulation: a fan from ring node 0 (quantifies the model's freedom)
+    double area2 = 0.0;  // twice the piece's area; fan triangles of three collinear ring nodes are dropped
+    for (int v = 0; v < n; v++) area2 += cross(piece[v], piece[(v + 1) % n]);
+    for (int q = 1; q + 1 < m; q++) {
+      SeamElement el;
+      el.p[0] = ring[0].p, el.p[1] = ring[q].p, el.p[2] = ring[q + 1].p;
+      if (std::fabs(cross(el.p[1] - el.p[0], el.p[2] - el.p[0])) < 1e-9 * std::fabs(area2)) continue;
+      el.w[0] = rw[0], el.w[1] = rw[q], el.w[2] = rw[q + 1];
+      // edge midpoints: ring edges carry their seam values, diagonals are inside the piece
+      auto midWeights = [&](int a, int b, Weights& w) -> int {
+        const int e = (b == a + 1) ? a : ((a == 0 && b == m - 1) ? m - 1 : -1);
+        if (e >= 0 && ringEdge[e].k >= 0) {
+          const Node mid{(ring[a].p + ring[b].p) * 0.5, 3 + ringEdge[e].k, 0.5 * (ringEdge[e].s0 + ringEdge[e].s1)};
+          return weightsOf(mid, w);
+        }
+        return Bilinear(h, c, (ring[a].p + ring[b].p) * 0.5, i, j, 1.0, w, err);
+      };
+      if (!midWeights(0, q, el.w[3]) || !midWeights(q, q + 1, el.w[4]) || !midWeights(0, q + 1, el.w[5])) return -1;
+      els.push_back(std::move(el));
+    }
+    return 1;
+  }
   V2 cen(0, 0);
   for (const V2& p : piece) cen = cen + p;
   cen = cen / (double)n;
