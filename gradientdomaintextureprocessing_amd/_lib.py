@@ -61,6 +61,8 @@ SIGNATURES = {
     "gdtp_solve_weighted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_geodesic_divergence": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p]),
+    "gdtp_solve_yielding": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_int, c_void_p, P(c_int), c_void_p, c_void_p,
+                                    c_void_p]),
     "gdtp_mixed_configure": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_int]),
     "gdtp_mixed_cycles": (c_int, [c_void_p, c_int]),
     "gdtp_hierarchy_create_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, P(c_void_p)]),

@@ -38,6 +38,9 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_YIELD = ctypes.CFUNCTYPE(None, ctypes.c_void_p)  # void (*between)(void*) of gdtp_solve_yielding
+
+
 class TexturedMesh:
     """Triangle mesh with per-corner texture coordinates (include/Src/SimpleMesh.h:45-49)."""
 
@@ -342,14 +345,21 @@ class MultigridSystem:
         (pu, pr), K = self._pair(potential, rhs, self._texels[0])
         _lib.check(_lib.load().gdtp_geodesic_divergence(self._h, pu, pr, K, _stream()))
 
-    def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed"):
-        """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res)."""
+    def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed", between=None):
+        """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res).
+        between: optional callable run after every unconverged iteration (the device is idle
+        then; the solve continues afterwards without restarting)."""
         (px, pb), K = self._pair(x, b, self._texels[0])
         rel = np.zeros(K)
         it = ctypes.c_int(0)
         m = 1 if mode == "mixed" else 0
-        _lib.check(_lib.load().gdtp_solve(self._h, px, pb, K, tol, max_iters, m, rel.ctypes.data,
-                                          ctypes.byref(it), _stream()))
+        if between is None:
+            _lib.check(_lib.load().gdtp_solve(self._h, px, pb, K, tol, max_iters, m, rel.ctypes.data,
+                                              ctypes.byref(it), _stream()))
+        else:
+            cb = _YIELD(lambda _user: between())
+            _lib.check(_lib.load().gdtp_solve_yielding(self._h, px, pb, K, tol, max_iters, m, rel.ctypes.data,
+                                                       ctypes.byref(it), _stream(), ctypes.cast(cb, ctypes.c_void_p), None))
         return it.value, rel
 
     def solve_weighted(self, x, b, weights, tol=1e-6, max_iters=60):
@@ -606,6 +616,32 @@ class LineIntegralConvolution:
         self.convolution_weight, self.stiffness, self.sharpening = convolution_weight, stiffness_weight, sharpening_weight
         self.lic = UpdateLinearSystem(self.anisotropic, convolution_weight, 1.0, max_cols=max_cols)
         self.sharpen = UpdateLinearSystem(self.embedding, stiffness_weight, 1.0, max_cols=max_cols)
+
+    def convolve_lengths(self, noise, lengths, rounds=4):
+        """Diffusion with one length per column (LineIntegralConvolution.cpp:503-527, 611-619 with
+        the licStiffness slider, batched): column c solves (M_a + lengths[c] S_a) x_c = M_a noise_c
+        by `rounds` iterations of the weighted solve (flexible CG preconditioned by this LIC
+        system's V-cycle), from x = 0, as the reference runs a fixed number of rounds.  Returns
+        (x [K, N], relative residual per column after the rounds)."""
+        import torch
+        b = torch.empty_like(noise)
+        self.lic.apply(OPERATOR_MASS, noise, b)
+        x = torch.zeros_like(b)
+        try:  # a fixed number of rounds: "not converged" is the expected outcome
+            _, rel = self.lic.solve_weighted(x, b, [[1.0, float(l)] for l in lengths], tol=1e-300, max_iters=rounds)
+        except _lib.GdtpError as e:
+            if e.code != 3:
+                raise
+            rel = self._weighted_residuals(x, b, lengths)
+        return x, rel
+
+    def _weighted_residuals(self, x, b, lengths):
+        import torch
+        m, s = torch.empty_like(x), torch.empty_like(x)
+        self.lic.apply(OPERATOR_MASS, x, m)
+        self.lic.apply(OPERATOR_STIFFNESS, x, s)
+        lam = torch.tensor(np.asarray(lengths, np.float64), device=x.device)[:, None]
+        return ((b - m - lam * s).norm(dim=1) / b.norm(dim=1)).cpu().numpy()
 
     def convolve(self, noise, passes=4, tol=None, mode="fp64", max_iters=200, between_rounds=None):
         """(x, y): the anisotropically diffused noise and its sharpened version ([K, N] FP64).

@@ -642,6 +642,11 @@ int gdtp_geodesic_divergence(gdtp_system* s, const double* potential, double* rh
 
 int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int maxIters, int mode, double* relRes, int* iterations,
                void* stream) {
+  return gdtp_solve_yielding(s, x, b, ncols, tol, maxIters, mode, relRes, iterations, stream, nullptr, nullptr);
+}
+
+int gdtp_solve_yielding(gdtp_system* s, double* x, const double* b, int ncols, double tol, int maxIters, int mode, double* relRes,
+                        int* iterations, void* stream, void (*between)(void*), void* user) {
   int rc = Ready(s, ncols);
   if (rc != GDTP_OK) return rc;
   if (!x || !b || !(tol > 0) || maxIters < 0 || (mode != GDTP_SOLVE_FP64_REFERENCE && mode != GDTP_SOLVE_MIXED))
@@ -654,7 +659,7 @@ int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol
     bool ok = false;
     if (mode == GDTP_SOLVE_MIXED) {
       MixedSystem& m = Mixed(s);
-      it = SolveMixed(ds, m, x, b, ncols, tol, maxIters, rel.data(), st, s->useGraph != 0);
+      it = SolveMixed(ds, m, x, b, ncols, tol, maxIters, rel.data(), st, s->useGraph != 0, nullptr, between, user);
       ok = true;
       for (double r : rel) ok = ok && r <= tol;
     } else {
@@ -675,6 +680,7 @@ int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol
           ok = ok && rel[k] <= tol;
         }
         if (ok || it >= maxIters) break;
+        if (between && it > 0) between(user);
         VCycleReference(ds, x, b, ncols, N, st);
       }
     }

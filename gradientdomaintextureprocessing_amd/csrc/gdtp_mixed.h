@@ -104,8 +104,11 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 // run; relRes (host, K entries) receives the final relative residuals.  With `weights` (host,
 // K pairs), column c solves (w[c][0] M + w[c][1] S) x = b instead, by flexible CG with the
 // system's own V-cycle as the preconditioner of every column.
+// `between` (may be null) is called with `ctx` after every iteration that leaves the solve
+// unconverged, on the calling thread, with the device idle: a caller can yield the GPU there
+// without ending the solve (flexible CG keeps its search direction).
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph, const double* weights = nullptr);
+               hipStream_t s, bool useGraph, const double* weights = nullptr, void (*between)(void*) = nullptr, void* ctx = nullptr);
 
 // Drop the captured refinement step (after the smoothing configuration changes).
 void InvalidateGraph(MixedSystem& ms);

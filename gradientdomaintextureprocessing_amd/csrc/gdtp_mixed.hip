@@ -1633,7 +1633,7 @@ static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hip
 }
 
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph, const double* weights) {
+               hipStream_t s, bool useGraph, const double* weights, void (*between)(void*), void* ctx) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
   if (K > 1024) throw std::runtime_error("at most 1024 columns per solve");
   // weighted columns run flexible CG: their operators differ from the one the V-cycle was built
@@ -1736,6 +1736,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
       if (useGraph) ge = graphFor(cycles);
     }
     prev.assign(relRes, relRes + K);
+    if (between && it < maxIter) between(ctx);
   }
   return it;
 }

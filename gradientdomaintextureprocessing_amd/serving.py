@@ -23,10 +23,10 @@ next batch.  Short work — filtering on meshes of at most `short_texels` texels
 on all meshes; everything else (geodesics, LIC, filtering on large meshes) is heavy: at most
 `heavy_concurrency` heavy batches run at a time over all meshes, and each holds back while any
 short batch runs (at most `yield_ms`): host-side priorities for the short requests (HIP stream
-priorities are not used).  Heavy batches also yield between chunks of their solves (one
-refinement iteration, or one LIC round): while short requests are queued or running, the next
-chunk waits (at most `yield_ms` each time), so a short request shares the GPU with at most one
-chunk.  Meshes are admitted against an HBM budget.  Results are the same alone
+priorities are not used).  Heavy batches also yield inside their solves, between refinement
+iterations (gdtp_solve_yielding: the solve continues afterwards, flexible CG keeping its search
+direction) or LIC rounds: while short requests are queued or running, the next iteration waits
+(at most `yield_ms` each time), so a short request shares the GPU with at most one iteration.  Meshes are admitted against an HBM budget.  Results are the same alone
 and in a batch up to the requested tolerance: every column is solved to it.
 """
 import math
@@ -290,21 +290,11 @@ class SolverServer:
         with self.running_lock:
             self.running_lock.wait_for(lambda: self.short_pending == 0, timeout=self.yield_ms / 1e3)
 
-    def _solve_in_chunks(self, system, x, b, chunk=1):
-        """Mixed solve to self.tol from x in chunks of `chunk` refinement iterations (each chunk
-        continues from the previous x), yielding to running filtering batches in between."""
-        done = 0
-        while True:
-            try:
-                its, _ = system.solve(x, b, tol=self.tol, max_iters=min(chunk, self.max_iters - done), mode="mixed")
-                return done + its
-            except _lib.GdtpError as e:
-                if e.code != 3:  # anything but "not converged yet" is an error
-                    raise
-                done += chunk
-                if done >= self.max_iters:
-                    raise
-            self._yield_to_filters()
+    def _solve_yielding(self, system, x, b):
+        """Mixed solve to self.tol from x that yields to running filtering batches between its
+        iterations (flexible CG keeps its search direction across the yields)."""
+        its, _ = system.solve(x, b, tol=self.tol, max_iters=self.max_iters, mode="mixed", between=self._yield_to_filters)
+        return its
 
     def _run(self, m, batch, lane):
         torch = self.torch
@@ -339,10 +329,10 @@ class SolverServer:
             b.zero_()
             b[rows, src] = 1.0
             x.zero_()
-            its = self._solve_in_chunks(system, x, b)
+            its = self._solve_yielding(system, x, b)
             system.geodesic_divergence(x, t0)
             t1.zero_()
-            its2 = self._solve_in_chunks(m.distance, t1, t0)
+            its2 = self._solve_yielding(m.distance, t1, t0)
             its += its2
             t1 -= t1[rows, src][:, None]
             out = t1

@@ -168,6 +168,13 @@ void gdtp_modulation_destroy(gdtp_modulation* m);
  * (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached first. */
 int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int max_iters, int mode, double* rel_res,
                int* iterations, void* stream);
+/* gdtp_solve that calls between(user) on the calling thread after every iteration (mixed: outer
+ * iteration; FP64: V-cycle) that leaves the solve unconverged, with the stream idle: a request loop
+ * can let other work use the GPU there without restarting the solve (flexible CG keeps its search
+ * direction).  between may be NULL.  No counterpart in the reference (its apps step one V-cycle per
+ * idle callback, TextureFiltering.cpp:366-376). */
+int gdtp_solve_yielding(gdtp_system* s, double* x, const double* b, int ncols, double tol, int max_iters, int mode, double* rel_res,
+                        int* iterations, void* stream, void (*between)(void*), void* user);
 /* Parametric solve (mixed precision): column c solves (w[2c] M + w[2c+1] S) x = b to `tol` by
  * flexible CG, the FP64 operator applied from the level-0 mass and stiffness with each column's
  * weights and the system's own FP32 V-cycle (built for its create/update weights) as
