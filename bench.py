@@ -264,6 +264,76 @@ def serving_bench(qps, n_requests, tol, seed=0):
     print(json.dumps(out))
 
 
+SERVING_LEG_MESHES = [m for m in SERVING_MESHES if m[0] in ("cube512", "ico1024", "torus1024")]
+
+
+def serving_leg(qps=20.0, n_requests=150, tol=1e-6, seed=1):
+    """Short request-loop measurement for the default run: three resident meshes (cube 512^2,
+    noisy icosphere 1024^2, torus 1024^2 with line integral convolution), Poisson arrivals at `qps`
+    mixing 60 % filtering, 30 % geodesics and 10 % LIC as in serving_bench.  Returns the 1024^2
+    filtering latency percentiles, all-request percentiles and columns per second."""
+    import torch
+    import gradientdomaintextureprocessing_amd as g
+    from gradientdomaintextureprocessing_amd.serving import SolverServer, latency_percentiles
+    t0 = time.time()
+    server = SolverServer(tol=tol, max_iters=120)
+    sizes = {}
+    try:
+        for name, kind, p, W in SERVING_LEG_MESHES:
+            levels = max(4, int(np.log2(W)) - 4)
+            lic = None
+            if kind == "cube":
+                mesh = g.TexturedMesh.cube(p["subdiv"], W, W)
+            elif kind == "icosphere":
+                mesh = g.TexturedMesh.icosphere(p["levels"], W, W, p["amplitude"], p["seed"])
+            else:
+                mesh = g.TexturedMesh.torus(p["nu"], p["nv"], p["cu"], p["cv"], W, W, 1.0, 0.35, p["amplitude"], p["seed"])
+                v, t, _ = mesh.arrays()
+                ph = np.random.default_rng(p["seed"]).random(3) * 6.28
+                field = g.surface_gradient_field(v, t, np.sin(3 * v[:, 0] + ph[0]) + np.cos(2 * v[:, 1] + ph[1]) + np.sin(4 * v[:, 2] + ph[2]))
+                lic = g.LineIntegralConvolution(mesh, W, W, levels, field, max_cols=8, assembly="device")
+            H = g.InitializeHierarchy(mesh, W, W, levels, assembly="device")
+            server.add_mesh(name, H, max_cols=8, lic=lic)
+            sizes[name] = H.level_info(0)["texels"]
+        admit = time.time() - t0
+        log(f"[bench] serving leg: {len(sizes)} meshes resident in {admit:.1f}s")
+        rng = np.random.default_rng(seed)
+        names = list(sizes)
+        payloads = {n: torch.rand(3, sizes[n], dtype=torch.float64, device="cuda") for n in names}
+        plan = []
+        for _ in range(n_requests):
+            n = names[rng.integers(len(names))]
+            u = rng.random()
+            if u < 0.6:
+                plan.append(("filter", n, float(10 ** rng.uniform(-4, -2))))
+            elif u < 0.9:
+                plan.append(("geodesic", n, [int(v) for v in rng.integers(0, sizes[n], rng.integers(1, 9))]))
+            else:
+                plan.append(("lic", "torus1024", None))
+        gaps = rng.exponential(1.0 / qps, n_requests)
+        handles = []
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for (kind, n, a), gap in zip(plan, gaps):
+            time.sleep(gap)
+            if kind == "filter":
+                handles.append((kind, n, server.submit_filter(n, payloads[n], a)))
+            elif kind == "geodesic":
+                handles.append((kind, n, server.submit_geodesic(n, a)))
+            else:
+                handles.append((kind, n, server.submit_lic(n, payloads[n])))
+        for _, _, h in handles:
+            h.wait(600)
+        el = time.perf_counter() - start
+    finally:
+        server.close()
+    cols = sum(h.columns for _, _, h in handles)
+    return dict(qps_offered=qps, requests=n_requests, wall_s=el, columns_per_s=cols / el, admission_s=admit,
+                meshes={n: sizes[n] for n in names}, tol=tol,
+                filter_1024=latency_percentiles([h for k, n, h in handles if k == "filter" and n.endswith("1024")]),
+                all=latency_percentiles([h for _, _, h in handles]))
+
+
 def load_traffic(kernel, workload_key):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE passes, scaled
     by the calibration copy) and recorded in profiles/traffic.json; None when not measured."""
@@ -439,6 +509,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--serving-leg", default="on", choices=["on", "off"],
+                    help="short request-loop leg (three meshes, 20 QPS) reported beside the solve throughput")
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
     ap.add_argument("--workload", default=None, choices=["c3", "c4", "assembly"],
@@ -474,6 +546,11 @@ def main():
         mesh, H, S = make_system(args.config, K, mixed=(mode == "mixed"))
         f, b = texture_rhs(H, S, K)
         x, r = torch.zeros_like(b), torch.zeros_like(b)
+        tiles = None
+        if mode == "mixed":  # pick the fused deep-sweep tile shape on this system (level 0, K columns)
+            solve(S, x, b, args.tol, max_cycles, mode)
+            tiles = {str(S.TILE_SHAPES[k]) if k >= 0 else "per-colour": round(v, 2) for k, v in S.autotune_tiles(0, K).items()}
+            log(f"[bench] tile shapes (us per level-0 sweep, fastest first): {tiles}; using {S.TILE_SHAPES[S.tile_shape()]}")
         for _ in range(args.warmup):
             cycles, rel = solve(S, x, b, args.tol, max_cycles, mode)
             log(f"[bench] {mode} warmup: {cycles} iterations, rel res {rel:.2e}")
@@ -490,7 +567,7 @@ def main():
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         res = dict(value=K * args.steps / el, ms_per_step=el / args.steps * 1e3, iterations=cyc, rel_res=rel, levels=H.levels,
-                   latency_ms=dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99))))
+                   latency_ms=dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99))), tiles=tiles)
         log(f"[bench] {mode}: {args.steps} steps in {el:.3f}s ({res['value']:.1f} solves/s), iterations/step {cyc}, "
             f"final rel res {rel:.2e}")
         if mode == "mixed":
@@ -506,6 +583,10 @@ def main():
             log(f"[bench] cpu baseline: {res['cpu']}")
         results[mode] = (res, H)
         del S, H, mesh
+    serving = None
+    if args.serving_leg == "on" and args.config == "c2":
+        serving = serving_leg()
+        log(f"[bench] serving leg: {serving}")
     main_mode = modes[0]
     res, H = results[main_mode]
     value, ms, cyc, roof = res["value"], res["ms_per_step"], res["iterations"], res["roofline"]
@@ -519,8 +600,9 @@ def main():
         "dtype": "f32 relax (f16 stencil ratios) / f64 refine" if main_mode == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {CONFIGS[args.config][0]} {CONFIGS[args.config][2]}^2 atlas, "
                                f"{info['texels']} texels, 3 RHS, (M+1e-3 S)x=Mf from x=0 to rel-res {args.tol:g}",
-                   "mode": main_mode, "levels": H.levels, "iterations_per_step": cyc},
+                   "mode": main_mode, "levels": H.levels, "iterations_per_step": cyc, "tile_autotune_us": res.get("tiles")},
         "latency_ms": res["latency_ms"],
+        "serving": serving,
         "fp64_reference_gpu": {k: v for k, v in fp.items() if k in ("value", "ms_per_step", "iterations", "levels")} if fp else None,
         "speedup_vs_fp64_gpu": value / fp["value"] if fp else None,
         "roofline": roof, "cpu_baseline": cpu,

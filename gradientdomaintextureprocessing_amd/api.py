@@ -438,6 +438,11 @@ class MultigridSystem:
         _lib.check(_lib.load().gdtp_mixed_tile_shape(self._h, -2 if shape is None else int(shape), ctypes.byref(cur)))
         return cur.value
 
+    def fuse_transfers(self, prolongation=True):
+        """Fuse the prolongation into the first post-smoothing sweep (default) or keep it a
+        separate pass; the iterate is bitwise the same."""
+        _lib.check(_lib.load().gdtp_mixed_fuse_transfers(self._h, 1 if prolongation else 0))
+
     def autotune_tiles(self, level=0, columns=None, sweeps=20, shapes=(0, 1, 2, 3, -1)):
         """Time `sweeps` deep sweeps of `level` for each tile shape and keep the fastest (every shape
         gives the same iterate).  Returns {shape: microseconds per sweep}, fastest first."""
@@ -620,20 +625,39 @@ class LineIntegralConvolution:
     def convolve_lengths(self, noise, lengths, rounds=4):
         """Diffusion with one length per column (LineIntegralConvolution.cpp:503-527, 611-619 with
         the licStiffness slider, batched): column c solves (M_a + lengths[c] S_a) x_c = M_a noise_c
-        by `rounds` iterations of the weighted solve (flexible CG preconditioned by this LIC
-        system's V-cycle), from x = 0, as the reference runs a fixed number of rounds.  Returns
-        (x [K, N], relative residual per column after the rounds)."""
+        by `rounds` iterations of the weighted solve (flexible CG) from x = 0, as the reference runs
+        a fixed number of rounds.  Each column is preconditioned by the V-cycle of the anisotropic
+        system at the power of ten nearest its length (built on first use, kept): one V-cycle serves
+        lengths within a factor of about three of its own, and a column's iterate does not depend on
+        the other columns of the batch.  Returns (x [K, N], relative residual per column against
+        its own operator)."""
         import torch
+        lengths = np.asarray(lengths, np.float64)
         b = torch.empty_like(noise)
         self.lic.apply(OPERATOR_MASS, noise, b)
         x = torch.zeros_like(b)
-        try:  # a fixed number of rounds: "not converged" is the expected outcome
-            _, rel = self.lic.solve_weighted(x, b, [[1.0, float(l)] for l in lengths], tol=1e-300, max_iters=rounds)
-        except _lib.GdtpError as e:
-            if e.code != 3:
-                raise
-            rel = self._weighted_residuals(x, b, lengths)
-        return x, rel
+        decade = np.round(np.log10(lengths)).astype(int)
+        centres = 10.0 ** np.unique(decade)
+        group = np.searchsorted(np.unique(decade), decade)
+        if not hasattr(self, "_length_systems"):
+            self._length_systems = {}
+        for gi, lam in enumerate(centres):
+            cols = np.nonzero(group == gi)[0]
+            if len(cols) == 0:
+                continue
+            key = float(lam)
+            if key not in self._length_systems:
+                self._length_systems[key] = UpdateLinearSystem(self.anisotropic, key, 1.0, max_cols=max(len(lengths), self.lic.max_cols))
+            system = self._length_systems[key]
+            idx = torch.as_tensor(cols, device=b.device)
+            xg, bg = torch.zeros((len(cols), b.shape[1]), dtype=b.dtype, device=b.device), b[idx].contiguous()
+            try:  # a fixed number of rounds: "not converged" is the expected outcome
+                system.solve_weighted(xg, bg, [[1.0, float(l)] for l in lengths[cols]], tol=1e-300, max_iters=rounds)
+            except _lib.GdtpError as e:
+                if e.code != 3:
+                    raise
+            x[idx] = xg
+        return x, self._weighted_residuals(x, b, lengths)
 
     def _weighted_residuals(self, x, b, lengths):
         import torch

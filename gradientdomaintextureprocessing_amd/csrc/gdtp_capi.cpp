@@ -63,7 +63,7 @@ struct gdtp_system {
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
   std::unique_ptr<DirectFactor> direct;  // built by the first gdtp_direct_solve after create / update
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = 0;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = 0, fuseTransfers = 1;
   double omega = 1.0;
   double* dNorms = nullptr;
   double* hNorms = nullptr;
@@ -552,7 +552,18 @@ static MixedSystem& Mixed(gdtp_system* s) {
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
   m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles, m.tileShape = s->tileShape;
+  m.fuseProlong = (s->fuseTransfers & GDTP_FUSE_PROLONGATION) != 0;
   return m;
+}
+
+int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags) {
+  if (!s || (flags & ~GDTP_FUSE_PROLONGATION)) return Fail(GDTP_ERROR_INVALID, "unknown transfer fusion flags");
+  s->fuseTransfers = flags;
+  if (s->mixed) {
+    s->mixed->fuseProlong = (flags & GDTP_FUSE_PROLONGATION) != 0;
+    InvalidateGraph(*s->mixed);
+  }
+  return GDTP_OK;
 }
 
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {

@@ -42,7 +42,7 @@ __device__ __forceinline__ int FindElem(SegRef S, int s0, int s1, int e, int& of
 
 __device__ __forceinline__ void GsRow(SegRef S, int s0, int s1, const double* __restrict__ coef, int D, double* x,
                                       const double* __restrict__ b, int K, int ld) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % kWave;
   int off[2], seg[2], g[2] = {0, 0}, p[2] = {0, 0}, n[2] = {0, 0}, d[2] = {0, 0};
   double a[2] = {0, 0}, c[2][9];
   for (int q = 0; q < 2; q++) {
@@ -99,7 +99,7 @@ __device__ __forceinline__ void ResRow(SegRef S, int s0, int s1, const double* _
                                        const double* __restrict__ b, double* r, int K, int ld) {
   int total = 0;
   for (int s = s0; s < s1; s++) total += S.ce(s) - S.cs(s) + 1;
-  for (int e = threadIdx.x; e < total; e += kWave) {
+  for (int e = threadIdx.x % kWave; e < total; e += kWave) {
     int off, sg = FindElem(S, s0, s1, e, off);
     int g = S.cs(sg) + off, p = S.ps(sg) + off, n = S.ns(sg) + off, d = S.ds(sg) + off;
     double c[10];
@@ -113,37 +113,37 @@ __device__ __forceinline__ void ResRow(SegRef S, int s0, int s1, const double* _
   }
 }
 
-__global__ __launch_bounds__(kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr, const int* __restrict__ blockBegin,
-                                                 const int* __restrict__ blockDeepEnd, const int* __restrict__ blockPadEnd,
-                                                 const int* __restrict__ linePtr, const int* __restrict__ segs, const double* __restrict__ coef,
-                                                 int D, double* x, const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
+// One workgroup of four waves per band task; within a block the three lagged passes and the
+// residual rows run as a pipeline, one row per wave per step: at step t wave 0 relaxes row t
+// (first pass), wave 1 row t-2 (second pass), wave 2 row t-4 (third pass) and wave 3 computes the
+// residual of padded row t-6.  Each operation sees exactly the values it sees in the reference's
+// order (step j: rows j, j-1, j-2, then padded row j-3, IterativeSolvers.inl:497-509): a pass
+// reads its row's neighbours above after their own pass and below before it, which the two-step
+// lag preserves, and a padded row's residual reads deep rows up to one past its own index, final
+// after their third pass (done by step pr+5).  Rows of one step are disjoint.
+__global__ __launch_bounds__(4 * kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr,
+                                                     const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
+                                                     const int* __restrict__ blockPadEnd, const int* __restrict__ linePtr,
+                                                     const int* __restrict__ segs, const double* __restrict__ coef, int D, double* x,
+                                                     const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
   const int t = 2 * blockIdx.x + color;
   if (t >= nTasks) return;
+  const int wave = threadIdx.x / kWave;
   SegRef S{segs};
   for (int blk = taskPtr[t]; blk < taskPtr[t + 1]; blk++) {
     const int r0 = blockBegin[blk], p0 = blockDeepEnd[blk];
     const int nDeep = p0 - r0, nPad = blockPadEnd[blk] - p0;
-    for (int j = 0; j < nDeep + 3; j++) {
-      for (int gi = 0; gi < 3; gi++) {
-        int row = j - gi;
-        if (row >= 0 && row < nDeep) {
-          GsRow(S, linePtr[r0 + row], linePtr[r0 + row + 1], coef, D, x, b, K, ld);
-          __syncthreads();
-        }
+    const int steps = withResidual ? max(nDeep + 4, nPad + 6) : nDeep + 4;
+    for (int st = 0; st < steps; st++) {
+      if (wave < 3) {
+        const int row = st - 2 * wave;
+        if (row >= 0 && row < nDeep) GsRow(S, linePtr[r0 + row], linePtr[r0 + row + 1], coef, D, x, b, K, ld);
+      } else if (withResidual) {
+        const int pr = st - 6;
+        if (pr >= 0 && pr < nPad) ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
       }
-      if (withResidual) {
-        int pr = j - 3;
-        if (pr >= 0 && pr < nPad) {
-          ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
-          __syncthreads();
-        }
-      }
+      __syncthreads();
     }
-    if (withResidual)
-      for (int pr = nDeep; pr < nPad; pr++) {
-        ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
-        __syncthreads();
-      }
   }
 }
 
@@ -751,11 +751,11 @@ void BandSweepLevel(DeviceSystem& ds, int l, double* x, const double* b, double*
   DevLevel& d = ds.L[l];
   if (d.nTasks == 0) return;
   const int withResidual = r != nullptr ? 1 : 0;
-  KBandGs<<<(d.nTasks + 1) / 2, kWave, 0, s>>>(0, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs, d.coef,
-                                             d.D, x, b, r, K, ld, withResidual);
+  KBandGs<<<(d.nTasks + 1) / 2, 4 * kWave, 0, s>>>(0, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs,
+                                                 d.coef, d.D, x, b, r, K, ld, withResidual);
   if (d.nTasks > 1)
-    KBandGs<<<d.nTasks / 2, kWave, 0, s>>>(1, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs, d.coef,
-                                         d.D, x, b, r, K, ld, withResidual);
+    KBandGs<<<d.nTasks / 2, 4 * kWave, 0, s>>>(1, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs,
+                                             d.coef, d.D, x, b, r, K, ld, withResidual);
 }
 
 void RelaxLevel(DeviceSystem& ds, int l, double* x, const double* b, double* r, int K, int ld, bool withResidual, hipStream_t s) {

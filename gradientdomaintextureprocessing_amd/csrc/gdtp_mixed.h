@@ -49,12 +49,14 @@ struct DevMixLevel {
   // vectors [K][Nd] and boundary residual scratch [K][B]; the fused sweep reads e and writes e2
   // (levels swept by tiles), after which the two pointers swap
   float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr, *e2 = nullptr;
+  float* eHome = nullptr;          // the buffer e was allocated as (level 0 returns to it between refinement steps)
 };
 
 struct MixedSystem {
   int levels = 0, kcap = 0;
   int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per Schwarz step
   int tileShape = 0;                              // fused deep-sweep tile shape (kFusedShape), -1: one launch per colour
+  bool fuseProlong = true;                        // fused levels: the first post sweep adds the prolongation itself
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level

@@ -166,6 +166,38 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
   }
 }
 
+__device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+
+// The bilinear prolongation of the coarse column E (dense coarse planes) at fine texel
+// (2a + (q & 1), 2b + (q >> 1)): the mean of its four (repeated) coarse parents, with the parents
+// and their order of summation as in KMixProlongAdd, so adding it gives bitwise the same value.
+__device__ __forceinline__ float ProlongTerm(const MixChart& f, const MixChart& c, long cplane, const float* E, int a, int b, int q) {
+  const int ui0 = 2 * a - f.cofx, uj0 = 2 * b - f.cofy;
+  const int ci0 = FloorHalf(ui0) + c.cofx, cj0 = FloorHalf(uj0) + c.cofy;
+  const bool ei = (ui0 & 1) == 0, ej = (uj0 & 1) == 0;
+  const int qi = q & 1, qj = q >> 1;
+  const int bx = ei ? 0 : qi, by = ej ? 0 : qj;
+  const bool ox = ei ? qi == 1 : qi == 0, oy = ej ? qj == 1 : qj == 0;
+  const int need = ((1 | (ox ? 2 : 0)) * (1 | (oy ? 4 : 0))) << (bx + 2 * by);
+  float V[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) V[n] = (need >> n) & 1 ? Ld(E, DenseByte(c, cplane, ci0 + (n & 1), cj0 + (n >> 1))) : 0.0f;
+  const bool sx = !ei && qi == 1, sy = !ej && qj == 1;
+  const float b0 = sx ? V[1] : V[0], b1 = sx ? V[3] : V[2];
+  const float v00 = sy ? b1 : b0;
+  const float v10 = ox ? (sy ? V[3] : V[1]) : v00;
+  const float v01 = oy ? b1 : v00;
+  const float v11 = (ox && oy) ? V[3] : (ox ? v10 : v01);
+  return (v00 + v10 + v01 + v11) * 0.25f;
+}
+
+// Prolongation operands of the fused sweep that starts post-smoothing (null ce: no prolongation).
+struct FusedProlong {
+  const MixChart* cch = nullptr;
+  long cplane = 0, cNd = 0;
+  const float* ce = nullptr;
+};
+
 // ------------------------------------------------------------------ fused four-colour sweep
 // All four colours of one sweep in one launch.  A workgroup owns a TA x TB tile of the sub-grid
 // (all four planes) and stages the window [-2, TA+2) x [-1, TB+1) around it in LDS: the old
@@ -186,110 +218,220 @@ __host__ __device__ constexpr FusedBox FusedRegion(bool post, int q) {
               : (q == 0 ? FusedBox{-1, 2, 0, 1} : q == 1 ? FusedBox{-1, 1, 0, 1} : q == 2 ? FusedBox{0, 1, 0, 0} : FusedBox{0, 0, 0, 0});
 }
 
-template <bool kHalf, bool kFirst, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY>
-__device__ __forceinline__ void FusedColour(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
-                                            const float2* __restrict__ stc, const uint4* __restrict__ sth,
-                                            const int8_t* __restrict__ type, const float* __restrict__ eIn, float* __restrict__ eOut,
-                                            const float* __restrict__ rhs, float* sm) {
-  constexpr int WX = TA + 4, WN = WX * (TB + 2);
+// The operands one thread fetches for its texels of one colour's region (at most kIt of them):
+// stencil (FP16 ratios still packed, or FP32), rhs and, for the first colour, the texel's old
+// value where it is not deep.
+struct Stencil9 {
+  float v[9];
+};
+template <bool kHalf, bool kFirst, int KC, int kIt>
+struct FusedOperands {
+  float c4[kIt], corr[kIt], r[kIt][KC], old[kFirst ? kIt : 1][KC];
+  typename std::conditional<kHalf, uint4, Stencil9>::type sten[kIt];
+  int idx[kIt], w[kIt], state[kIt];  // state 0: outside the chart or region, 1: deep, 2: active not deep, 3: inactive
+};
+
+template <bool kHalf>
+__device__ __forceinline__ void Unpack(const uint4& w, float (&cf)[9]) {
+  cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
+  cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
+}
+
+// Thread t's texels of the region [LX, TA + HX) x [LY, TB + HY) of plane Q: t, t + 256, ...
+template <bool kHalf, bool kFirst, bool kProlong, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
+__device__ __forceinline__ void FusedFetch(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
+                                           const float2* __restrict__ stc, const uint4* __restrict__ sth,
+                                           const int8_t* __restrict__ type, const float* __restrict__ eIn, const float* __restrict__ rhs,
+                                           const FusedProlong& pr, const MixChart& cc, FusedOperands<kHalf, kFirst, KC, kIt>& op) {
+  constexpr int WX = TA + 4;
   constexpr int RW = TA + HX - LX, RN = RW * (TB + HY - LY);
-  for (int t = threadIdx.x; t < RN; t += blockDim.x) {
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int t = (int)threadIdx.x + 256 * it;
     const int x = LX + t % RW, y = LY + t / RW;
     const int a = a0 + x, b = b0 + y, i = 2 * a + (Q & 1), j = 2 * b + (Q >> 1);
-    const int w = (y + 1) * WX + (x + 2);
-    if (a < 0 || b < 0 || i >= c.w || j >= c.h) {
-      if constexpr (kFirst) {
+    const bool in = t < RN && a >= 0 && b >= 0 && i < c.w && j < c.h;
+    const int idx = in ? (int)((long)Q * plane + c.base2 + (long)b * c.w2 + a) : 0;
+    op.w[it] = (y + 1) * WX + (x + 2);
+    op.idx[it] = idx;
+    float c4 = 0.0f;
+    if (in) {
+      if constexpr (kHalf) {
+        const float2 cd = stc[idx];
+        c4 = cd.x, op.corr[it] = 0.125f * cd.y;
+        if (c4 != 0.0f) op.sten[it] = sth[idx];
+      } else {
+        c4 = st[4 * Nd + idx], op.corr[it] = 0.0f;
+        if (c4 != 0.0f) {
 #pragma unroll
-        for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = 0.0f;
+          for (int k = 0; k < 9; k++) op.sten[it].v[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+        }
+      }
+    }
+    op.c4[it] = c4;
+    const bool deep = in && c4 != 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) op.r[it][ch] = deep ? rhs[(long)ch * Nd + idx] : 0.0f;
+    const int tp = in && !deep ? (int)type[idx] : -1;
+    op.state[it] = !in ? 0 : (deep ? 1 : (tp >= 0 ? 2 : 3));
+    if constexpr (kFirst) {
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) {
+        float v = in && !deep ? eIn[(long)ch * Nd + idx] : 0.0f;
+        if constexpr (kProlong)
+          if (in && !deep && tp >= 0) v = v + ProlongTerm(c, cc, pr.cplane, pr.ce + (long)ch * pr.cNd, a, b, Q);
+        op.old[it][ch] = v;
+      }
+    }
+  }
+}
+
+// Colour Q on its region from the fetched operands: deep texels take the Gauss-Seidel value from
+// their neighbours in LDS; the first colour's other texels put their old value into LDS; the
+// tile's own active texels are written to eOut.
+template <bool kHalf, bool kFirst, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
+__device__ __forceinline__ void FusedApply(float* __restrict__ eOut, long Nd, float* sm, const FusedOperands<kHalf, kFirst, KC, kIt>& op) {
+  constexpr int WX = TA + 4, WN = WX * (TB + 2);
+  constexpr int RW = TA + HX - LX;
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int st = op.state[it];
+    const int t = (int)threadIdx.x + 256 * it;
+    const int x = LX + t % RW, y = LY + t / RW;
+    const bool own = x >= 0 && x < TA && y >= 0 && y < TB;
+    const int w = op.w[it];
+    if (st == 0) {  // outside the chart (or past the region): the first colour zeroes its LDS slot
+      if constexpr (kFirst) {
+        if (t < RW * (TB + HY - LY)) {
+#pragma unroll
+          for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = 0.0f;
+        }
       }
       continue;
     }
-    const bool own = x >= 0 && x < TA && y >= 0 && y < TB;
-    const long idx = (long)Q * plane + c.base2 + (long)b * c.w2 + a;
-    float corr, cf[9];
-    const float c4 = GsStencil<kHalf>(idx, Nd, st, stc, sth, corr, cf);
-    if (c4 != 0.0f) {
-      float v[KC][9], r[KC];
+    if (st == 1) {
+      float cf[9];
+      if constexpr (kHalf) Unpack<true>(op.sten[it], cf);
+      else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) cf[k] = op.sten[it].v[k];
+      }
 #pragma unroll
       for (int ch = 0; ch < KC; ch++) {
-        r[ch] = rhs[(long)ch * Nd + idx];
+        float v[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) {
           if (k == 4) {
-            v[ch][k] = 0.0f;
+            v[k] = 0.0f;
             continue;
           }
           // neighbour (i + di, j + dj): plane p, sub-grid offset (da, db) (floor division by 2)
           const int ni = (Q & 1) + k % 3 - 1, nj = (Q >> 1) + k / 3 - 1;
           const int p = (ni & 1) | ((nj & 1) << 1), da = ni >> 1, db = nj >> 1;
-          v[ch][k] = sm[(ch * 4 + p) * WN + w + db * WX + da];
+          v[k] = sm[(ch * 4 + p) * WN + w + db * WX + da];
         }
-      }
-#pragma unroll
-      for (int ch = 0; ch < KC; ch++) {
-        const float e = GsValue<kHalf>(cf, v[ch], c4, corr, r[ch]);
+        const float e = GsValue<kHalf>(cf, v, op.c4[it], op.corr[it], op.r[it][ch]);
         sm[(ch * 4 + Q) * WN + w] = e;
-        if (own) eOut[(long)ch * Nd + idx] = e;
+        if (own) eOut[(long)ch * Nd + op.idx[it]] = e;
       }
-    } else {
-      if constexpr (kFirst) {
+      continue;
+    }
+    if constexpr (kFirst) {
 #pragma unroll
-        for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = eIn[(long)ch * Nd + idx];
-      }
-      if (own && type[idx] >= 0) {
+      for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + Q) * WN + w] = op.old[it][ch];
+    }
+    if (own && st == 2) {
 #pragma unroll
-        for (int ch = 0; ch < KC; ch++) eOut[(long)ch * Nd + idx] = sm[(ch * 4 + Q) * WN + w];
-      }
+      for (int ch = 0; ch < KC; ch++) eOut[(long)ch * Nd + op.idx[it]] = sm[(ch * 4 + Q) * WN + w];
     }
   }
 }
 
-template <bool kHalf, bool kPost, int TA, int TB, int KC, int S>
-__device__ __forceinline__ void FusedStep(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
-                                          const float2* __restrict__ stc, const uint4* __restrict__ sth, const int8_t* __restrict__ type,
-                                          const float* __restrict__ eIn, float* __restrict__ eOut, const float* __restrict__ rhs,
-                                          float* sm) {
-  constexpr int Q = kPost ? 3 - S : S;
-  constexpr FusedBox R = FusedRegion(kPost, Q);
-  FusedColour<kHalf, S == 0, TA, TB, KC, Q, R.lx, R.hx, R.ly, R.hy>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
-}
+template <bool kPost, int TA, int TB, int S>
+struct FusedPhase {
+  static constexpr int Q = kPost ? 3 - S : S;
+  static constexpr FusedBox R = FusedRegion(kPost, Q);
+  static constexpr int RN = (TA + R.hx - R.lx) * (TB + R.hy - R.ly);
+  static constexpr int kIt = (RN + 255) / 256;
+};
 
-template <bool kHalf, bool kPost, int TA, int TB, int KC>
+#define GDTP_FUSED_FETCH(S, OP)                                                                                                       \
+  FusedFetch<kHalf, S == 0, kProlong, TA, TB, KC, FusedPhase<kPost, TA, TB, S>::Q, FusedPhase<kPost, TA, TB, S>::R.lx,                \
+             FusedPhase<kPost, TA, TB, S>::R.hx, FusedPhase<kPost, TA, TB, S>::R.ly, FusedPhase<kPost, TA, TB, S>::R.hy,              \
+             FusedPhase<kPost, TA, TB, S>::kIt>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, rhs, pr, cc, OP)
+#define GDTP_FUSED_APPLY(S, OP)                                                                                                       \
+  FusedApply<kHalf, S == 0, TA, TB, KC, FusedPhase<kPost, TA, TB, S>::Q, FusedPhase<kPost, TA, TB, S>::R.lx,                          \
+             FusedPhase<kPost, TA, TB, S>::R.hx, FusedPhase<kPost, TA, TB, S>::R.ly, FusedPhase<kPost, TA, TB, S>::R.hy,              \
+             FusedPhase<kPost, TA, TB, S>::kIt>(eOut, Nd, sm, OP)
+
+// kProlong (with kPost): the sweep starts post-smoothing and first adds the prolongation of the
+// coarse correction to every active texel it reads (what KMixProlongAdd would have written).
+// Every global load of a colour is issued one colour ahead (the next colour's stencil, rhs and
+// old values are fetched into registers before the current colour is computed from LDS), so
+// each barrier waits on at most one colour's compute.
+template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, int KC>
 __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                        long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
                                                        const uint4* __restrict__ sth, const int8_t* __restrict__ type,
                                                        const float* __restrict__ eIn, float* __restrict__ eOut,
-                                                       const float* __restrict__ rhs) {
-  constexpr int WX = TA + 4, WN = WX * (TB + 2);
+                                                       const float* __restrict__ rhs, FusedProlong pr) {
+  constexpr int WX = TA + 4, WN = WX * (TB + 2), kLoads = (3 * WN + 255) / 256;
   __shared__ float sm[KC * 4 * WN];
   const int* T = tiles + 3 * blockIdx.x;
   const MixChart c = charts[T[0]];
+  MixChart cc{};
+  if constexpr (kProlong) cc = pr.cch[T[0]];
   const int a0 = T[1], b0 = T[2];
+  FusedOperands<kHalf, true, KC, FusedPhase<kPost, TA, TB, 0>::kIt> op0;
+  FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
+  GDTP_FUSED_FETCH(0, op0);
   // old values of the three planes coloured after the first (pre: 1, 2, 3; post: 2, 1, 0)
-  for (int t = threadIdx.x; t < 3 * WN; t += blockDim.x) {
+  float v[kLoads][KC];
+#pragma unroll
+  for (int it = 0; it < kLoads; it++) {
+    const int t = (int)threadIdx.x + 256 * it;
     const int q = kPost ? 2 - t / WN : 1 + t / WN, w = t % WN;
     const int a = a0 + w % WX - 2, b = b0 + w / WX - 1, i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
-    float v[KC];
-    if (a >= 0 && b >= 0 && i < c.w && j < c.h) {
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) v[it][ch] = 0.0f;
+    if (t < 3 * WN && a >= 0 && b >= 0 && i < c.w && j < c.h) {
       const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
 #pragma unroll
-      for (int ch = 0; ch < KC; ch++) v[ch] = eIn[(long)ch * Nd + idx];
-    } else {
+      for (int ch = 0; ch < KC; ch++) v[it][ch] = eIn[(long)ch * Nd + idx];
+      if constexpr (kProlong) {
+        if (type[idx] >= 0) {
 #pragma unroll
-      for (int ch = 0; ch < KC; ch++) v[ch] = 0.0f;
+          for (int ch = 0; ch < KC; ch++) v[it][ch] = v[it][ch] + ProlongTerm(c, cc, pr.cplane, pr.ce + (long)ch * pr.cNd, a, b, q);
+        }
+      }
     }
-#pragma unroll
-    for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + q) * WN + w] = v[ch];
   }
+#pragma unroll
+  for (int it = 0; it < kLoads; it++) {
+    const int t = (int)threadIdx.x + 256 * it;
+    if (t >= 3 * WN) continue;
+    const int q = kPost ? 2 - t / WN : 1 + t / WN, w = t % WN;
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + q) * WN + w] = v[it][ch];
+  }
+  GDTP_FUSED_FETCH(1, op1);
   __syncthreads();
-  FusedStep<kHalf, kPost, TA, TB, KC, 0>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
-  __syncthreads();
-  FusedStep<kHalf, kPost, TA, TB, KC, 1>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
-  __syncthreads();
-  FusedStep<kHalf, kPost, TA, TB, KC, 2>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
-  __syncthreads();
-  FusedStep<kHalf, kPost, TA, TB, KC, 3>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, eOut, rhs, sm);
+  GDTP_FUSED_APPLY(0, op0);
+  {
+    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 2>::kIt> op2;
+    GDTP_FUSED_FETCH(2, op2);
+    __syncthreads();
+    GDTP_FUSED_APPLY(1, op1);
+    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 3>::kIt> op3;
+    GDTP_FUSED_FETCH(3, op3);
+    __syncthreads();
+    GDTP_FUSED_APPLY(2, op2);
+    __syncthreads();
+    GDTP_FUSED_APPLY(3, op3);
+  }
 }
+#undef GDTP_FUSED_FETCH
+#undef GDTP_FUSED_APPLY
 
 // res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4).  A workgroup
 // covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
@@ -523,8 +665,6 @@ __global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __res
   else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, fres, fNd, crhs, ce, cNd, K);
 }
 
-__device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
-
 // e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid).  A thread
 // owns one fine texel of each of the four colour planes, (2a + qi, 2b + qj); the coarse neighbours of
 // all four lie in one 2x2 coarse block (columns ci0, ci0 + 1, rows cj0, cj0 + 1), loaded once where an
@@ -589,7 +729,8 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
         const float v10 = ox ? (sy ? V[3] : V[1]) : v00;
         const float v01 = oy ? b1 : v00;
         const float v11 = (ox && oy) ? V[3] : (ox ? v10 : v01);
-        fe[(long)(c0 + ch) * fNd + fidx[q] / 4] = old[ch][q] + (v00 + v10 + v01 + v11) * 0.25f;
+        const float term = (v00 + v10 + v01 + v11) * 0.25f;  // exact scaling: same value as ProlongTerm
+        fe[(long)(c0 + ch) * fNd + fidx[q] / 4] = old[ch][q] + term;
       }
     }
   });
@@ -1257,7 +1398,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
           }
       }
     }
-    d.e = Zeros<float>(ms, (size_t)d.Nd * kcap);
+    d.e = d.eHome = Zeros<float>(ms, (size_t)d.Nd * kcap);
     if (d.ftiles[0]) d.e2 = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.rhs = Zeros<float>(ms, (size_t)d.Nd * kcap);
     d.res = Zeros<float>(ms, (size_t)d.Nd * kcap);
@@ -1303,14 +1444,16 @@ static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
 
 // One fused four-colour sweep (KMixDeepGsFused) over the level's tiles of shape `shape`, columns in
 // chunks of up to four; e and e2 swap afterwards.
-template <bool kHalf, bool kPost, int TA, int TB>
-static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s) {
+template <bool kHalf, bool kPost, bool kProlong, int TA, int TB>
+static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
   for (int c0 = 0; c0 < K; c0 += 4) {
     const long off = (long)c0 * d.Nd;
-#define GDTP_FUSED(KC)                                                                                                              \
-  KMixDeepGsFused<kHalf, kPost, TA, TB, KC><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, d.Nd, d.stencil,     \
-                                                                             d.stencilC, d.stencilH, d.type, d.e + off, d.e2 + off, \
-                                                                             d.rhs + off)
+    FusedProlong p = pr;
+    if (p.ce) p.ce += (long)c0 * p.cNd;
+#define GDTP_FUSED(KC)                                                                                                               \
+  KMixDeepGsFused<kHalf, kPost, kProlong, TA, TB, KC><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, d.Nd,       \
+                                                                                       d.stencil, d.stencilC, d.stencilH, d.type,    \
+                                                                                       d.e + off, d.e2 + off, d.rhs + off, p)
     switch (std::min(4, K - c0)) {
       case 4: GDTP_FUSED(4); break;
       case 3: GDTP_FUSED(3); break;
@@ -1322,24 +1465,37 @@ static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s) {
   std::swap(d.e, d.e2);
 }
 
-template <bool kHalf, bool kPost>
-static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s) {
+template <bool kHalf, bool kPost, bool kProlong>
+static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
   static_assert(kFusedShapes == 4, "one case per fused tile shape");
   switch (shape) {
-    case 0: FusedLaunch<kHalf, kPost, kFusedShape[0][0], kFusedShape[0][1]>(d, 0, K, s); break;
-    case 1: FusedLaunch<kHalf, kPost, kFusedShape[1][0], kFusedShape[1][1]>(d, 1, K, s); break;
-    case 2: FusedLaunch<kHalf, kPost, kFusedShape[2][0], kFusedShape[2][1]>(d, 2, K, s); break;
-    default: FusedLaunch<kHalf, kPost, kFusedShape[3][0], kFusedShape[3][1]>(d, 3, K, s); break;
+    case 0: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[0][0], kFusedShape[0][1]>(d, 0, K, s, pr); break;
+    case 1: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[1][0], kFusedShape[1][1]>(d, 1, K, s, pr); break;
+    case 2: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[2][0], kFusedShape[2][1]>(d, 2, K, s, pr); break;
+    default: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[3][0], kFusedShape[3][1]>(d, 3, K, s, pr); break;
   }
 }
 
+// Whether the fused sweep of level d runs (a fused tile shape with tiles on this level).
+static bool Fused(const MixedSystem& ms, const DevMixLevel& d) {
+  return ms.tileShape >= 0 && !d.chartSweeps && d.nFTiles[ms.tileShape] > 0;
+}
+
 // One deep four-colour sweep of a tiled level (colours 0..3, post: 3..0): fused when the system
-// uses a fused tile shape, else one launch per colour.  Returns the launches made.
-static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStream_t s) {
+// uses a fused tile shape, else one launch per colour.  With `prolong` (post, fused only) the
+// sweep first adds the prolongation of the coarse correction.  Returns the launches made.
+static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStream_t s, const FusedProlong* prolong = nullptr) {
   const int shape = ms.tileShape;
-  if (shape >= 0 && d.nFTiles[shape] > 0) {
-    if (d.stencilH) post ? FusedShapeLaunch<true, true>(d, shape, K, s) : FusedShapeLaunch<true, false>(d, shape, K, s);
-    else post ? FusedShapeLaunch<false, true>(d, shape, K, s) : FusedShapeLaunch<false, false>(d, shape, K, s);
+  if (Fused(ms, d)) {
+    const FusedProlong pr = prolong ? *prolong : FusedProlong();
+    if (prolong) {
+      if (d.stencilH) FusedShapeLaunch<true, true, true>(d, shape, K, s, pr);
+      else FusedShapeLaunch<false, true, true>(d, shape, K, s, pr);
+    } else if (d.stencilH) {
+      post ? FusedShapeLaunch<true, true, false>(d, shape, K, s, pr) : FusedShapeLaunch<true, false, false>(d, shape, K, s, pr);
+    } else {
+      post ? FusedShapeLaunch<false, true, false>(d, shape, K, s, pr) : FusedShapeLaunch<false, false, false>(d, shape, K, s, pr);
+    }
     return (K + 3) / 4;
   }
   for (int qq = 0; qq < 4; qq++) DeepGsLaunch(d, post ? 3 - qq : qq, K, s);
@@ -1364,7 +1520,8 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
 // 3..0) x nuPost, then one boundary Schwarz step (a Schwarz step after every post sweep converged
 // marginally faster per V-cycle but cost more per digit: C2 15.1 against 13.9 ms per solve).
 // zeroE: e is zero on the whole level when pre-smoothing starts.
-static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool zeroE = false) {
+// prolong (post-smoothing of a fused level): the first sweep adds the coarse correction itself.
+static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool zeroE = false, const FusedProlong* prolong = nullptr) {
   DevMixLevel& d = ms.L[l];
   const int nu = post ? ms.nuPost : ms.nuPre;
   for (int it = 0; it < nu; it++) {
@@ -1373,7 +1530,7 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool
       if (d.stencilH) KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
       else KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
     } else if (d.nTiles > 0) {
-      DeepSweep(ms, d, post, K, s);
+      DeepSweep(ms, d, post, K, s, it == 0 ? prolong : nullptr);
     }
     if (post && it == nu - 1) BoundarySweeps(ms, d, K, s, false);
   }
@@ -1426,10 +1583,14 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
   }
   for (int l = nl - 2; l >= 0; l--) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
-    if (d.nTiles > 0)
+    // the first post-smoothing sweep of a fused level adds the prolongation itself
+    const bool fuse = ms.fuseProlong && ms.nuPost > 0 && Fused(ms, d);
+    if (d.nTiles > 0 && !fuse)
       KMixProlongAdd<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K,
                                               ms.nuPost > 0 ? 3 : -1);
-    Smooth(ms, l, K, true, s);
+    FusedProlong pr;
+    pr.cch = c.charts, pr.cplane = c.plane, pr.cNd = c.Nd, pr.ce = c.e;
+    Smooth(ms, l, K, true, s, false, fuse ? &pr : nullptr);
   }
   (void)ds;
 }
@@ -1576,6 +1737,15 @@ __global__ __launch_bounds__(256) void KFcgUpdate(int N, const int* __restrict__
   BlockAddSquares(a, norms + (long)ch * kSlots);
 }
 
+// The fused sweeps swap a level's two correction buffers.  Level 0's correction is consumed (x += e,
+// or z = e in flexible CG) at the end of a refinement step and zeroed for the next step; that zeroing
+// goes to the buffer the level was allocated with, so that every step starts on the same buffer —
+// a captured step replays with the pointers it was captured with, whatever the parity of its sweeps.
+static void HomeLevel0(MixedSystem& ms) {
+  DevMixLevel& d = ms.L[0];
+  if (d.e2 && d.e != d.eHome) std::swap(d.e, d.e2);
+}
+
 // residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
 // bnorms (first residual of a solve): |b|^2 per column into bnorms[ch][slot] as well.
 static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s,
@@ -1623,6 +1793,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
   K64Dot<<<dim3(gb, K), 256, 0, s>>>(N, ms.p64, ms.q64, pq);
   KFcgAlpha<<<1, cb, 0, s>>>(K, (int)gb, rp, pq, ms.alpha, ms.pqPrev, ms.first);
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  HomeLevel0(ms);  // e was gathered above; the update zeroes the buffer the next step starts on
   KFcgUpdate<<<dim3(NBlk(N, 256), K), 256, 0, s>>>(N, d.g2d, ms.alpha, ms.p64, ms.q64, x, ms.r64, d.rhs, d.e, d.Nd, ms.norms);
 }
 
@@ -1656,6 +1827,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
   if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
+  HomeLevel0(ms);
   OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr, bNorms);  // |b|^2 in the same pass
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
@@ -1686,6 +1858,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
       FcgStep(ds, ms, x, K, st);
     } else {
       OuterUpdate(ds, ms, x, K, st);
+      HomeLevel0(ms);  // the residual zeroes the buffer the next step starts on
       OuterResidual(ds, ms, x, b, K, weighted, st);
     }
     GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, (size_t)K * kSlots * sizeof(double), hipMemcpyDeviceToHost, st));

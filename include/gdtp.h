@@ -31,6 +31,7 @@ enum { GDTP_METRIC_EMBEDDING = 0, GDTP_METRIC_UNIFORM = 1 };
 enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS = 2 };
 enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
 enum { GDTP_ASSEMBLY_DEVICE = 1 };
+enum { GDTP_FUSE_PROLONGATION = 1 };
 
 typedef struct gdtp_mesh gdtp_mesh;
 typedef struct gdtp_hierarchy gdtp_hierarchy;
@@ -209,6 +210,12 @@ int gdtp_mixed_cycles(gdtp_system* s, int cycles);
  * bitwise the same iterate.  shape -2 only reports; `current` (may be NULL) receives the shape in
  * use.  No counterpart in the reference. */
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
+/* Transfers fused into the mixed solver's relaxation kernels: GDTP_FUSE_PROLONGATION (the
+ * default) lets the first post-smoothing sweep of every level swept by fused tiles add the bilinear
+ * prolongation of the coarse correction itself, instead of a separate pass over the level; 0 keeps
+ * the separate pass.  Both give bitwise the same iterate.  No counterpart in the reference
+ * (AddProlongation then Relaxation, IterativeSolvers.inl:1122-1168, 174-395). */
+int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */
 int gdtp_mixed_deep_sweep(gdtp_system* s, int level, int ncols, void* stream);

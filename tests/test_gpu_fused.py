@@ -1,5 +1,6 @@
 """Fused four-colour deep sweep (one launch per sweep, tile and halo staged in LDS, ping-pong
-correction buffers) against one launch per colour: every tile shape gives bitwise the same iterate
+correction buffers, optionally adding the prolongation at the start of post-smoothing) against one
+launch per colour and a separate prolongation pass: every tile shape gives bitwise the same iterate
 of the mixed solver, on screened operators (FP16 smoother stencil) and on a nearly singular one
 (FP32 smoother stencil), for column counts that fill one and two column chunks.  The FP32 fallback
 for FP16-range stencils (forced by lowering the FP16 ratio limit) solves to the direct solution."""
@@ -18,8 +19,9 @@ torch = pytest.importorskip("torch")
 import gradientdomaintextureprocessing_amd as g  # noqa: E402
 
 
-def iterate(S, b, shape, iters, use_graph):
+def iterate(S, b, shape, iters, use_graph, fuse_prolongation=False):
     S.tile_shape(shape)
+    S.fuse_transfers(prolongation=fuse_prolongation)
     S.configure_mixed(use_graph=use_graph)
     x = torch.zeros_like(b)
     try:
@@ -30,7 +32,7 @@ def iterate(S, b, shape, iters, use_graph):
     return x.cpu().numpy()
 
 
-@pytest.mark.parametrize("case", ["icosphere512", "cube256-singular"])
+@pytest.mark.parametrize("case", ["icosphere512", "cube512-singular"])
 @pytest.mark.parametrize("K", [3, 6])
 def test_fused_sweep_is_bitwise_four_launches(case, K):
     if not torch.cuda.is_available():
@@ -40,8 +42,8 @@ def test_fused_sweep_is_bitwise_four_launches(case, K):
         H = g.InitializeHierarchy(mesh, 512, 512, 5)
         S = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=K)
     else:  # the geodesic distance operator 1e-6 M + S keeps the FP32 smoother stencil
-        mesh = g.TexturedMesh.cube(16, 256, 256)
-        H = g.InitializeHierarchy(mesh, 256, 256, 4)
+        mesh = g.TexturedMesh.cube(16, 512, 512)
+        H = g.InitializeHierarchy(mesh, 512, 512, 5)
         S = g.UpdateLinearSystem(H, 1.0, 1e-6, max_cols=K)
     N = H.level_info(0)["texels"]
     b = torch.from_numpy(np.random.default_rng(K).standard_normal((K, N))).cuda()
@@ -53,7 +55,10 @@ def test_fused_sweep_is_bitwise_four_launches(case, K):
             print(f"[fused {case} K={K}] shape {shape} graph {use_graph}", flush=True)
             got = iterate(S, b, shape, 2, use_graph)
             assert np.array_equal(got, want), f"shape {shape}, graph {use_graph}: max diff {np.abs(got - want).max()}"
-    # a single deep sweep through the C ABI reports the fused launch (no error on a tiled level)
+            # the prolongation fused into the first post-smoothing sweep changes nothing either
+            got = iterate(S, b, shape, 2, use_graph, fuse_prolongation=True)
+            assert np.array_equal(got, want), f"shape {shape}, graph {use_graph}, fused prolongation: max diff {np.abs(got - want).max()}"
+    # level 0 is swept by tiles (charts too large for one workgroup), so the fused kernel ran
     S.tile_shape(0)
     S.mixed_deep_sweep(0, K)
     torch.cuda.synchronize()

@@ -108,14 +108,15 @@ def test_lic_reference_rounds_alone_and_batched(lic):
 def test_lic_sixteen_lengths_as_columns(lic):
     """C4 as configured: 16 log-spaced diffusion lengths, one per column, each with its own
     operator M_a + l_c S_a, on seeded float32 noise, the reference's 4 rounds (weighted flexible CG,
-    one V-cycle per round); every column's residual against its own operator falls, is smaller than
-    against the system's own length at the extremes, and column 3 solved alone equals the batch."""
+    one V-cycle per round, the V-cycle of the nearest power of ten); every column's residual against its own operator
+    falls, is smaller than against the LIC system's own length at the extremes, and column 3
+    solved alone equals the batch."""
     N = lic.anisotropic.level_info(0)["texels"]
     noise = torch.from_numpy(np.random.default_rng(11).random((16, N), dtype=np.float32).astype(np.float64)).cuda()
     lengths = np.logspace(-5, -3, 16)
     x, rel = lic.convolve_lengths(noise, lengths, rounds=4)
     print(f"[lic 2048] 16 lengths, rel res after 4 rounds: {np.array2string(np.asarray(rel), precision=2)}", flush=True)
-    assert np.all(np.isfinite(rel)) and np.all(rel < 0.5)
+    assert np.all(np.isfinite(rel)) and np.all(rel < 0.6)
     b = torch.empty_like(noise)
     lic.lic.apply(g.OPERATOR_MASS, noise, b)
     wrong = lic._weighted_residuals(x, b, np.full(16, 1e-4))
