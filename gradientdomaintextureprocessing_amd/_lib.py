@@ -57,6 +57,7 @@ SIGNATURES = {
     "gdtp_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "gdtp_residual_norms": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "gdtp_residual_norms_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "gdtp_direct_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_solve_weighted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int, c_void_p, P(c_int), c_void_p]),
     "gdtp_geodesic_divergence": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),

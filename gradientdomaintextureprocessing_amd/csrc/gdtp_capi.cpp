@@ -543,6 +543,17 @@ int gdtp_direct_solve(gdtp_system* s, double* x, const double* b, int ncols, dou
   })
 }
 
+int gdtp_residual_norms_noalloc(gdtp_system* s, const double* x, const double* b, int ncols, double* rel_res, double* norms,
+                                void* stream) {
+  if (s && s->ds && ncols > kNoAllocMaxCols)
+    return Fail(GDTP_ERROR_INVALID, "ncols exceeds the 64 columns of the no-allocation scratch");
+  GDTP_DEVICE_OP(x && b, {
+    double sq[2 * kNoAllocMaxCols];
+    ResidualNormsNoAlloc(ds, x, b, ncols, ds.L[0].N, sq, st);
+    FinishNorms(sq, ncols, rel_res, norms);
+  })
+}
+
 static MixedSystem& Mixed(gdtp_system* s) {
   if (!s->mixed) {
     std::unique_ptr<MixedSystem> m(new MixedSystem());

@@ -104,6 +104,10 @@ void ResidualLevel(DeviceSystem& ds, int level, const double* x, const double* b
 constexpr int kNormParts = 512;
 constexpr size_t NormPartsSize(int K) { return (size_t)2 * K * kNormParts; }
 void ResidualNorms(DeviceSystem& ds, const double* x, const double* b, int K, int ld, double* parts, double* out, hipStream_t s);
+// The same with the scratch in a static device array (no allocation): at most kNoAllocMaxCols
+// columns; the 2 K squared norms are copied to hostSquares and the stream is synchronised.
+constexpr int kNoAllocMaxCols = 64;
+void ResidualNormsNoAlloc(DeviceSystem& ds, const double* x, const double* b, int K, int ld, double* hostSquares, hipStream_t s);
 
 // Normalised-gradient integration samples on the device (cell samples SoA [13][n]).
 struct DevVectorField {

@@ -6,6 +6,8 @@
 
 #include "gdtp_device.h"
 
+#include <mutex>
+
 namespace gdtp {
 
 static constexpr int kWave = 64;
@@ -1009,6 +1011,39 @@ void ResidualNorms(DeviceSystem& ds, const double* x, const double* b, int K, in
 #undef GDTP_NORM_PARTS
   }
   KNormPartsSum<<<2 * K, 256, 0, s>>>(parts, kNormParts, out);
+}
+
+// ------------------------------------------------------------------ residual norms without allocation
+// For callers that cannot allocate device memory: the partial sums and squared norms of
+// ResidualNorms live in this file-scope array (one copy per GPU, held until the process ends).
+// Host code never passes `scratch` itself to a kernel: its device address, looked up once per
+// device with hipGetSymbolAddress, is kept in g_scratchAddr.  Every call on one GPU shares the
+// array, so calls that overlap on one GPU race; each call overwrites every entry it reads.
+static __device__ __align__(256) double scratch[NormPartsSize(kNoAllocMaxCols) + 2 * kNoAllocMaxCols];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratchLock;
+static double* g_scratchAddr[kMaxDevices] = {};
+
+static double* ScratchAddress() {
+  int dev = 0;
+  GDTP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal beyond the no-allocation scratch table");
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  if (!g_scratchAddr[dev]) {
+    void* p = nullptr;
+    GDTP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchAddr[dev] = static_cast<double*>(p);
+  }
+  return g_scratchAddr[dev];
+}
+
+void ResidualNormsNoAlloc(DeviceSystem& ds, const double* x, const double* b, int K, int ld, double* hostSquares, hipStream_t s) {
+  if (K < 1 || K > kNoAllocMaxCols) throw std::invalid_argument("column count outside the no-allocation scratch");
+  double* p = ScratchAddress();
+  double* out = p + NormPartsSize(kNoAllocMaxCols);
+  ResidualNorms(ds, x, b, K, ld, p, out, s);
+  GDTP_HIP(hipMemcpyAsync(hostSquares, out, 2 * (size_t)K * sizeof(double), hipMemcpyDeviceToHost, s));
+  GDTP_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace gdtp

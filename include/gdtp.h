@@ -130,6 +130,13 @@ int gdtp_residual_norms(gdtp_system* s, const double* x, const double* b, int nc
  * until the relative residual reaches 1e-14 or stops falling (the FP64 floor).  x is overwritten;
  * rel_res (host, ncols) and method (host: 0 factorisation, 1 V-cycles) may be NULL. */
 int gdtp_direct_solve(gdtp_system* s, double* x, const double* b, int ncols, double* rel_res, int* method, void* stream);
+/* gdtp_residual_norms for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): no counterpart in the reference.  Its
+ * partial sums live in a static device array of the library (525 KB per GPU, held until the
+ * process ends) instead of a buffer allocated on first use, so at most 64 columns
+ * (GDTP_ERROR_INVALID beyond).  Calls that overlap on one GPU share that array and race. */
+int gdtp_residual_norms_noalloc(gdtp_system* s, const double* x, const double* b, int ncols, double* rel_res, double* norms,
+                                void* stream);
 
 /* Normalised-gradient divergence of the heat-method geodesics: for K potentials u (level-0
  * columns), rhs[:, t] = integral < -grad u / |grad u|_g , grad phi_t >_g over the surface, one
