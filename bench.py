@@ -78,7 +78,7 @@ def mixed_roofline(S, H, K, reps=10):
     e written) with the declared FP16-ratio stencil s*q = 8*2 + 2*4 = 24 bytes and p = 4."""
     import torch
     info = H.level_info(0)
-    shape = S.tile_shape()
+    shape = S.tile_shape(level=0)
     fused = shape >= 0
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -155,8 +155,9 @@ def profile_kernel(cfg, K, which, reps=10):
     for _ in range(4):
         dst.copy_(src)
     torch.cuda.synchronize()
-    mesh, H, S = make_system(cfg, K, mixed=(which == "mixed"))
-    if which == "mixed":
+    mesh, H, S = make_system(cfg, K, mixed=(which != "band"))
+    if which != "band":  # "mixed": one launch per colour; "fused": the fused sweep on 64 x 4 tiles
+        S.tile_shape(-1 if which == "mixed" else 1)
         f, b = texture_rhs(H, S, K)
         x = torch.zeros_like(b)
         solve(S, x, b, 1e-6, 40, "mixed")  # leaves meaningful FP32 vectors in the solver's buffers
@@ -511,7 +512,7 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--serving-leg", default="on", choices=["on", "off"],
                     help="short request-loop leg (three meshes, 20 QPS) reported beside the solve throughput")
-    ap.add_argument("--profile-kernel", default=None, choices=["mixed", "band"])
+    ap.add_argument("--profile-kernel", default=None, choices=["mixed", "fused", "band"])
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
     ap.add_argument("--workload", default=None, choices=["c3", "c4", "assembly"],
                     help="other workloads: c3 batched geodesics, c4 line integral convolution, assembly = mesh admission "
@@ -549,8 +550,8 @@ def main():
         tiles = None
         if mode == "mixed":  # pick the fused deep-sweep tile shape on this system (level 0, K columns)
             solve(S, x, b, args.tol, max_cycles, mode)
-            tiles = {str(S.TILE_SHAPES[k]) if k >= 0 else "per-colour": round(v, 2) for k, v in S.autotune_tiles(0, K).items()}
-            log(f"[bench] tile shapes (us per level-0 sweep, fastest first): {tiles}; using {S.TILE_SHAPES[S.tile_shape()]}")
+            tiles = {f"level {l}": {str(S.TILE_SHAPES[k]): round(v, 2) for k, v in t.items()} for l, t in S.autotune_tiles(K).items()}
+            log(f"[bench] tile shapes per level (us per sweep, fastest first): {tiles}")
         for _ in range(args.warmup):
             cycles, rel = solve(S, x, b, args.tol, max_cycles, mode)
             log(f"[bench] {mode} warmup: {cycles} iterations, rel res {rel:.2e}")

@@ -76,6 +76,7 @@ SIGNATURES = {
     "gdtp_mixed_acceleration": (c_int, [c_void_p, c_int]),
     "gdtp_mixed_deep_sweep": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "gdtp_mixed_tile_shape": (c_int, [c_void_p, c_int, P(c_int)]),
+    "gdtp_mixed_level_tile_shape": (c_int, [c_void_p, c_int, c_int, P(c_int)]),
     "gdtp_mixed_fuse_transfers": (c_int, [c_void_p, c_int]),
     "gdtp_system_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),
 }
@@ -89,15 +90,20 @@ class GdtpError(RuntimeError):
         self.code = code
 
 
+def built_id():
+    """The source identifier embedded in libgdtp.so, read from the file (the library is not loaded:
+    a process that loaded a library keeps that copy even after the file is rebuilt)."""
+    if not os.path.exists(LIB_PATH):
+        return None
+    with open(LIB_PATH, "rb") as fh:
+        data = fh.read()
+    at = data.find(b"gdtp-build-id:")
+    return data[at + 14:at + 30].decode(errors="replace") if at >= 0 else None
+
+
 def is_current():
     """True when libgdtp.so exists and was built from the sources in csrc/ as they are now."""
-    if not os.path.exists(LIB_PATH):
-        return False
-    lib = ctypes.CDLL(LIB_PATH)
-    if not hasattr(lib, "gdtp_build_id"):
-        return False
-    lib.gdtp_build_id.restype = c_char_p
-    return lib.gdtp_build_id().decode() == source_hash()
+    return built_id() == source_hash()
 
 
 def load():

@@ -430,12 +430,17 @@ class MultigridSystem:
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
 
     # fused deep-sweep tile shapes (sub-grid texels): index -> (width, height); -1 = one launch per colour
-    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), -1: (64, 4)}
+    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), -1: "per-colour"}
 
-    def tile_shape(self, shape=None):
-        """Set (0..3 fused, -1 one launch per colour) and/or return the deep-sweep tile shape."""
+    def tile_shape(self, shape=None, level=None):
+        """Set (0..3 fused, -1 one launch per colour) and/or return the deep-sweep tile shape of
+        every level, or of one level."""
         cur = ctypes.c_int(0)
-        _lib.check(_lib.load().gdtp_mixed_tile_shape(self._h, -2 if shape is None else int(shape), ctypes.byref(cur)))
+        arg = -2 if shape is None else int(shape)
+        if level is None:
+            _lib.check(_lib.load().gdtp_mixed_tile_shape(self._h, arg, ctypes.byref(cur)))
+        else:
+            _lib.check(_lib.load().gdtp_mixed_level_tile_shape(self._h, int(level), arg, ctypes.byref(cur)))
         return cur.value
 
     def fuse_transfers(self, prolongation=True):
@@ -443,27 +448,34 @@ class MultigridSystem:
         separate pass; the iterate is bitwise the same."""
         _lib.check(_lib.load().gdtp_mixed_fuse_transfers(self._h, 1 if prolongation else 0))
 
-    def autotune_tiles(self, level=0, columns=None, sweeps=20, shapes=(0, 1, 2, 3, -1)):
-        """Time `sweeps` deep sweeps of `level` for each tile shape and keep the fastest (every shape
-        gives the same iterate).  Returns {shape: microseconds per sweep}, fastest first."""
+    def autotune_tiles(self, columns=None, sweeps=20, shapes=(0, 1, 2, 3, -1)):
+        """Time `sweeps` deep sweeps of every level swept by tiles for each tile shape and keep
+        each level's fastest (every shape gives the same iterate).  Returns
+        {level: {shape: microseconds per sweep}}, fastest first."""
         import torch
         K = columns or self.max_cols
         st = torch.cuda.current_stream()
-        times = {}
-        for sh in shapes:
-            self.tile_shape(sh)
-            self.mixed_deep_sweep(level, K)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            for _ in range(sweeps):
-                self.mixed_deep_sweep(level, K)
-            e1.record(st)
-            torch.cuda.synchronize()
-            times[sh] = e0.elapsed_time(e1) * 1e3 / sweeps
-        best = min(times, key=times.get)
-        self.tile_shape(best)
-        return dict(sorted(times.items(), key=lambda kv: kv[1]))
+        out = {}
+        for level in range(self.hierarchy.levels - 1):
+            times = {}
+            try:
+                for sh in shapes:
+                    self.tile_shape(sh, level)
+                    self.mixed_deep_sweep(level, K)
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(st)
+                    for _ in range(sweeps):
+                        self.mixed_deep_sweep(level, K)
+                    e1.record(st)
+                    torch.cuda.synchronize()
+                    times[sh] = e0.elapsed_time(e1) * 1e3 / sweeps
+            except _lib.GdtpError:  # a level swept one chart per workgroup has no tiles
+                continue
+            best = min(times, key=times.get)
+            self.tile_shape(best, level)
+            out[level] = dict(sorted(times.items(), key=lambda kv: kv[1]))
+        return out
 
     def memory(self):
         r, m = ctypes.c_size_t(), ctypes.c_size_t()

@@ -65,6 +65,7 @@ struct gdtp_system {
   std::unique_ptr<DirectFactor> direct;  // built by the first gdtp_direct_solve after create / update
   int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = 0, fuseTransfers = 1;
   double omega = 1.0;
+  std::vector<int> levelShape;  // per-level deep-sweep tile shapes set since the last all-level setting
   double* dNorms = nullptr;
   double* hNorms = nullptr;
   double* dNormParts = nullptr;  // residual-norm partial sums, NormPartsSize(kcap) + 2 kcap doubles
@@ -98,7 +99,10 @@ const char* gdtp_last_error(void) { return g_error.c_str(); }
 #ifndef GDTP_BUILD_ID
 #define GDTP_BUILD_ID "unknown"
 #endif
-const char* gdtp_build_id(void) { return GDTP_BUILD_ID; }
+// kept whole in the library's read-only data, so that the identifier can be read from the file
+// without loading it
+static const char kBuildTag[] = "gdtp-build-id:" GDTP_BUILD_ID;
+const char* gdtp_build_id(void) { return kBuildTag + 14; }
 
 int gdtp_device_count(int* count) {
   if (!count) return Fail(GDTP_ERROR_INVALID, "null count");
@@ -562,7 +566,8 @@ static MixedSystem& Mixed(gdtp_system* s) {
   }
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
-  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles, m.tileShape = s->tileShape;
+  m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles;
+  for (int l = 0; l < m.levels; l++) m.L[l].shape = l < (int)s->levelShape.size() ? s->levelShape[l] : s->tileShape;
   m.fuseProlong = (s->fuseTransfers & GDTP_FUSE_PROLONGATION) != 0;
   return m;
 }
@@ -579,14 +584,30 @@ int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags) {
 
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
   if (!s || shape < -2 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-2, 4)");
-  if (shape >= -1 && shape != s->tileShape) {
+  if (shape >= -1) {
     s->tileShape = shape;
+    s->levelShape.clear();
     if (s->mixed) {
-      s->mixed->tileShape = shape;
+      for (DevMixLevel& d : s->mixed->L) d.shape = shape;
       InvalidateGraph(*s->mixed);
     }
   }
   if (current) *current = s->tileShape;
+  return GDTP_OK;
+}
+
+int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current) {
+  if (!s || !s->ds || level < 0 || level >= s->ds->levels || shape < -2 || shape >= kFusedShapes)
+    return Fail(GDTP_ERROR_INVALID, "level or tile shape out of range");
+  if (s->levelShape.empty()) s->levelShape.assign(s->ds->levels, s->tileShape);
+  if (shape >= -1 && shape != s->levelShape[level]) {
+    s->levelShape[level] = shape;
+    if (s->mixed) {
+      s->mixed->L[level].shape = shape;
+      InvalidateGraph(*s->mixed);
+    }
+  }
+  if (current) *current = s->levelShape[level];
   return GDTP_OK;
 }
 

@@ -31,6 +31,7 @@ struct DevMixLevel {
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
   int* ftiles[kFusedShapes] = {};  // the same for each fused-sweep tile shape
   int nFTiles[kFusedShapes] = {};
+  int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, -1: one launch per colour)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;
   int *bRowptr = nullptr, *bCol = nullptr;
@@ -55,7 +56,6 @@ struct DevMixLevel {
 struct MixedSystem {
   int levels = 0, kcap = 0;
   int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per Schwarz step
-  int tileShape = 0;                              // fused deep-sweep tile shape (kFusedShape), -1: one launch per colour
   bool fuseProlong = true;                        // fused levels: the first post sweep adds the prolongation itself
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;

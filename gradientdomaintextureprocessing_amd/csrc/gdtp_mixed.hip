@@ -1477,16 +1477,14 @@ static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, Fu
 }
 
 // Whether the fused sweep of level d runs (a fused tile shape with tiles on this level).
-static bool Fused(const MixedSystem& ms, const DevMixLevel& d) {
-  return ms.tileShape >= 0 && !d.chartSweeps && d.nFTiles[ms.tileShape] > 0;
-}
+static bool Fused(const DevMixLevel& d) { return d.shape >= 0 && !d.chartSweeps && d.nFTiles[d.shape] > 0; }
 
 // One deep four-colour sweep of a tiled level (colours 0..3, post: 3..0): fused when the system
 // uses a fused tile shape, else one launch per colour.  With `prolong` (post, fused only) the
 // sweep first adds the prolongation of the coarse correction.  Returns the launches made.
 static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStream_t s, const FusedProlong* prolong = nullptr) {
-  const int shape = ms.tileShape;
-  if (Fused(ms, d)) {
+  const int shape = d.shape;
+  if (Fused(d)) {
     const FusedProlong pr = prolong ? *prolong : FusedProlong();
     if (prolong) {
       if (d.stencilH) FusedShapeLaunch<true, true, true>(d, shape, K, s, pr);
@@ -1584,7 +1582,7 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
   for (int l = nl - 2; l >= 0; l--) {
     DevMixLevel &d = ms.L[l], &c = ms.L[l + 1];
     // the first post-smoothing sweep of a fused level adds the prolongation itself
-    const bool fuse = ms.fuseProlong && ms.nuPost > 0 && Fused(ms, d);
+    const bool fuse = ms.fuseProlong && ms.nuPost > 0 && Fused(d);
     if (d.nTiles > 0 && !fuse)
       KMixProlongAdd<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K,
                                               ms.nuPost > 0 ? 3 : -1);

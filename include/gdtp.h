@@ -217,6 +217,8 @@ int gdtp_mixed_cycles(gdtp_system* s, int cycles);
  * bitwise the same iterate.  shape -2 only reports; `current` (may be NULL) receives the shape in
  * use.  No counterpart in the reference. */
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
+/* The same for one level only (the best shape differs between levels); shape -2 only reports. */
+int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current);
 /* Transfers fused into the mixed solver's relaxation kernels: GDTP_FUSE_PROLONGATION (the
  * default) lets the first post-smoothing sweep of every level swept by fused tiles add the bilinear
  * prolongation of the coarse correction itself, instead of a separate pass over the level; 0 keeps
