@@ -145,7 +145,7 @@ def error_vs_direct(S, x, b):
 COPY_BYTES = 1 << 28
 
 
-def profile_kernel(cfg, K, which, reps=10):
+def profile_kernel(cfg, K, which, reps=10, shape=1):
     """Kernel-only run for hardware-counter passes: four copies of a 256 MiB buffer (the FETCH_SIZE /
     WRITE_SIZE calibration: each reads and writes exactly COPY_BYTES), then `reps` launches of the
     dominant kernel of the chosen path on the benchmark workload."""
@@ -157,7 +157,7 @@ def profile_kernel(cfg, K, which, reps=10):
     torch.cuda.synchronize()
     mesh, H, S = make_system(cfg, K, mixed=(which != "band"))
     if which != "band":  # "mixed": one launch per colour; "fused": the fused sweep on 64 x 4 tiles
-        S.tile_shape(-1 if which == "mixed" else 1)
+        S.tile_shape(-1 if which == "mixed" else shape)
         f, b = texture_rhs(H, S, K)
         x = torch.zeros_like(b)
         solve(S, x, b, 1e-6, 40, "mixed")  # leaves meaningful FP32 vectors in the solver's buffers
@@ -513,6 +513,7 @@ def main():
     ap.add_argument("--serving-leg", default="on", choices=["on", "off"],
                     help="short request-loop leg (three meshes, 20 QPS) reported beside the solve throughput")
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "fused", "band"])
+    ap.add_argument("--tile-shape", type=int, default=1, help="fused tile shape index for --profile-kernel fused")
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
     ap.add_argument("--workload", default=None, choices=["c3", "c4", "assembly"],
                     help="other workloads: c3 batched geodesics, c4 line integral convolution, assembly = mesh admission "
@@ -528,7 +529,7 @@ def main():
     K = 3
     max_cycles = 40
     if args.profile_kernel:
-        profile_kernel(args.config, K, args.profile_kernel)
+        profile_kernel(args.config, K, args.profile_kernel, shape=args.tile_shape)
         return
     if args.serving:
         serving_bench(args.qps, args.requests, args.tol)

@@ -430,7 +430,8 @@ class MultigridSystem:
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
 
     # fused deep-sweep tile shapes (sub-grid texels): index -> (width, height); -1 = one launch per colour
-    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), -1: "per-colour"}
+    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), 4: "(64, 4) fetch per colour", 5: "(64, 4) XCD-contiguous",
+                   6: "(64, 8) XCD-contiguous", -1: "per-colour"}
 
     def tile_shape(self, shape=None, level=None):
         """Set (0..3 fused, -1 one launch per colour) and/or return the deep-sweep tile shape of
@@ -448,7 +449,7 @@ class MultigridSystem:
         separate pass; the iterate is bitwise the same."""
         _lib.check(_lib.load().gdtp_mixed_fuse_transfers(self._h, 1 if prolongation else 0))
 
-    def autotune_tiles(self, columns=None, sweeps=20, shapes=(0, 1, 2, 3, -1)):
+    def autotune_tiles(self, columns=None, sweeps=20, shapes=(0, 1, 2, 3, 4, 5, 6, -1)):
         """Time `sweeps` deep sweeps of every level swept by tiles for each tile shape and keep
         each level's fastest (every shape gives the same iterate).  Returns
         {level: {shape: microseconds per sweep}}, fastest first."""

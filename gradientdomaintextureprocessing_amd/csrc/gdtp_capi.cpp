@@ -583,7 +583,7 @@ int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags) {
 }
 
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
-  if (!s || shape < -2 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-2, 4)");
+  if (!s || shape < -2 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-2, 7)");
   if (shape >= -1) {
     s->tileShape = shape;
     s->levelShape.clear();

@@ -11,8 +11,14 @@ struct MixChart {  // per chart geometry of one level
 
 // Tile shapes (sub-grid texels, width x height) of the fused four-colour deep sweep; the system
 // uses one of them (or, with shape -1, one launch per colour on 64 x 4 tiles).
-constexpr int kFusedShapes = 4;
-constexpr int kFusedShape[kFusedShapes][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}};
+constexpr int kFusedShapes = 7;
+constexpr int kFusedShape[kFusedShapes][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {64, 4}, {64, 4}, {64, 8}};
+// variants: kFusedNoPrefetch fetches each colour's operands right before computing it (fewer
+// registers, more resident workgroups) instead of one colour ahead; kFusedXcd hands each of the
+// eight XCDs (workgroup b runs on XCD b % 8) a contiguous eighth of the tile list, so that tiles
+// sharing halo texels share an L2
+constexpr int kFusedNoPrefetch = 1, kFusedXcd = 2;
+constexpr int kFusedVariant[kFusedShapes] = {0, 0, 0, 0, kFusedNoPrefetch, kFusedXcd, kFusedXcd};
 
 struct DevMixLevel {
   int N = 0, B = 0, nCharts = 0;

@@ -369,7 +369,14 @@ struct FusedPhase {
 // Every global load of a colour is issued one colour ahead (the next colour's stencil, rhs and
 // old values are fetched into registers before the current colour is computed from LDS), so
 // each barrier waits on at most one colour's compute.
-template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, int KC>
+// Tile of workgroup b when the eight XCDs (b % 8) each take a contiguous eighth of the n tiles, in
+// order: XCD x owns tiles [x per + min(x, rem), ...), per + (x < rem) of them (a bijection on [0, n)).
+__device__ __forceinline__ int XcdTile(int b, int n) {
+  const int x = b % 8, k = b / 8, per = n / 8, rem = n % 8;
+  return x * per + min(x, rem) + k;
+}
+
+template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, int KC, bool kPrefetch = true, bool kXcd = false>
 __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                        long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
                                                        const uint4* __restrict__ sth, const int8_t* __restrict__ type,
@@ -377,14 +384,13 @@ __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ t
                                                        const float* __restrict__ rhs, FusedProlong pr) {
   constexpr int WX = TA + 4, WN = WX * (TB + 2), kLoads = (3 * WN + 255) / 256;
   __shared__ float sm[KC * 4 * WN];
-  const int* T = tiles + 3 * blockIdx.x;
+  const int* T = tiles + 3 * (kXcd ? XcdTile(blockIdx.x, gridDim.x) : (int)blockIdx.x);
   const MixChart c = charts[T[0]];
   MixChart cc{};
   if constexpr (kProlong) cc = pr.cch[T[0]];
   const int a0 = T[1], b0 = T[2];
   FusedOperands<kHalf, true, KC, FusedPhase<kPost, TA, TB, 0>::kIt> op0;
-  FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
-  GDTP_FUSED_FETCH(0, op0);
+  if constexpr (kPrefetch) GDTP_FUSED_FETCH(0, op0);
   // old values of the three planes coloured after the first (pre: 1, 2, 3; post: 2, 1, 0)
   float v[kLoads][KC];
 #pragma unroll
@@ -414,10 +420,11 @@ __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ t
 #pragma unroll
     for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + q) * WN + w] = v[it][ch];
   }
-  GDTP_FUSED_FETCH(1, op1);
-  __syncthreads();
-  GDTP_FUSED_APPLY(0, op0);
-  {
+  if constexpr (kPrefetch) {
+    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
+    GDTP_FUSED_FETCH(1, op1);
+    __syncthreads();
+    GDTP_FUSED_APPLY(0, op0);
     FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 2>::kIt> op2;
     GDTP_FUSED_FETCH(2, op2);
     __syncthreads();
@@ -428,6 +435,28 @@ __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ t
     GDTP_FUSED_APPLY(2, op2);
     __syncthreads();
     GDTP_FUSED_APPLY(3, op3);
+  } else {  // each colour's operands fetched right before it is computed
+    __syncthreads();
+    GDTP_FUSED_FETCH(0, op0);
+    GDTP_FUSED_APPLY(0, op0);
+    __syncthreads();
+    {
+      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
+      GDTP_FUSED_FETCH(1, op1);
+      GDTP_FUSED_APPLY(1, op1);
+    }
+    __syncthreads();
+    {
+      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 2>::kIt> op2;
+      GDTP_FUSED_FETCH(2, op2);
+      GDTP_FUSED_APPLY(2, op2);
+    }
+    __syncthreads();
+    {
+      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 3>::kIt> op3;
+      GDTP_FUSED_FETCH(3, op3);
+      GDTP_FUSED_APPLY(3, op3);
+    }
   }
 }
 #undef GDTP_FUSED_FETCH
@@ -1444,16 +1473,16 @@ static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
 
 // One fused four-colour sweep (KMixDeepGsFused) over the level's tiles of shape `shape`, columns in
 // chunks of up to four; e and e2 swap afterwards.
-template <bool kHalf, bool kPost, bool kProlong, int TA, int TB>
+template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, bool kPrefetch, bool kXcd>
 static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
   for (int c0 = 0; c0 < K; c0 += 4) {
     const long off = (long)c0 * d.Nd;
     FusedProlong p = pr;
     if (p.ce) p.ce += (long)c0 * p.cNd;
 #define GDTP_FUSED(KC)                                                                                                               \
-  KMixDeepGsFused<kHalf, kPost, kProlong, TA, TB, KC><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, d.Nd,       \
-                                                                                       d.stencil, d.stencilC, d.stencilH, d.type,    \
-                                                                                       d.e + off, d.e2 + off, d.rhs + off, p)
+  KMixDeepGsFused<kHalf, kPost, kProlong, TA, TB, KC, kPrefetch, kXcd><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, \
+                                                                                       d.Nd, d.stencil, d.stencilC, d.stencilH,     \
+                                                                                       d.type, d.e + off, d.e2 + off, d.rhs + off, p)
     switch (std::min(4, K - c0)) {
       case 4: GDTP_FUSED(4); break;
       case 3: GDTP_FUSED(3); break;
@@ -1467,13 +1496,20 @@ static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedPr
 
 template <bool kHalf, bool kPost, bool kProlong>
 static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
-  static_assert(kFusedShapes == 4, "one case per fused tile shape");
+  static_assert(kFusedShapes == 7, "one case per fused tile shape");
+#define GDTP_SHAPE(I)                                                                                 \
+  FusedLaunch<kHalf, kPost, kProlong, kFusedShape[I][0], kFusedShape[I][1], !(kFusedVariant[I] & kFusedNoPrefetch), \
+              (kFusedVariant[I] & kFusedXcd) != 0>(d, I, K, s, pr)
   switch (shape) {
-    case 0: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[0][0], kFusedShape[0][1]>(d, 0, K, s, pr); break;
-    case 1: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[1][0], kFusedShape[1][1]>(d, 1, K, s, pr); break;
-    case 2: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[2][0], kFusedShape[2][1]>(d, 2, K, s, pr); break;
-    default: FusedLaunch<kHalf, kPost, kProlong, kFusedShape[3][0], kFusedShape[3][1]>(d, 3, K, s, pr); break;
+    case 0: GDTP_SHAPE(0); break;
+    case 1: GDTP_SHAPE(1); break;
+    case 2: GDTP_SHAPE(2); break;
+    case 3: GDTP_SHAPE(3); break;
+    case 4: GDTP_SHAPE(4); break;
+    case 5: GDTP_SHAPE(5); break;
+    default: GDTP_SHAPE(6); break;
   }
+#undef GDTP_SHAPE
 }
 
 // Whether the fused sweep of level d runs (a fused tile shape with tiles on this level).

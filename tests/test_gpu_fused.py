@@ -50,7 +50,7 @@ def test_fused_sweep_is_bitwise_four_launches(case, K):
     print(f"[fused {case} K={K}] per-colour launches", flush=True)
     want = iterate(S, b, -1, 2, False)
     assert np.all(np.isfinite(want)) and np.abs(want).max() > 0
-    for shape in (0, 1, 2, 3):
+    for shape in range(7):
         for use_graph in (False, True):
             print(f"[fused {case} K={K}] shape {shape} graph {use_graph}", flush=True)
             got = iterate(S, b, shape, 2, use_graph)
