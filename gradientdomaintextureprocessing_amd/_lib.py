@@ -77,6 +77,7 @@ SIGNATURES = {
     "gdtp_mixed_deep_sweep": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "gdtp_mixed_tile_shape": (c_int, [c_void_p, c_int, P(c_int)]),
     "gdtp_mixed_level_tile_shape": (c_int, [c_void_p, c_int, c_int, P(c_int)]),
+    "gdtp_mixed_tile_shape_info": (c_int, [c_int, P(c_int), P(c_int), P(c_int), P(c_int)]),
     "gdtp_mixed_fuse_transfers": (c_int, [c_void_p, c_int]),
     "gdtp_system_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),
 }

@@ -429,13 +429,17 @@ class MultigridSystem:
     def mixed_deep_sweep(self, level, ncols):
         _lib.check(_lib.load().gdtp_mixed_deep_sweep(self._h, level, ncols, _stream()))
 
-    # fused deep-sweep tile shapes (sub-grid texels): index -> (width, height); -1 = one launch per colour
-    TILE_SHAPES = {0: (64, 8), 1: (64, 4), 2: (128, 4), 3: (32, 16), 4: "(64, 4) fetch per colour", 5: "(64, 4) XCD-contiguous",
-                   6: "(64, 8) XCD-contiguous", -1: "per-colour"}
+    @property
+    def TILE_SHAPES(self):
+        """Deep-sweep tile shapes: index -> label (fused shapes 0..6, -1 one launch per colour)."""
+        return tile_shapes()
+
+    TILE_SHAPE_AUTO = -3
 
     def tile_shape(self, shape=None, level=None):
-        """Set (0..3 fused, -1 one launch per colour) and/or return the deep-sweep tile shape of
-        every level, or of one level."""
+        """Set (0..6 fused, -1 one launch per colour, TILE_SHAPE_AUTO per level by size, the
+        default) and/or return the deep-sweep tile shape setting of every level, or the shape one
+        level uses."""
         cur = ctypes.c_int(0)
         arg = -2 if shape is None else int(shape)
         if level is None:
@@ -445,16 +449,17 @@ class MultigridSystem:
         return cur.value
 
     def fuse_transfers(self, prolongation=True):
-        """Fuse the prolongation into the first post-smoothing sweep (default) or keep it a
-        separate pass; the iterate is bitwise the same."""
+        """Fuse the prolongation into the first post-smoothing sweep of fused levels, or keep it a
+        separate pass (the default); the iterate is bitwise the same."""
         _lib.check(_lib.load().gdtp_mixed_fuse_transfers(self._h, 1 if prolongation else 0))
 
-    def autotune_tiles(self, columns=None, sweeps=20, shapes=(0, 1, 2, 3, 4, 5, 6, -1)):
+    def autotune_tiles(self, columns=None, sweeps=20, shapes=None):
         """Time `sweeps` deep sweeps of every level swept by tiles for each tile shape and keep
         each level's fastest (every shape gives the same iterate).  Returns
         {level: {shape: microseconds per sweep}}, fastest first."""
         import torch
         K = columns or self.max_cols
+        shapes = list(tile_shapes()) if shapes is None else shapes
         st = torch.cuda.current_stream()
         out = {}
         for level in range(self.hierarchy.levels - 1):
@@ -487,6 +492,22 @@ class MultigridSystem:
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.gdtp_system_destroy(self._h)
             self._h = None
+
+
+_PREFETCH = {0: "fetch per colour", 1: "one colour ahead", 2: "all colours ahead"}
+
+
+def tile_shapes():
+    """The fused deep-sweep tile shapes the library offers, index -> label ("64x4 tile, 256
+    threads, one colour ahead, XCD-contiguous"), and -1 -> "per-colour" (one launch per colour)."""
+    out, lib = {}, _lib.load()
+    w, h, t, v = (ctypes.c_int() for _ in range(4))
+    k = 0
+    while lib.gdtp_mixed_tile_shape_info(k, ctypes.byref(w), ctypes.byref(h), ctypes.byref(t), ctypes.byref(v)) == 0:
+        out[k] = f"{w.value}x{h.value} tile, {t.value} threads, {_PREFETCH[v.value & 3]}" + (", XCD-contiguous" if v.value & 4 else "")
+        k += 1
+    out[-1] = "per-colour"
+    return out
 
 
 def UpdateLinearSystem(hierarchy, stiffness_weight, screen_weight=1.0, max_cols=3):

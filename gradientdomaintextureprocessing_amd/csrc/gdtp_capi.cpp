@@ -63,7 +63,7 @@ struct gdtp_system {
   std::unique_ptr<MixedSystem> mixed;
   std::unique_ptr<DevVectorField> vf;
   std::unique_ptr<DirectFactor> direct;  // built by the first gdtp_direct_solve after create / update
-  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = 0, fuseTransfers = 1;
+  int nuPre = 1, nuPost = 2, boundarySweeps = 1, useGraph = 1, krylov = 0, cycles = 0, tileShape = GDTP_TILE_SHAPE_AUTO, fuseTransfers = 0;
   double omega = 1.0;
   std::vector<int> levelShape;  // per-level deep-sweep tile shapes set since the last all-level setting
   double* dNorms = nullptr;
@@ -558,6 +558,13 @@ int gdtp_residual_norms_noalloc(gdtp_system* s, const double* x, const double* b
   })
 }
 
+// The automatic deep-sweep tile shape of a level: one launch per colour where a colour plane holds
+// more than 2^21 sub-grid texels (there the per-colour sweep streams at the memory's rate and the
+// fused one does not keep up: C2 level 0, 3.7 M texels per plane, 186 against 188-192 us per sweep),
+// else the fused 64x4 tiles of 384 threads, one colour ahead, XCD-contiguous (C2 levels 1-3: 46.7,
+// 17.6 and 8.3 us per sweep against 58.0, 19.7 and 14.6 us per colour).
+static int AutoTileShape(const DevMixLevel& d) { return d.plane > (1L << 21) ? -1 : 2; }
+
 static MixedSystem& Mixed(gdtp_system* s) {
   if (!s->mixed) {
     std::unique_ptr<MixedSystem> m(new MixedSystem());
@@ -567,7 +574,10 @@ static MixedSystem& Mixed(gdtp_system* s) {
   MixedSystem& m = *s->mixed;
   m.nuPre = s->nuPre, m.nuPost = s->nuPost, m.boundarySweeps = s->boundarySweeps, m.omega = (float)s->omega;
   m.screen = s->screen, m.stiff = s->stiff, m.krylov = s->krylov != 0, m.cycles = s->cycles;
-  for (int l = 0; l < m.levels; l++) m.L[l].shape = l < (int)s->levelShape.size() ? s->levelShape[l] : s->tileShape;
+  for (int l = 0; l < m.levels; l++) {
+    const int sh = l < (int)s->levelShape.size() ? s->levelShape[l] : s->tileShape;
+    m.L[l].shape = sh == GDTP_TILE_SHAPE_AUTO ? AutoTileShape(m.L[l]) : sh;
+  }
   m.fuseProlong = (s->fuseTransfers & GDTP_FUSE_PROLONGATION) != 0;
   return m;
 }
@@ -583,12 +593,12 @@ int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags) {
 }
 
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
-  if (!s || shape < -2 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-2, 7)");
-  if (shape >= -1) {
+  if (!s || shape < GDTP_TILE_SHAPE_AUTO || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-3, 7)");
+  if (shape != -2) {
     s->tileShape = shape;
     s->levelShape.clear();
     if (s->mixed) {
-      for (DevMixLevel& d : s->mixed->L) d.shape = shape;
+      for (DevMixLevel& d : s->mixed->L) d.shape = shape == GDTP_TILE_SHAPE_AUTO ? AutoTileShape(d) : shape;
       InvalidateGraph(*s->mixed);
     }
   }
@@ -596,19 +606,30 @@ int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
   return GDTP_OK;
 }
 
-int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current) {
-  if (!s || !s->ds || level < 0 || level >= s->ds->levels || shape < -2 || shape >= kFusedShapes)
-    return Fail(GDTP_ERROR_INVALID, "level or tile shape out of range");
-  if (s->levelShape.empty()) s->levelShape.assign(s->ds->levels, s->tileShape);
-  if (shape >= -1 && shape != s->levelShape[level]) {
-    s->levelShape[level] = shape;
-    if (s->mixed) {
-      s->mixed->L[level].shape = shape;
-      InvalidateGraph(*s->mixed);
-    }
-  }
-  if (current) *current = s->levelShape[level];
+int gdtp_mixed_tile_shape_info(int shape, int* width, int* height, int* threads, int* variant) {
+  if (shape < 0 || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [0, 7)");
+  const FusedShapeInfo& f = kFusedShape[shape];
+  if (width) *width = f.ta;
+  if (height) *height = f.tb;
+  if (threads) *threads = f.threads;
+  if (variant) *variant = f.variant;
   return GDTP_OK;
+}
+
+int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current) {
+  if (!s || !s->ds || level < 0 || level >= s->ds->levels || shape < GDTP_TILE_SHAPE_AUTO || shape >= kFusedShapes)
+    return Fail(GDTP_ERROR_INVALID, "level or tile shape out of range");
+  GDTP_TRY({
+    MixedSystem& m = Mixed(s);  // the automatic shape depends on the level's size
+    if (s->levelShape.empty()) s->levelShape.assign(s->ds->levels, s->tileShape);
+    if (shape != -2 && shape != s->levelShape[level]) {
+      s->levelShape[level] = shape;
+      m.L[level].shape = shape == GDTP_TILE_SHAPE_AUTO ? AutoTileShape(m.L[level]) : shape;
+      InvalidateGraph(m);
+    }
+    if (current) *current = m.L[level].shape;
+    return GDTP_OK;
+  })
 }
 
 int gdtp_mixed_configure(gdtp_system* s, int nuPre, int nuPost, int boundarySweeps, double omega, int useGraph) {

@@ -9,16 +9,31 @@ struct MixChart {  // per chart geometry of one level
   int w, h, w2, h2, base2, cofx, cofy;
 };
 
-// Tile shapes (sub-grid texels, width x height) of the fused four-colour deep sweep; the system
-// uses one of them (or, with shape -1, one launch per colour on 64 x 4 tiles).
+// Tile shapes of the fused four-colour deep sweep: a tile of ta x tb sub-grid texels per
+// workgroup of `threads` threads, and a variant.  The system uses one of them per level (or, with
+// shape -1, one launch per colour on 64 x 4 tiles).
+// Variants: the operands of a colour (stencil, rhs) are fetched right before it is computed
+// (kFusedPerColour: fewest registers, most resident workgroups), one colour ahead
+// (kFusedOneAhead), or all four colours' before the first is computed (kFusedAllAhead: one memory
+// round trip per tile, most registers); kFusedXcd hands each of the eight XCDs (workgroup b runs on
+// XCD b % 8) a contiguous eighth of the tile list, so that tiles sharing halo texels share an L2.
+// A workgroup of `threads` threads covers a colour's region (the tile and the halo texels later
+// colours read) in ceil(region / threads) passes: 384 threads cover every region of a 64 x 4
+// tile in one pass, 640 those of a 64 x 8 tile.
+constexpr int kFusedPerColour = 0, kFusedOneAhead = 1, kFusedAllAhead = 2, kFusedXcd = 4;
+struct FusedShapeInfo {
+  int ta, tb, threads, variant;
+};
 constexpr int kFusedShapes = 7;
-constexpr int kFusedShape[kFusedShapes][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 16}, {64, 4}, {64, 4}, {64, 8}};
-// variants: kFusedNoPrefetch fetches each colour's operands right before computing it (fewer
-// registers, more resident workgroups) instead of one colour ahead; kFusedXcd hands each of the
-// eight XCDs (workgroup b runs on XCD b % 8) a contiguous eighth of the tile list, so that tiles
-// sharing halo texels share an L2
-constexpr int kFusedNoPrefetch = 1, kFusedXcd = 2;
-constexpr int kFusedVariant[kFusedShapes] = {0, 0, 0, 0, kFusedNoPrefetch, kFusedXcd, kFusedXcd};
+constexpr FusedShapeInfo kFusedShape[kFusedShapes] = {
+    {64, 4, 256, kFusedOneAhead},
+    {64, 4, 256, kFusedOneAhead | kFusedXcd},
+    {64, 4, 384, kFusedOneAhead | kFusedXcd},
+    {64, 4, 384, kFusedAllAhead | kFusedXcd},
+    {64, 8, 640, kFusedOneAhead | kFusedXcd},
+    {64, 8, 640, kFusedAllAhead | kFusedXcd},
+    {64, 4, 256, kFusedAllAhead | kFusedXcd},
+};
 
 struct DevMixLevel {
   int N = 0, B = 0, nCharts = 0;
@@ -62,7 +77,7 @@ struct DevMixLevel {
 struct MixedSystem {
   int levels = 0, kcap = 0;
   int nuPre = 1, nuPost = 2, boundarySweeps = 1;  // Schwarz sweeps over the boundary rows per Schwarz step
-  bool fuseProlong = true;                        // fused levels: the first post sweep adds the prolongation itself
+  bool fuseProlong = false;                       // fused levels: the first post sweep adds the prolongation itself
   float omega = 1.0f;                             // damping of the Schwarz block update
   std::vector<DevMixLevel> L;
   DevLdlt coarse;                  // exact FP64 solve of the coarsest level

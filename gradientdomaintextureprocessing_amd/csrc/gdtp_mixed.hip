@@ -47,14 +47,25 @@ __device__ __forceinline__ int DenseByte(const MixChart& c, long plane, int i, i
   return 4 * (((i & 1) | ((j & 1) << 1)) * (int)plane + c.base2 + (j >> 1) * c.w2 + (i >> 1));
 }
 
+// Tile (or block of rows) of workgroup b when the eight XCDs (workgroup b runs on XCD b % 8) each
+// take a contiguous eighth of the n tiles, in order: XCD x owns tiles [x per + min(x, rem), ...),
+// per + (x < rem) of them (a bijection on [0, n)).  Neighbouring tiles then share an L2, so the halo
+// texels a tile reads from its neighbours are mostly L2 hits instead of HBM re-reads.
+__device__ __forceinline__ int XcdTile(int b, int n) {
+  const int x = b % 8, k = b / 8, per = n / 8, rem = n % 8;
+  return x * per + min(x, rem) + k;
+}
+
 // Columns in chunks of up to four with the chunk width as a compile-time constant, so that a
 // kernel can issue every load of a chunk before its first store (one memory round trip per chunk
 // instead of one per column).
-template <class F>
+// kMax: the widest chunk (4, or 3 where a kernel holds many values per column and the register
+// count of the widest chunk would halve its occupancy).
+template <int kMax = 4, class F>
 __device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
-  for (int c0 = 0; c0 < K; c0 += 4) {
-    switch (min(4, K - c0)) {
-      case 4: f(c0, std::integral_constant<int, 4>()); break;
+  for (int c0 = 0; c0 < K; c0 += kMax) {
+    switch (min(kMax, K - c0)) {
+      case 4: if constexpr (kMax >= 4) f(c0, std::integral_constant<int, 4>()); break;
       case 3: f(c0, std::integral_constant<int, 3>()); break;
       case 2: f(c0, std::integral_constant<int, 2>()); break;
       default: f(c0, std::integral_constant<int, 1>()); break;
@@ -237,8 +248,8 @@ __device__ __forceinline__ void Unpack(const uint4& w, float (&cf)[9]) {
   cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
 }
 
-// Thread t's texels of the region [LX, TA + HX) x [LY, TB + HY) of plane Q: t, t + 256, ...
-template <bool kHalf, bool kFirst, bool kProlong, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
+// Thread t's texels of the region [LX, TA + HX) x [LY, TB + HY) of plane Q: t, t + NT, ...
+template <bool kHalf, bool kFirst, bool kProlong, int TA, int TB, int NT, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
 __device__ __forceinline__ void FusedFetch(const MixChart& c, int a0, int b0, long plane, long Nd, const float* __restrict__ st,
                                            const float2* __restrict__ stc, const uint4* __restrict__ sth,
                                            const int8_t* __restrict__ type, const float* __restrict__ eIn, const float* __restrict__ rhs,
@@ -247,7 +258,7 @@ __device__ __forceinline__ void FusedFetch(const MixChart& c, int a0, int b0, lo
   constexpr int RW = TA + HX - LX, RN = RW * (TB + HY - LY);
 #pragma unroll
   for (int it = 0; it < kIt; it++) {
-    const int t = (int)threadIdx.x + 256 * it;
+    const int t = (int)threadIdx.x + NT * it;
     const int x = LX + t % RW, y = LY + t / RW;
     const int a = a0 + x, b = b0 + y, i = 2 * a + (Q & 1), j = 2 * b + (Q >> 1);
     const bool in = t < RN && a >= 0 && b >= 0 && i < c.w && j < c.h;
@@ -289,14 +300,14 @@ __device__ __forceinline__ void FusedFetch(const MixChart& c, int a0, int b0, lo
 // Colour Q on its region from the fetched operands: deep texels take the Gauss-Seidel value from
 // their neighbours in LDS; the first colour's other texels put their old value into LDS; the
 // tile's own active texels are written to eOut.
-template <bool kHalf, bool kFirst, int TA, int TB, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
+template <bool kHalf, bool kFirst, int TA, int TB, int NT, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
 __device__ __forceinline__ void FusedApply(float* __restrict__ eOut, long Nd, float* sm, const FusedOperands<kHalf, kFirst, KC, kIt>& op) {
   constexpr int WX = TA + 4, WN = WX * (TB + 2);
   constexpr int RW = TA + HX - LX;
 #pragma unroll
   for (int it = 0; it < kIt; it++) {
     const int st = op.state[it];
-    const int t = (int)threadIdx.x + 256 * it;
+    const int t = (int)threadIdx.x + NT * it;
     const int x = LX + t % RW, y = LY + t / RW;
     const bool own = x >= 0 && x < TA && y >= 0 && y < TB;
     const int w = op.w[it];
@@ -347,55 +358,58 @@ __device__ __forceinline__ void FusedApply(float* __restrict__ eOut, long Nd, fl
   }
 }
 
-template <bool kPost, int TA, int TB, int S>
+template <bool kPost, int TA, int TB, int NT, int S>
 struct FusedPhase {
   static constexpr int Q = kPost ? 3 - S : S;
   static constexpr FusedBox R = FusedRegion(kPost, Q);
   static constexpr int RN = (TA + R.hx - R.lx) * (TB + R.hy - R.ly);
-  static constexpr int kIt = (RN + 255) / 256;
+  static constexpr int kIt = (RN + NT - 1) / NT;
 };
 
 #define GDTP_FUSED_FETCH(S, OP)                                                                                                       \
-  FusedFetch<kHalf, S == 0, kProlong, TA, TB, KC, FusedPhase<kPost, TA, TB, S>::Q, FusedPhase<kPost, TA, TB, S>::R.lx,                \
-             FusedPhase<kPost, TA, TB, S>::R.hx, FusedPhase<kPost, TA, TB, S>::R.ly, FusedPhase<kPost, TA, TB, S>::R.hy,              \
-             FusedPhase<kPost, TA, TB, S>::kIt>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, rhs, pr, cc, OP)
+  FusedFetch<kHalf, S == 0, kProlong, TA, TB, NT, KC, FusedPhase<kPost, TA, TB, NT, S>::Q, FusedPhase<kPost, TA, TB, NT, S>::R.lx,                \
+             FusedPhase<kPost, TA, TB, NT, S>::R.hx, FusedPhase<kPost, TA, TB, NT, S>::R.ly, FusedPhase<kPost, TA, TB, NT, S>::R.hy,              \
+             FusedPhase<kPost, TA, TB, NT, S>::kIt>(c, a0, b0, plane, Nd, st, stc, sth, type, eIn, rhs, pr, cc, OP)
 #define GDTP_FUSED_APPLY(S, OP)                                                                                                       \
-  FusedApply<kHalf, S == 0, TA, TB, KC, FusedPhase<kPost, TA, TB, S>::Q, FusedPhase<kPost, TA, TB, S>::R.lx,                          \
-             FusedPhase<kPost, TA, TB, S>::R.hx, FusedPhase<kPost, TA, TB, S>::R.ly, FusedPhase<kPost, TA, TB, S>::R.hy,              \
-             FusedPhase<kPost, TA, TB, S>::kIt>(eOut, Nd, sm, OP)
+  FusedApply<kHalf, S == 0, TA, TB, NT, KC, FusedPhase<kPost, TA, TB, NT, S>::Q, FusedPhase<kPost, TA, TB, NT, S>::R.lx,                          \
+             FusedPhase<kPost, TA, TB, NT, S>::R.hx, FusedPhase<kPost, TA, TB, NT, S>::R.ly, FusedPhase<kPost, TA, TB, NT, S>::R.hy,              \
+             FusedPhase<kPost, TA, TB, NT, S>::kIt>(eOut, Nd, sm, OP)
 
 // kProlong (with kPost): the sweep starts post-smoothing and first adds the prolongation of the
 // coarse correction to every active texel it reads (what KMixProlongAdd would have written).
-// Every global load of a colour is issued one colour ahead (the next colour's stencil, rhs and
-// old values are fetched into registers before the current colour is computed from LDS), so
-// each barrier waits on at most one colour's compute.
-// Tile of workgroup b when the eight XCDs (b % 8) each take a contiguous eighth of the n tiles, in
-// order: XCD x owns tiles [x per + min(x, rem), ...), per + (x < rem) of them (a bijection on [0, n)).
-__device__ __forceinline__ int XcdTile(int b, int n) {
-  const int x = b % 8, k = b / 8, per = n / 8, rem = n % 8;
-  return x * per + min(x, rem) + k;
-}
-
-template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, int KC, bool kPrefetch = true, bool kXcd = false>
-__global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                       long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
-                                                       const uint4* __restrict__ sth, const int8_t* __restrict__ type,
-                                                       const float* __restrict__ eIn, float* __restrict__ eOut,
-                                                       const float* __restrict__ rhs, FusedProlong pr) {
-  constexpr int WX = TA + 4, WN = WX * (TB + 2), kLoads = (3 * WN + 255) / 256;
+// kAhead: each colour's global loads (stencil, rhs, the first colour's old values) are issued right
+// before the colour is computed (kFusedPerColour), one colour ahead (kFusedOneAhead: the next
+// colour's operands are in flight while the current one is computed from LDS, so each barrier waits
+// on at most one colour's compute), or all at the start (kFusedAllAhead: one memory round trip per
+// tile).  NT threads per workgroup; kXcd: XCD-contiguous tile order (XcdTile).
+template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, int NT, int KC, int kAhead, bool kXcd>
+__global__ __launch_bounds__(NT) void KMixDeepGsFused(const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+                                                      long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+                                                      const uint4* __restrict__ sth, const int8_t* __restrict__ type,
+                                                      const float* __restrict__ eIn, float* __restrict__ eOut,
+                                                      const float* __restrict__ rhs, FusedProlong pr) {
+  constexpr int WX = TA + 4, WN = WX * (TB + 2), kLoads = (3 * WN + NT - 1) / NT;
   __shared__ float sm[KC * 4 * WN];
   const int* T = tiles + 3 * (kXcd ? XcdTile(blockIdx.x, gridDim.x) : (int)blockIdx.x);
   const MixChart c = charts[T[0]];
   MixChart cc{};
   if constexpr (kProlong) cc = pr.cch[T[0]];
   const int a0 = T[1], b0 = T[2];
-  FusedOperands<kHalf, true, KC, FusedPhase<kPost, TA, TB, 0>::kIt> op0;
-  if constexpr (kPrefetch) GDTP_FUSED_FETCH(0, op0);
+  FusedOperands<kHalf, true, KC, FusedPhase<kPost, TA, TB, NT, 0>::kIt> op0;
+  FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, NT, 1>::kIt> op1;
+  FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, NT, 2>::kIt> op2;
+  FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, NT, 3>::kIt> op3;
+  if constexpr (kAhead != kFusedPerColour) GDTP_FUSED_FETCH(0, op0);
+  if constexpr (kAhead == kFusedAllAhead) {
+    GDTP_FUSED_FETCH(1, op1);
+    GDTP_FUSED_FETCH(2, op2);
+    GDTP_FUSED_FETCH(3, op3);
+  }
   // old values of the three planes coloured after the first (pre: 1, 2, 3; post: 2, 1, 0)
   float v[kLoads][KC];
 #pragma unroll
   for (int it = 0; it < kLoads; it++) {
-    const int t = (int)threadIdx.x + 256 * it;
+    const int t = (int)threadIdx.x + NT * it;
     const int q = kPost ? 2 - t / WN : 1 + t / WN, w = t % WN;
     const int a = a0 + w % WX - 2, b = b0 + w / WX - 1, i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
 #pragma unroll
@@ -412,59 +426,59 @@ __global__ __launch_bounds__(256) void KMixDeepGsFused(const int* __restrict__ t
       }
     }
   }
+  // one colour ahead: the second colour's loads go out before the window is stored (and waited for)
+  if constexpr (kAhead == kFusedOneAhead) GDTP_FUSED_FETCH(1, op1);
 #pragma unroll
   for (int it = 0; it < kLoads; it++) {
-    const int t = (int)threadIdx.x + 256 * it;
+    const int t = (int)threadIdx.x + NT * it;
     if (t >= 3 * WN) continue;
     const int q = kPost ? 2 - t / WN : 1 + t / WN, w = t % WN;
 #pragma unroll
     for (int ch = 0; ch < KC; ch++) sm[(ch * 4 + q) * WN + w] = v[it][ch];
   }
-  if constexpr (kPrefetch) {
-    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
-    GDTP_FUSED_FETCH(1, op1);
+  if constexpr (kAhead == kFusedAllAhead) {
     __syncthreads();
     GDTP_FUSED_APPLY(0, op0);
-    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 2>::kIt> op2;
+    __syncthreads();
+    GDTP_FUSED_APPLY(1, op1);
+    __syncthreads();
+    GDTP_FUSED_APPLY(2, op2);
+    __syncthreads();
+    GDTP_FUSED_APPLY(3, op3);
+  } else if constexpr (kAhead == kFusedOneAhead) {
+    __syncthreads();
+    GDTP_FUSED_APPLY(0, op0);
     GDTP_FUSED_FETCH(2, op2);
     __syncthreads();
     GDTP_FUSED_APPLY(1, op1);
-    FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 3>::kIt> op3;
     GDTP_FUSED_FETCH(3, op3);
     __syncthreads();
     GDTP_FUSED_APPLY(2, op2);
     __syncthreads();
     GDTP_FUSED_APPLY(3, op3);
-  } else {  // each colour's operands fetched right before it is computed
+  } else {
     __syncthreads();
     GDTP_FUSED_FETCH(0, op0);
     GDTP_FUSED_APPLY(0, op0);
     __syncthreads();
-    {
-      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 1>::kIt> op1;
-      GDTP_FUSED_FETCH(1, op1);
-      GDTP_FUSED_APPLY(1, op1);
-    }
+    GDTP_FUSED_FETCH(1, op1);
+    GDTP_FUSED_APPLY(1, op1);
     __syncthreads();
-    {
-      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 2>::kIt> op2;
-      GDTP_FUSED_FETCH(2, op2);
-      GDTP_FUSED_APPLY(2, op2);
-    }
+    GDTP_FUSED_FETCH(2, op2);
+    GDTP_FUSED_APPLY(2, op2);
     __syncthreads();
-    {
-      FusedOperands<kHalf, false, KC, FusedPhase<kPost, TA, TB, 3>::kIt> op3;
-      GDTP_FUSED_FETCH(3, op3);
-      GDTP_FUSED_APPLY(3, op3);
-    }
+    GDTP_FUSED_FETCH(3, op3);
+    GDTP_FUSED_APPLY(3, op3);
   }
 }
 #undef GDTP_FUSED_FETCH
 #undef GDTP_FUSED_APPLY
 
-// res = rhs - A e at deep texels (reciprocal form: (sum_{k!=4} c_k e_k + e_i) * a_4).  A workgroup
-// covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are fetched
-// once rather than once per plane.
+// res = rhs - A e at deep texels (raw A: the symmetric five-word stencil, a4 the diagonal).  A
+// workgroup covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are
+// fetched once rather than once per plane.  (Measured and not kept: a thread owning the 2x2 block of
+// texels of one sub-grid position, its 4x4 window of e and all four stencils loaded before any is
+// used: one memory round trip per thread, but 138 registers and 232 instead of 221 us at C2.)
 __device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                  long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                  const float* __restrict__ rhs, float* res, int K, int zeroQ) {
@@ -568,7 +582,7 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
                                                     const float* __restrict__ rhs, float* res, int K, int zeroQ, int B,
                                                     const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
                                                     const float* __restrict__ vl, const float* __restrict__ dg) {
-  if ((int)blockIdx.x < nTiles) ResidualDeepTile(blockIdx.x, tiles, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
+  if ((int)blockIdx.x < nTiles) ResidualDeepTile(XcdTile(blockIdx.x, nTiles), tiles, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
   else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
 }
 
@@ -690,7 +704,8 @@ __global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __res
                                                     const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
                                                     int K, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
                                                     const int* __restrict__ cl, const float* __restrict__ vl) {
-  if ((int)blockIdx.x < nTiles) RestrictDeepTile(blockIdx.x, tiles, cch, fch, cplane, cNd, fplane, fNd, ctype, fres, crhs, ce, K);
+  if ((int)blockIdx.x < nTiles)
+    RestrictDeepTile(XcdTile(blockIdx.x, nTiles), tiles, cch, fch, cplane, cNd, fplane, fNd, ctype, fres, crhs, ce, K);
   else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, fres, fNd, crhs, ce, cNd, K);
 }
 
@@ -706,7 +721,7 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
                                                       const MixChart* __restrict__ cch, long fplane, long fNd, long cplane, long cNd,
                                                       const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K,
                                                       int skipQ) {
-  const int* T = tiles + 3 * blockIdx.x;
+  const int* T = tiles + 3 * XcdTile(blockIdx.x, gridDim.x);
   const MixChart f = fch[T[0]], c = cch[T[0]];
   const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
   if (a >= f.w2 || b >= f.h2) return;
@@ -812,6 +827,27 @@ __global__ void KMixScatter(int N, const int* __restrict__ g2d, const double* __
 // ------------------------------------------------------------------ FP64 outer loop
 static constexpr int kSlots = 64;  // norm accumulators per column (spreads the atomics)
 
+// Block-wide sums of v[0..kc) added to target[c * kSlots + blockIdx.x % kSlots]: one barrier pair for
+// all columns of a chunk; every thread of the block must call it with the same kc.
+__device__ __forceinline__ void BlockAddSquares4(const double (&v)[4], int kc, double* target) {
+  __shared__ double part[4][16];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if (c >= kc) break;
+    double s = v[c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) part[c][w] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kc) {
+    double s = 0;
+    for (int k = 0; k < (int)(blockDim.x / 64); k++) s += part[threadIdx.x][k];
+    atomicAdd(target + (long)threadIdx.x * kSlots + (blockIdx.x % kSlots), s);
+  }
+  __syncthreads();
+}
+
 // Block-wide sum of v added to target[blockIdx.x % kSlots]; every thread of the block must call it.
 __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -827,15 +863,36 @@ __device__ __forceinline__ void BlockAddSquares(double v, double* target) {
   __syncthreads();
 }
 
-// deep rows: r = b - A x with the symmetric five-word FP64 stencil along raster lines, up to four columns per pass
-// over the stencil; r -> FP32 dense rhs of the correction, e -> 0, |r|^2 -> norms[ch][slot]
-// One thread per deep texel d; its raster line gives the global indices of the rows above and
-// below (line = deepLine[d]: start, end, prev, next, first deep index).
+// Deep texel d's raster line gives the global indices of the rows above and below
+// (line = deepLine[d]: start, end, prev, next, first deep index).
 __device__ __forceinline__ void DeepRow(const int* __restrict__ deepLine, const int* __restrict__ lines, int d, int& cur, int& prev,
                                         int& next) {
   const int* L = lines + 5 * deepLine[d];
   const int i = d - L[4];
   cur = L[0] + i, prev = L[2] + i, next = L[3] + i;
+}
+
+// f(std::integral_constant<int, kc>) for a chunk width kc in 1..kMax known only at run time, so that
+// the chunk's loads (every column's) are all issued before the first is used.
+template <int kMax = 3, class F>
+__device__ __forceinline__ void WithChunkWidth(int kc, F&& f) {
+  switch (kc) {
+    case 4: if constexpr (kMax >= 4) f(std::integral_constant<int, 4>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    default: f(std::integral_constant<int, 1>()); break;
+  }
+}
+
+// The nine x values of deep row `cur` (rows prev, cur, next), every column of a chunk.
+template <int KC>
+__device__ __forceinline__ void RowValues(const double* __restrict__ x, int N, int c0, int prev, int cur, int next, double (&xv)[KC][9]) {
+#pragma unroll
+  for (int cc = 0; cc < KC; cc++) {
+    const double* X = x + (long)(c0 + cc) * N;
+#pragma unroll
+    for (int k = 0; k < 9; k++) xv[cc][k] = X[(k < 3 ? prev : (k < 6 ? cur : next)) + k % 3 - 1];
+  }
 }
 
 // The nine couplings of deep row `cur` from the symmetric five-word stencil ([5][N], global indices):
@@ -846,87 +903,96 @@ __device__ __forceinline__ void SymRow(const double* __restrict__ sym, int N, in
   cf[5] = sym[3L * N + cur + 1], cf[6] = sym[2L * N + next - 1], cf[7] = sym[(long)N + next], cf[8] = sym[next + 1];
 }
 
+// deep rows: r = b - A x with the symmetric five-word FP64 stencil along raster lines; r -> FP32
+// dense rhs of the correction, e -> 0, |r|^2 -> norms[ch][slot] (|b|^2 -> bnorms on the first
+// residual of a solve).
+// Workgroup vb takes kDeepGroups consecutive groups of blockDim deep texels, one texel per thread
+// per group, and adds its squared residuals to the norm slots once per column chunk (four times
+// fewer atomics than one group per workgroup).  A chunk's loads (every column's) are issued
+// together.  (C2, 12.1 M deep rows, 3 columns: 470 us per launch; one group per workgroup 463 us,
+// one column per wave 557 us: the launch time does not follow the atomics or the load batching.)
+static constexpr int kDeepGroups = 4;
 __device__ __forceinline__ void K64ResidualDeepRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
                                                     const double* __restrict__ sym, const double* __restrict__ x,
                                                     const double* __restrict__ b, int N, const int* __restrict__ g2d, float* rhs0,
                                                     float* e0, long Nd, double* norms, int K, double* bnorms) {
-  const int d = vb * blockDim.x + threadIdx.x;
-  const bool on = d < D;
-  int cur = 0, prev = 0, next = 0;
-  if (on) DeepRow(deepLine, lines, d, cur, prev, next);
-  for (int c0 = 0; c0 < K; c0 += 4) {
-    const int kc = min(4, K - c0);
+  for (int c0 = 0; c0 < K; c0 += 3) {  // chunks of three columns: 27 x values in registers
+    const int kc = min(3, K - c0);
     double acc[4] = {0, 0, 0, 0}, bacc[4] = {0, 0, 0, 0};
-    if (on) {
+    for (int g = 0; g < kDeepGroups; g++) {
+      const int d = (vb * kDeepGroups + g) * blockDim.x + threadIdx.x;
+      if (d >= D) break;
+      int cur, prev, next;
+      DeepRow(deepLine, lines, d, cur, prev, next);
       double cf[9];
       SymRow(sym, N, cur, next, cf);
       const long di = g2d[cur];
+      WithChunkWidth<3>(kc, [&](auto kcc) {
+        constexpr int KC = decltype(kcc)::value;
+        double xv[KC][9], bv[KC];
+        RowValues<KC>(x, N, c0, prev, cur, next, xv);
 #pragma unroll
-      for (int cc = 0; cc < 4; cc++) {
-        if (cc >= kc) break;
-        const int ch = c0 + cc;
-        const double* X = x + (long)ch * N;
-        double s = 0.0;
+        for (int cc = 0; cc < KC; cc++) bv[cc] = b[(long)(c0 + cc) * N + cur];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const int base = k < 3 ? prev : (k < 6 ? cur : next);
-          s += cf[k] * X[base + k % 3 - 1];
+        for (int cc = 0; cc < KC; cc++) {
+          const int ch = c0 + cc;
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 9; k++) s += cf[k] * xv[cc][k];
+          const double r = bv[cc] - s;
+          rhs0[(long)ch * Nd + di] = (float)r;
+          e0[(long)ch * Nd + di] = 0.0f;
+          acc[cc] += r * r, bacc[cc] += bv[cc] * bv[cc];
         }
-        const double bv = b[(long)ch * N + cur], r = bv - s;
-        rhs0[(long)ch * Nd + di] = (float)r;
-        e0[(long)ch * Nd + di] = 0.0f;
-        acc[cc] += r * r, bacc[cc] += bv * bv;
-      }
+      });
     }
-    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
-    if (bnorms)  // the same for every thread: |b|^2 on the first residual of a solve
-      for (int cc = 0; cc < kc; cc++) BlockAddSquares(bacc[cc], bnorms + (long)(c0 + cc) * kSlots);
+    BlockAddSquares4(acc, kc, norms + (long)c0 * kSlots);  // every thread of the block
+    if (bnorms) BlockAddSquares4(bacc, kc, bnorms + (long)c0 * kSlots);
   }
 }
 
-// boundary rows (blockIdx.y = column)
-// The same residual for per-column operators A_c = w[c][0] M + w[c][1] S: both level-0 stencils are
-// read once per texel for up to four columns (parametric solves: one refinement loop serves
-// columns with different weights, its FP32 V-cycle built for one nearby pair of weights).
+// The same residual for per-column operators A_c = w[c][0] M + w[c][1] S (parametric solves: one
+// refinement loop serves columns with different weights, its FP32 V-cycle built for one nearby pair
+// of weights): both level-0 stencils read once per texel for a chunk of columns.
 __device__ __forceinline__ void K64ResidualDeepWRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
                                                      const double* __restrict__ symM, const double* __restrict__ symS,
                                                      const double* __restrict__ x, const double* __restrict__ b, int N,
                                                      const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
                                                      const double* __restrict__ w, double* r64, double* bnorms) {
-  const int d = vb * blockDim.x + threadIdx.x;
-  const bool on = d < D;
-  int cur = 0, prev = 0, next = 0;
-  if (on) DeepRow(deepLine, lines, d, cur, prev, next);
-  for (int c0 = 0; c0 < K; c0 += 4) {
-    const int kc = min(4, K - c0);
+  for (int c0 = 0; c0 < K; c0 += 2) {  // chunks of two columns (two stencils in registers)
+    const int kc = min(2, K - c0);
     double acc[4] = {0, 0, 0, 0}, bacc[4] = {0, 0, 0, 0};
-    if (on) {
+    for (int g = 0; g < kDeepGroups; g++) {
+      const int d = (vb * kDeepGroups + g) * blockDim.x + threadIdx.x;
+      if (d >= D) break;
+      int cur, prev, next;
+      DeepRow(deepLine, lines, d, cur, prev, next);
       double cm[9], cs[9];
       SymRow(symM, N, cur, next, cm);
       SymRow(symS, N, cur, next, cs);
       const long di = g2d[cur];
+      WithChunkWidth<2>(kc, [&](auto kcc) {
+        constexpr int KC = decltype(kcc)::value;
+        double xv[KC][9], bv[KC];
+        RowValues<KC>(x, N, c0, prev, cur, next, xv);
 #pragma unroll
-      for (int cc = 0; cc < 4; cc++) {
-        if (cc >= kc) break;
-        const int ch = c0 + cc;
-        const double* X = x + (long)ch * N;
-        double sm = 0.0, ss = 0.0;
+        for (int cc = 0; cc < KC; cc++) bv[cc] = b[(long)(c0 + cc) * N + cur];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const int base = k < 3 ? prev : (k < 6 ? cur : next);
-          const double xv = X[base + k % 3 - 1];
-          sm += cm[k] * xv, ss += cs[k] * xv;
+        for (int cc = 0; cc < KC; cc++) {
+          const int ch = c0 + cc;
+          double sm = 0.0, ss = 0.0;
+#pragma unroll
+          for (int k = 0; k < 9; k++) sm += cm[k] * xv[cc][k], ss += cs[k] * xv[cc][k];
+          const double r = bv[cc] - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
+          if (r64) r64[(long)ch * N + cur] = r;
+          rhs0[(long)ch * Nd + di] = (float)r;
+          e0[(long)ch * Nd + di] = 0.0f;
+          acc[cc] += r * r, bacc[cc] += bv[cc] * bv[cc];
         }
-        const double bv = b[(long)ch * N + cur], r = bv - (w[2 * ch] * sm + w[2 * ch + 1] * ss);
-        if (r64) r64[(long)ch * N + cur] = r;
-        rhs0[(long)ch * Nd + di] = (float)r;
-        e0[(long)ch * Nd + di] = 0.0f;
-        acc[cc] += r * r, bacc[cc] += bv * bv;
-      }
+      });
     }
-    for (int cc = 0; cc < kc; cc++) BlockAddSquares(acc[cc], norms + (long)(c0 + cc) * kSlots);  // every thread of the block
-    if (bnorms)
-      for (int cc = 0; cc < kc; cc++) BlockAddSquares(bacc[cc], bnorms + (long)(c0 + cc) * kSlots);
+    BlockAddSquares4(acc, kc, norms + (long)c0 * kSlots);  // every thread of the block
+    if (bnorms) BlockAddSquares4(bacc, kc, bnorms + (long)c0 * kSlots);
   }
 }
 
@@ -972,9 +1038,14 @@ __device__ __forceinline__ void K64ResidualBoundaryWRows(int vb, int B, const in
       e0[(long)ch * Nd + g2d[g]] = 0.0f;
       sq = rv * rv, bq = bv * bv;
     }
-    for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
-    if (bnorms)
-      for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? bq : 0.0, bnorms + (long)(c0 + q) * kSlots);
+    {
+      const double sqs[4] = {lane == 0 ? sq : 0.0, lane == 1 ? sq : 0.0, lane == 2 ? sq : 0.0, lane == 3 ? sq : 0.0};
+      BlockAddSquares4(sqs, kc, norms + (long)c0 * kSlots);
+    }
+    if (bnorms) {
+      const double bqs[4] = {lane == 0 ? bq : 0.0, lane == 1 ? bq : 0.0, lane == 2 ? bq : 0.0, lane == 3 ? bq : 0.0};
+      BlockAddSquares4(bqs, kc, bnorms + (long)c0 * kSlots);
+    }
   }
 }
 
@@ -1016,9 +1087,14 @@ __device__ __forceinline__ void K64ResidualBoundaryRows(int vb, int B, const int
       e0[(long)(c0 + lane) * Nd + g2d[g]] = 0.0f;
       sq = rv * rv, bq = bv * bv;
     }
-    for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? sq : 0.0, norms + (long)(c0 + q) * kSlots);
-    if (bnorms)
-      for (int q = 0; q < kc; q++) BlockAddSquares(lane == q ? bq : 0.0, bnorms + (long)(c0 + q) * kSlots);
+    {
+      const double sqs[4] = {lane == 0 ? sq : 0.0, lane == 1 ? sq : 0.0, lane == 2 ? sq : 0.0, lane == 3 ? sq : 0.0};
+      BlockAddSquares4(sqs, kc, norms + (long)c0 * kSlots);
+    }
+    if (bnorms) {
+      const double bqs[4] = {lane == 0 ? bq : 0.0, lane == 1 ? bq : 0.0, lane == 2 ? bq : 0.0, lane == 3 ? bq : 0.0};
+      BlockAddSquares4(bqs, kc, bnorms + (long)c0 * kSlots);
+    }
   }
 }
 
@@ -1031,7 +1107,8 @@ __global__ __launch_bounds__(256) void K64Residual(int deepBlocks, int D, const 
                                                    const double* __restrict__ x, const double* __restrict__ b, int N,
                                                    const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
                                                    double* bnorms) {
-  if ((int)blockIdx.x < deepBlocks) K64ResidualDeepRows(blockIdx.x, D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
+  if ((int)blockIdx.x < deepBlocks)
+    K64ResidualDeepRows(XcdTile(blockIdx.x, deepBlocks), D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
   else
     K64ResidualBoundaryRows(blockIdx.x - deepBlocks, B, bgi, rpd, cold, vald, rpb, colb, valb, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
 }
@@ -1049,7 +1126,8 @@ __global__ __launch_bounds__(256) void K64ResidualW(int deepBlocks, int D, const
                                                     const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
                                                     const double* __restrict__ w, double* r64, double* bnorms) {
   if ((int)blockIdx.x < deepBlocks)
-    K64ResidualDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64, bnorms);
+    K64ResidualDeepWRows(XcdTile(blockIdx.x, deepBlocks), D, deepLine, lines, symM, symS, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64,
+                         bnorms);
   else
     K64ResidualBoundaryWRows(blockIdx.x - deepBlocks, B, bgi, mrpd, mcold, mvald, mrpb, mcolb, mvalb, srpd, scold, svald, srpb, scolb,
                              svalb, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64, bnorms);
@@ -1293,7 +1371,14 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     }
     if (l + 1 < nl && !d.chartSweeps)
       for (int sh = 0; sh < kFusedShapes; sh++) {
-        const std::vector<int> tiles = buildTiles(kFusedShape[sh][0], kFusedShape[sh][1]);
+        int same = -1;  // shapes of equal tile size share one tile list
+        for (int o = 0; o < sh && same < 0; o++)
+          if (kFusedShape[o].ta == kFusedShape[sh].ta && kFusedShape[o].tb == kFusedShape[sh].tb) same = o;
+        if (same >= 0) {
+          d.nFTiles[sh] = d.nFTiles[same], d.ftiles[sh] = d.ftiles[same];
+          continue;
+        }
+        const std::vector<int> tiles = buildTiles(kFusedShape[sh].ta, kFusedShape[sh].tb);
         d.nFTiles[sh] = (int)tiles.size() / 3;
         d.ftiles[sh] = Up(ms, tiles);
       }
@@ -1473,16 +1558,17 @@ static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
 
 // One fused four-colour sweep (KMixDeepGsFused) over the level's tiles of shape `shape`, columns in
 // chunks of up to four; e and e2 swap afterwards.
-template <bool kHalf, bool kPost, bool kProlong, int TA, int TB, bool kPrefetch, bool kXcd>
-static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
+template <bool kHalf, bool kPost, bool kProlong, int I>
+static void FusedLaunch(DevMixLevel& d, int K, hipStream_t s, FusedProlong pr) {
+  constexpr FusedShapeInfo F = kFusedShape[I];
   for (int c0 = 0; c0 < K; c0 += 4) {
     const long off = (long)c0 * d.Nd;
     FusedProlong p = pr;
     if (p.ce) p.ce += (long)c0 * p.cNd;
 #define GDTP_FUSED(KC)                                                                                                               \
-  KMixDeepGsFused<kHalf, kPost, kProlong, TA, TB, KC, kPrefetch, kXcd><<<d.nFTiles[shape], 256, 0, s>>>(d.ftiles[shape], d.charts, d.plane, \
-                                                                                       d.Nd, d.stencil, d.stencilC, d.stencilH,     \
-                                                                                       d.type, d.e + off, d.e2 + off, d.rhs + off, p)
+  KMixDeepGsFused<kHalf, kPost, kProlong, F.ta, F.tb, F.threads, KC, F.variant & 3, (F.variant & kFusedXcd) != 0>                  \
+      <<<d.nFTiles[I], F.threads, 0, s>>>(d.ftiles[I], d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.type, d.e + off, \
+                                          d.e2 + off, d.rhs + off, p)
     switch (std::min(4, K - c0)) {
       case 4: GDTP_FUSED(4); break;
       case 3: GDTP_FUSED(3); break;
@@ -1497,9 +1583,7 @@ static void FusedLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedPr
 template <bool kHalf, bool kPost, bool kProlong>
 static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, FusedProlong pr) {
   static_assert(kFusedShapes == 7, "one case per fused tile shape");
-#define GDTP_SHAPE(I)                                                                                 \
-  FusedLaunch<kHalf, kPost, kProlong, kFusedShape[I][0], kFusedShape[I][1], !(kFusedVariant[I] & kFusedNoPrefetch), \
-              (kFusedVariant[I] & kFusedXcd) != 0>(d, I, K, s, pr)
+#define GDTP_SHAPE(I) FusedLaunch<kHalf, kPost, kProlong, I>(d, K, s, pr)
   switch (shape) {
     case 0: GDTP_SHAPE(0); break;
     case 1: GDTP_SHAPE(1); break;
@@ -1641,17 +1725,20 @@ __device__ __forceinline__ void K64ApplyDeepWRows(int vb, int D, const int* __re
   double cm[9], cs[9];
   SymRow(symM, N, cur, next, cm);
   SymRow(symS, N, cur, next, cs);
-  for (int ch = 0; ch < K; ch++) {
-    const double* P = p + (long)ch * N;
-    double sm = 0.0, ss = 0.0;
+  for (int c0 = 0; c0 < K; c0 += 3)
+    WithChunkWidth<3>(min(3, K - c0), [&](auto kcc) {
+      constexpr int KC = decltype(kcc)::value;
+      double pv[KC][9];
+      RowValues<KC>(p, N, c0, prev, cur, next, pv);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int base = k < 3 ? prev : (k < 6 ? cur : next);
-      const double v = P[base + k % 3 - 1];
-      sm += cm[k] * v, ss += cs[k] * v;
-    }
-    q[(long)ch * N + cur] = w[2 * ch] * sm + w[2 * ch + 1] * ss;
-  }
+      for (int cc = 0; cc < KC; cc++) {
+        const int ch = c0 + cc;
+        double sm = 0.0, ss = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) sm += cm[k] * pv[cc][k], ss += cs[k] * pv[cc][k];
+        q[(long)ch * N + cur] = w[2 * ch] * sm + w[2 * ch + 1] * ss;
+      }
+    });
 }
 
 __device__ __forceinline__ void K64ApplyBoundaryWRows(int vb, int B, const int* __restrict__ bgi, const int* __restrict__ mrpd,
@@ -1698,7 +1785,7 @@ __global__ __launch_bounds__(256) void K64ApplyW(int deepBlocks, int D, const in
                                                  const double* __restrict__ svald, const int* __restrict__ srpb, const int* __restrict__ scolb,
                                                  const double* __restrict__ svalb, const double* __restrict__ p, int N, double* q, int K,
                                                  const double* __restrict__ w) {
-  if ((int)blockIdx.x < deepBlocks) K64ApplyDeepWRows(blockIdx.x, D, deepLine, lines, symM, symS, p, N, q, K, w);
+  if ((int)blockIdx.x < deepBlocks) K64ApplyDeepWRows(XcdTile(blockIdx.x, deepBlocks), D, deepLine, lines, symM, symS, p, N, q, K, w);
   else
     K64ApplyBoundaryWRows(blockIdx.x - deepBlocks, B, bgi, mrpd, mcold, mvald, mrpb, mcolb, mvalb, srpd, scold, svald, srpb, scolb, svalb, p,
                           N, q, K, w);
@@ -1787,17 +1874,18 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, co
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
-  const int db = L0.nRaster > 0 ? (int)NBlk(L0.D, 256) : 0;
-  const int bb = L0.B > 0 ? (int)NBlk((long)L0.B * kRowLanes64, 256) : 0;
+  // deep rows: kDeepGroups x 256 per block; boundary rows: kRowLanes64 lanes each
+  const int nt = 256, db = L0.nRaster > 0 ? (int)NBlk(L0.D, 256 * kDeepGroups) : 0;
+  const int bb = L0.B > 0 ? (int)NBlk((long)L0.B * kRowLanes64, nt) : 0;
   if (db + bb == 0) return;
   if (weighted) {
     const DevLevel &M = ds.mass0, &S = ds.stiff0;
-    K64ResidualW<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, L0.B, L0.bgi, M.bd.rowptr, M.bd.col,
+    K64ResidualW<<<db + bb, nt, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, M.coefSym, S.coefSym, L0.B, L0.bgi, M.bd.rowptr, M.bd.col,
                                          M.bd.val, M.bb.rowptr, M.bb.col, M.bb.val, S.bd.rowptr, S.bd.col, S.bd.val, S.bb.rowptr, S.bb.col,
                                          S.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, ms.weights, r64, bnorms);
     return;
   }
-  K64Residual<<<db + bb, 256, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, L0.coefSym, L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val,
+  K64Residual<<<db + bb, nt, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, L0.coefSym, L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val,
                                       L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, bnorms);
 }
 
@@ -1922,6 +2010,10 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     return ge;
   };
   MixedSystem::GraphEntry* ge = useGraph ? graphFor(cycles) : nullptr;
+  // (Measured and not kept: sizing the V-cycles of later steps to the per-cycle reduction the first
+  // step showed, so that C2 would run 2 + 4 V-cycles on two FP64 residuals instead of 2 + 2 + 2 on
+  // three.  Four V-cycles on one FP32 correction equation reduce its residual less than two and two
+  // with a fresh FP64 residual between them, and C2 needed a third step: 7 V-cycles, 16.2 ms.)
   std::vector<double> prev(relRes, relRes + K);
   int it = 0;
   while (it < maxIter) {

@@ -32,6 +32,7 @@ enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS
 enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
 enum { GDTP_ASSEMBLY_DEVICE = 1 };
 enum { GDTP_FUSE_PROLONGATION = 1 };
+enum { GDTP_TILE_SHAPE_AUTO = -3 };
 
 typedef struct gdtp_mesh gdtp_mesh;
 typedef struct gdtp_hierarchy gdtp_hierarchy;
@@ -211,19 +212,28 @@ int gdtp_mixed_acceleration(gdtp_system* s, int flexible_cg);
  * residual grows continues with 1.  No counterpart in the reference, whose solve loop is VCycle
  * repeated (TextureFiltering.cpp:562-579). */
 int gdtp_mixed_cycles(gdtp_system* s, int cycles);
-/* Tile shape of the mixed solver's deep sweep on levels swept by tiles: 0..3 = one fused launch per
- * sweep (all four colours, the tile and its halo staged in LDS) on tiles of 64x8, 64x4, 128x4 or
- * 32x16 sub-grid texels (the default 0); -1 = one launch per colour on 64x4 tiles.  Every shape gives
- * bitwise the same iterate.  shape -2 only reports; `current` (may be NULL) receives the shape in
- * use.  No counterpart in the reference. */
+/* Tile shape of the mixed solver's deep sweep on levels swept by tiles: 0..6 = one fused launch per
+ * sweep (all four colours, the tile and its halo staged in LDS; the shapes' tile sizes, threads
+ * and variants are listed by gdtp_mixed_tile_shape_info); -1 = one launch per colour on 64x4
+ * tiles; GDTP_TILE_SHAPE_AUTO (the default) = per level, one launch per colour where a colour plane
+ * holds more than 2^21 texels, else shape 2.  Every shape gives bitwise the same iterate.  Setting a
+ * shape here replaces any per-level shapes.  shape -2 only reports; `current` (may be NULL)
+ * receives the setting (GDTP_TILE_SHAPE_AUTO included).  No counterpart in the reference. */
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
-/* The same for one level only (the best shape differs between levels); shape -2 only reports. */
+/* Fused tile shape `shape` (0..6): tile width x height in sub-grid texels (one texel of each of
+ * the four colour planes), threads per workgroup, and the variant bits: 3 = prefetch depth (0:
+ * each colour's operands fetched right before it, 1: one colour ahead, 2: all four colours at the
+ * start), 4 = XCD-contiguous tile order.  Any pointer may be NULL. */
+int gdtp_mixed_tile_shape_info(int shape, int* width, int* height, int* threads, int* variant);
+/* The same for one level only (the best shape differs between levels); shape -2 only reports;
+ * `current` receives the shape the level uses (-1..6, the automatic choice resolved). */
 int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current);
-/* Transfers fused into the mixed solver's relaxation kernels: GDTP_FUSE_PROLONGATION (the
- * default) lets the first post-smoothing sweep of every level swept by fused tiles add the bilinear
- * prolongation of the coarse correction itself, instead of a separate pass over the level; 0 keeps
- * the separate pass.  Both give bitwise the same iterate.  No counterpart in the reference
- * (AddProlongation then Relaxation, IterativeSolvers.inl:1122-1168, 174-395). */
+/* Transfers fused into the mixed solver's relaxation kernels: GDTP_FUSE_PROLONGATION lets the first
+ * post-smoothing sweep of every level swept by fused tiles add the bilinear prolongation of the
+ * coarse correction itself, instead of a separate pass over the level; 0 (the default) keeps the
+ * separate pass, which measured faster on every level at C2 (level 1: 77 against 87 us for the
+ * prolongation and the first post sweep).  Both give bitwise the same iterate.  No counterpart in
+ * the reference (AddProlongation then Relaxation, IterativeSolvers.inl:1122-1168, 174-395). */
 int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags);
 /* One four-colour deep Gauss-Seidel sweep of the mixed solver on `level` (its internal FP32
  * vectors); exposed to time the dominant kernel. */
