@@ -280,6 +280,9 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
   else if (name == "tasks.blockDeepEnd") v = L.tasks.blockDeepEnd;
   else if (name == "tasks.blockPadEnd") v = L.tasks.blockPadEnd;
   else if (name == "tasks.linePtr") v = L.tasks.linePtr;
+  else if (name == "tasks.lineRow") v = L.tasks.lineRow;
+  else if (name == "tasks.blockStart") v = L.tasks.blockStart;
+  else if (name == "tasks.taskSteps") v = L.tasks.taskSteps;
   else if (name == "tasks.segments") { for (auto& s : L.tasks.segments) v.insert(v.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart}); }
   else if (name == "charts") {
     for (auto& c : L.charts)

@@ -54,6 +54,8 @@ struct DevLevel {
   // band tasks
   int nTasks = 0;
   int *taskPtr = nullptr, *blockBegin = nullptr, *blockDeepEnd = nullptr, *blockPadEnd = nullptr, *linePtr = nullptr;
+  int *blockStart = nullptr, *taskSteps = nullptr;  // pipeline schedule: first task step of each block, steps per task
+  int bandGroups = 1;                                // wave groups of the band launches (1 or kBandGroups)
   int* segs = nullptr;         // 5 ints per segment
   int nRaster = 0, nRestr = 0, nProl = 0;
   int *raster = nullptr, *restr = nullptr, *prol = nullptr;  // 5, 5, 6 ints per line

@@ -11,7 +11,6 @@
 namespace gdtp {
 
 static constexpr int kWave = 64;
-static constexpr int kKc = 4;  // channels processed together in registers
 static constexpr int kDenseTopMax = 4096;  // largest LDL^T top solved through a dense inverse
 
 // ------------------------------------------------------------------ band Gauss-Seidel
@@ -42,17 +41,45 @@ __device__ __forceinline__ int FindElem(SegRef S, int s0, int s1, int e, int& of
   return -1;
 }
 
-__device__ __forceinline__ void GsRow(SegRef S, int s0, int s1, const double* __restrict__ coef, int D, double* x,
-                                      const double* __restrict__ b, int K, int ld) {
+// Where one lane's texels of a row are: up to three elements (a pass takes elements 2 lane and
+// 2 lane + 1, for the scan; a residual row lane, lane + 64 and lane + 128), each with its global
+// index g, the rows above and below (p, n), its deep index d and whether it starts a segment.
+// Locating is pure index work on the task arrays, independent of x: with the blocks of a task
+// overlapped in time, a wave locates the row of its next step before the barrier that ends the
+// current one (measured no faster, and 4 % slower at C2 level 0, when blocks run one at a time).
+struct RowLoc {
+  int g[3], p[3], n[3], d[3];
+  bool first[3];
+};
+template <int kElems, int kStride, int kLaneMul>
+__device__ __forceinline__ RowLoc LocateRow(SegRef S, int s0, int s1) {
   const int lane = threadIdx.x % kWave;
-  int off[2], seg[2], g[2] = {0, 0}, p[2] = {0, 0}, n[2] = {0, 0}, d[2] = {0, 0};
+  RowLoc L;
+  for (int q = 0; q < 3; q++) L.g[q] = -1, L.p[q] = L.n[q] = L.d[q] = 0, L.first[q] = false;
+#pragma unroll
+  for (int q = 0; q < kElems; q++) {
+    int off = 0;
+    const int sg = FindElem(S, s0, s1, kLaneMul * lane + kStride * q, off);
+    if (sg >= 0) {
+      L.g[q] = S.cs(sg) + off, L.p[q] = S.ps(sg) + off, L.n[q] = S.ns(sg) + off, L.d[q] = S.ds(sg) + off;
+      L.first[q] = off == 0;
+    }
+  }
+  return L;
+}
+
+// One left-to-right Gauss-Seidel sweep of a row (UpdateRow, IterativeSolvers.inl:448-470): the
+// in-row dependence x_i = beta_i - c3_i x_{i-1} as a wavefront scan over the lanes' element pairs,
+// columns in chunks of kKc (registers: the chunk's values and scan state).
+template <int kKc>
+__device__ __forceinline__ void GsRow(const RowLoc& L, const double* __restrict__ coef, int D, double* x, const double* __restrict__ b,
+                                      int K, int ld) {
+  const int lane = threadIdx.x % kWave;
   double a[2] = {0, 0}, c[2][9];
   for (int q = 0; q < 2; q++) {
-    seg[q] = FindElem(S, s0, s1, 2 * lane + q, off[q]);
-    if (seg[q] >= 0) {
-      g[q] = S.cs(seg[q]) + off[q], p[q] = S.ps(seg[q]) + off[q], n[q] = S.ns(seg[q]) + off[q], d[q] = S.ds(seg[q]) + off[q];
-      for (int k = 0; k < 9; k++) c[q][k] = coef[(size_t)k * D + d[q]];
-      a[q] = off[q] == 0 ? 0.0 : -c[q][3];
+    if (L.g[q] >= 0) {
+      for (int k = 0; k < 9; k++) c[q][k] = coef[(size_t)k * D + L.d[q]];
+      a[q] = L.first[q] ? 0.0 : -c[q][3];
     }
   }
   for (int c0 = 0; c0 < K; c0 += kKc) {
@@ -61,13 +88,14 @@ __device__ __forceinline__ void GsRow(SegRef S, int s0, int s1, const double* __
     for (int q = 0; q < 2; q++)
       for (int cc = 0; cc < kKc; cc++) {
         beta[q][cc] = 0.0;
-        if (seg[q] < 0 || cc >= kc) continue;
+        if (L.g[q] < 0 || cc >= kc) continue;
         const double* X = x + (size_t)(c0 + cc) * ld;
         const double* B = b + (size_t)(c0 + cc) * ld;
-        double s = X[p[q] - 1] * c[q][0] + X[p[q]] * c[q][1] + X[p[q] + 1] * c[q][2] + X[g[q] + 1] * c[q][5] +
-                   X[n[q] - 1] * c[q][6] + X[n[q]] * c[q][7] + X[n[q] + 1] * c[q][8];
-        double v = B[g[q]] * c[q][4] - s;
-        if (off[q] == 0) v -= X[g[q] - 1] * c[q][3];
+        const int g = L.g[q], p = L.p[q], n = L.n[q];
+        double s = X[p - 1] * c[q][0] + X[p] * c[q][1] + X[p + 1] * c[q][2] + X[g + 1] * c[q][5] + X[n - 1] * c[q][6] + X[n] * c[q][7] +
+                   X[n + 1] * c[q][8];
+        double v = B[g] * c[q][4] - s;
+        if (L.first[q]) v -= X[g - 1] * c[q][3];
         beta[q][cc] = v;
       }
     // lane-local composition of the two elements, then an inclusive wavefront scan
@@ -90,22 +118,21 @@ __device__ __forceinline__ void GsRow(SegRef S, int s0, int s1, const double* __
       double x1 = a[1] * x0 + beta[1][cc];
       if (cc < kc) {
         double* X = x + (size_t)(c0 + cc) * ld;
-        if (seg[0] >= 0) X[g[0]] = x0;
-        if (seg[1] >= 0) X[g[1]] = x1;
+        if (L.g[0] >= 0) X[L.g[0]] = x0;
+        if (L.g[1] >= 0) X[L.g[1]] = x1;
       }
     }
   }
 }
 
-__device__ __forceinline__ void ResRow(SegRef S, int s0, int s1, const double* __restrict__ coef, int D, const double* x,
+// The true residual of a padded row (UpdateResidual, IterativeSolvers.inl:473-495).
+__device__ __forceinline__ void ResRow(const RowLoc& L, const double* __restrict__ coef, int D, const double* x,
                                        const double* __restrict__ b, double* r, int K, int ld) {
-  int total = 0;
-  for (int s = s0; s < s1; s++) total += S.ce(s) - S.cs(s) + 1;
-  for (int e = threadIdx.x % kWave; e < total; e += kWave) {
-    int off, sg = FindElem(S, s0, s1, e, off);
-    int g = S.cs(sg) + off, p = S.ps(sg) + off, n = S.ns(sg) + off, d = S.ds(sg) + off;
+  for (int q = 0; q < 3; q++) {
+    if (L.g[q] < 0) continue;
+    const int g = L.g[q], p = L.p[q], n = L.n[q];
     double c[10];
-    for (int k = 0; k < 10; k++) c[k] = coef[(size_t)k * D + d];
+    for (int k = 0; k < 10; k++) c[k] = coef[(size_t)k * D + L.d[q]];
     for (int cc = 0; cc < K; cc++) {
       const double* X = x + (size_t)cc * ld;
       double s = X[p - 1] * c[0] + X[p] * c[1] + X[p + 1] * c[2] + X[g - 1] * c[3] + X[g] + X[g + 1] * c[5] + X[n - 1] * c[6] +
@@ -115,37 +142,98 @@ __device__ __forceinline__ void ResRow(SegRef S, int s0, int s1, const double* _
   }
 }
 
-// One workgroup of four waves per band task; within a block the three lagged passes and the
-// residual rows run as a pipeline, one row per wave per step: at step t wave 0 relaxes row t
-// (first pass), wave 1 row t-2 (second pass), wave 2 row t-4 (third pass) and wave 3 computes the
-// residual of padded row t-6.  Each operation sees exactly the values it sees in the reference's
-// order (step j: rows j, j-1, j-2, then padded row j-3, IterativeSolvers.inl:497-509): a pass
-// reads its row's neighbours above after their own pass and below before it, which the two-step
-// lag preserves, and a padded row's residual reads deep rows up to one past its own index, final
-// after their third pass (done by step pr+5).  Rows of one step are disjoint.
-__global__ __launch_bounds__(4 * kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr,
-                                                     const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
-                                                     const int* __restrict__ blockPadEnd, const int* __restrict__ linePtr,
-                                                     const int* __restrict__ segs, const double* __restrict__ coef, int D, double* x,
-                                                     const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
+// One workgroup per band task.  Within a block the three lagged passes and the residual rows run as
+// a pipeline, one row per wave per step: at local step t wave 0 relaxes row t (first pass), wave 1
+// row t-2 (second pass), wave 2 row t-4 (third pass) and wave 3 computes the residual of padded row
+// t-6.  Each operation sees exactly the values it sees in the reference's order (step j: rows j,
+// j-1, j-2, then padded row j-3, IterativeSolvers.inl:497-509): a pass reads its row's neighbours
+// above after their own pass and below before it, which the two-step lag preserves, and a padded
+// row's residual reads deep rows up to one past its own index, final after their third pass (done
+// by step pr+5).  Rows of one step are disjoint.
+// With G = 1 the blocks of a task (left to right, overlapping by two columns,
+// IterativeSolvers.inl:511-515) run one after another.  With G = 3 they also overlap in time: three
+// groups of four waves take blocks in turn, block b running its local steps at task steps
+// blockStart[b] + t; the host schedule (BandSchedule) starts a block only once no chart row it
+// touches (reads or writes, rows and residuals, one row either side) is still touched by the block
+// before it, so every value is read and written in the reference's order.  G = 3 pays where a
+// launch has fewer tasks than CUs and several blocks per task; with more tasks, the 12-wave
+// workgroups (about 160 registers a lane) fit one per CU and the launch takes several rounds.
+struct BandBlock {  // one block's rows as the band kernel walks them
+  int r0 = 0, p0 = 0, nDeep = 0, nPad = 0, steps = 0;
+};
+__device__ __forceinline__ BandBlock LoadBlock(const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
+                                               const int* __restrict__ blockPadEnd, int blk) {
+  BandBlock B;
+  B.r0 = blockBegin[blk], B.p0 = blockDeepEnd[blk];
+  B.nDeep = B.p0 - B.r0, B.nPad = blockPadEnd[blk] - B.p0;
+  B.steps = max(B.nDeep + 4, B.nPad + 6);
+  return B;
+}
+// The row wave gw (0..3 in its group) works on at local step ls of block B: 1 pass, 2 residual, 0 none.
+__device__ __forceinline__ int LocateStep(SegRef S, const int* __restrict__ linePtr, const BandBlock& B, int gw, int ls,
+                                          int withResidual, RowLoc& L) {
+  if (gw < 3) {
+    const int row = ls - 2 * gw;
+    if (row < 0 || row >= B.nDeep) return 0;
+    L = LocateRow<2, 1, 2>(S, linePtr[B.r0 + row], linePtr[B.r0 + row + 1]);
+    return 1;
+  }
+  const int pr = ls - 6;
+  if (!withResidual || pr < 0 || pr >= B.nPad) return 0;
+  L = LocateRow<3, kWave, 1>(S, linePtr[B.p0 + pr], linePtr[B.p0 + pr + 1]);
+  return 2;
+}
+template <int G, int KC>
+__global__ __launch_bounds__(G * 4 * kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr,
+                                                         const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
+                                                         const int* __restrict__ blockPadEnd, const int* __restrict__ linePtr,
+                                                         const int* __restrict__ blockStart, const int* __restrict__ taskSteps,
+                                                         const int* __restrict__ segs, const double* __restrict__ coef, int D, double* x,
+                                                         const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
   const int t = 2 * blockIdx.x + color;
   if (t >= nTasks) return;
-  const int wave = threadIdx.x / kWave;
+  const int wave = threadIdx.x / kWave, group = wave / 4, gw = wave % 4;
   SegRef S{segs};
-  for (int blk = taskPtr[t]; blk < taskPtr[t + 1]; blk++) {
-    const int r0 = blockBegin[blk], p0 = blockDeepEnd[blk];
-    const int nDeep = p0 - r0, nPad = blockPadEnd[blk] - p0;
-    const int steps = withResidual ? max(nDeep + 4, nPad + 6) : nDeep + 4;
-    for (int st = 0; st < steps; st++) {
-      if (wave < 3) {
-        const int row = st - 2 * wave;
-        if (row >= 0 && row < nDeep) GsRow(S, linePtr[r0 + row], linePtr[r0 + row + 1], coef, D, x, b, K, ld);
-      } else if (withResidual) {
-        const int pr = st - 6;
-        if (pr >= 0 && pr < nPad) ResRow(S, linePtr[p0 + pr], linePtr[p0 + pr + 1], coef, D, x, b, r, K, ld);
+  const int b1 = taskPtr[t + 1];
+  if constexpr (G == 1) {  // blocks one after another, each step's row located at the step
+    for (int blk = taskPtr[t]; blk < b1; blk++) {
+      const BandBlock B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk);
+      const int steps = withResidual ? B.steps : B.nDeep + 4;
+      for (int st = 0; st < steps; st++) {
+        RowLoc L;
+        const int kind = LocateStep(S, linePtr, B, gw, st, withResidual, L);
+        if (kind == 1) GsRow<KC>(L, coef, D, x, b, K, ld);
+        else if (kind == 2) ResRow(L, coef, D, x, b, r, K, ld);
+        __syncthreads();
       }
-      __syncthreads();
     }
+    return;
+  }
+  // this group's current block (every G-th block of the task) and the task step it starts at
+  int blk = taskPtr[t] + group;
+  BandBlock B;
+  int bStart = 0;
+  if (blk < b1) B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk), bStart = blockStart[blk];
+  const int total = taskSteps[t];
+  RowLoc cur, nxt;
+  int curKind = blk < b1 && bStart == 0 ? LocateStep(S, linePtr, B, gw, 0, withResidual, cur) : 0;
+  for (int st = 0;; st++) {
+    // the group's block and local step at task step st + 1 (located now, used after the barrier)
+    int nblk = blk, nStart = bStart;
+    BandBlock NB = B;
+    if (blk < b1 && st + 1 >= bStart + B.steps) {
+      nblk = blk + G;
+      if (nblk < b1) NB = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, nblk), nStart = blockStart[nblk];
+    }
+    const bool more = st + 1 < total;
+    if (curKind == 1) GsRow<KC>(cur, coef, D, x, b, K, ld);
+    else if (curKind == 2) ResRow(cur, coef, D, x, b, r, K, ld);
+    // locate the next step's row before the barrier: its index loads overlap the other waves' work
+    int nxtKind = 0;
+    if (more && nblk < b1 && st + 1 >= nStart) nxtKind = LocateStep(S, linePtr, NB, gw, st + 1 - nStart, withResidual, nxt);
+    __syncthreads();
+    if (!more) break;
+    blk = nblk, bStart = nStart, B = NB, cur = nxt, curKind = nxtKind;
   }
 }
 
@@ -463,6 +551,19 @@ static T* UploadTo(std::vector<void*>& allocs, const std::vector<T>& v) {
   return p;
 }
 
+// Wave groups of a level's band launches: kBandGroups (blocks of a task overlapped in time) where a
+// launch has no more tasks than the device has CUs (its 12-wave workgroups then run in one round)
+// and the tasks have two or more blocks on average (else there is little to overlap); else 1.
+// (C2 4096^2: levels 0 and 1 have 640 and 320 tasks per launch, level 2 160 tasks of 2.2 blocks.)
+static int BandGroups(const BandTasks& T) {
+  const int nt = (int)T.taskPtr.size() - 1;
+  if (nt <= 0) return 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+  const double blocksPerTask = double(T.blockBegin.size()) / nt;
+  return (nt + 1) / 2 <= cus && blocksPerTask >= 2.0 ? kBandGroups : 1;
+}
+
 template <class T>
 static T* Upload(DeviceSystem& ds, const std::vector<T>& v) {
   return UploadTo(ds.allocations, v);
@@ -672,6 +773,9 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     d.blockDeepEnd = Upload(ds, H.tasks.blockDeepEnd);
     d.blockPadEnd = Upload(ds, H.tasks.blockPadEnd);
     d.linePtr = Upload(ds, H.tasks.linePtr);
+    d.blockStart = Upload(ds, H.tasks.blockStart);
+    d.taskSteps = Upload(ds, H.tasks.taskSteps);
+    d.bandGroups = BandGroups(H.tasks);
     std::vector<int> segs;
     segs.reserve(H.tasks.segments.size() * 5);
     for (auto& s : H.tasks.segments) segs.insert(segs.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart});
@@ -753,11 +857,28 @@ void BandSweepLevel(DeviceSystem& ds, int l, double* x, const double* b, double*
   DevLevel& d = ds.L[l];
   if (d.nTasks == 0) return;
   const int withResidual = r != nullptr ? 1 : 0;
-  KBandGs<<<(d.nTasks + 1) / 2, 4 * kWave, 0, s>>>(0, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs,
-                                                 d.coef, d.D, x, b, r, K, ld, withResidual);
-  if (d.nTasks > 1)
-    KBandGs<<<d.nTasks / 2, 4 * kWave, 0, s>>>(1, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, d.segs,
-                                             d.coef, d.D, x, b, r, K, ld, withResidual);
+#define GDTP_BAND(G, KC)                                                                                                       \
+  KBandGs<G, KC><<<blocks, G * 4 * kWave, 0, s>>>(color, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, \
+                                                  d.blockStart, d.taskSteps, d.segs, d.coef, d.D, x, b, r, K, ld, withResidual)
+  // column chunks of K up to three, else of two (a chunk of four needs 168-179 registers a lane:
+  // occupancy 2 or spills)
+#define GDTP_BAND_KC(G)             \
+  switch (K >= 4 ? 2 : K) {         \
+    case 1: GDTP_BAND(G, 1); break; \
+    case 2: GDTP_BAND(G, 2); break; \
+    default: GDTP_BAND(G, 3); break; \
+  }
+  for (int color = 0; color < 2; color++) {
+    const int blocks = color == 0 ? (d.nTasks + 1) / 2 : d.nTasks / 2;
+    if (blocks == 0) continue;
+    if (d.bandGroups == kBandGroups) {
+      GDTP_BAND_KC(kBandGroups)
+    } else {
+      GDTP_BAND_KC(1)
+    }
+  }
+#undef GDTP_BAND_KC
+#undef GDTP_BAND
 }
 
 void RelaxLevel(DeviceSystem& ds, int l, double* x, const double* b, double* r, int K, int ld, bool withResidual, hipStream_t s) {

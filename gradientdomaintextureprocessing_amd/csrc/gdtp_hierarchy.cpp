@@ -2,6 +2,9 @@
 // Each function names the reference routine whose semantics it follows.
 #include "gdtp_hierarchy.h"
 
+#include <algorithm>
+#include <cstdint>
+
 #include <cstring>
 #include <map>
 #include <queue>
@@ -374,24 +377,27 @@ static int BuildGridChart(int chartId, const AtlasChart& ac, GridChart& g, Level
     for (int bI = 0; bI < nH; bI++) {
       int hS = bI * hOff, hE = std::min((bI + 1) * hOff + bi.paddingWidth + 2 - 1, w - 1);
       std::vector<std::vector<Segment>> deepRows, padRows;
+      std::vector<int> deepJ, padJ;
       for (int j = vS + 1; j <= vE - 1; j++) {
         if (!FindSegments(g, j, hS + 1, hE - 1, segs)) { err = "invalid block segment"; return 0; }
-        if (!segs.empty()) deepRows.push_back(segs);
+        if (!segs.empty()) deepRows.push_back(segs), deepJ.push_back(j);
       }
       if (deepRows.empty()) continue;
       for (int j = vS; j <= vE; j++) {
         if (!FindSegments(g, j, hS, hE, segs)) { err = "invalid block segment"; return 0; }
-        if (!segs.empty()) padRows.push_back(segs);
+        if (!segs.empty()) padRows.push_back(segs), padJ.push_back(j);
       }
       T.blockBegin.push_back((int)T.linePtr.size() - 1);
-      for (auto& r : deepRows) {
-        T.segments.insert(T.segments.end(), r.begin(), r.end());
+      for (size_t k = 0; k < deepRows.size(); k++) {
+        T.segments.insert(T.segments.end(), deepRows[k].begin(), deepRows[k].end());
         T.linePtr.push_back((int)T.segments.size());
+        T.lineRow.push_back(deepJ[k]);
       }
       T.blockDeepEnd.push_back((int)T.linePtr.size() - 1);
-      for (auto& r : padRows) {
-        T.segments.insert(T.segments.end(), r.begin(), r.end());
+      for (size_t k = 0; k < padRows.size(); k++) {
+        T.segments.insert(T.segments.end(), padRows[k].begin(), padRows[k].end());
         T.linePtr.push_back((int)T.segments.size());
+        T.lineRow.push_back(padJ[k]);
       }
       T.blockPadEnd.push_back((int)T.linePtr.size() - 1);
     }
@@ -436,6 +442,7 @@ static int BuildLevel(const Atlas& A, double cw, double ch, const BlockInfo& bi,
         if (nt == 0 || nt == 1) L.boundaryGlobalIndex.push_back(g), L.boundaryAndDeepIndex[g] = b++, g++;
         else if (nt == 2) L.deepGlobalIndex.push_back(g), L.boundaryAndDeepIndex[g] = d--, g++;
       }
+  BandSchedule(L.tasks);
   return 1;
 }
 
@@ -567,6 +574,64 @@ int BuildHierarchy(const TexturedMesh& mesh, int W, int H, int numLevels, const 
     if (!BuildProlongation(h.levels[l], h.levels[l + 1], err)) return 0;
   }
   return 1;
+}
+
+// Pipeline schedule of the band tasks (the GPU band kernel): block b of a task runs its local steps
+// t = 0 .. max(nDeep + 4, nPad + 6) - 1 at task steps blockStart[b] + t.  At local step t it touches (reads
+// or writes) chart rows Y - 1 .. Y + 1 of deep row i = t - 2w (passes w = 0, 1, 2) and of padded row
+// t - 6 (residual).  A block starts once every row it shares with the block before it is touched
+// by it for the last time, so that each row's accesses keep the reference's block order (blocks
+// further apart share no columns), and once the wave group it takes (every kBandGroups-th block)
+// is free.  taskSteps[t]: task steps of task t.
+void BandSchedule(BandTasks& T) {
+  const int nb = (int)T.blockBegin.size(), nt = (int)T.taskPtr.size() - 1;
+  std::vector<int>& start = T.blockStart;
+  std::vector<int>& steps = T.taskSteps;
+  start.assign(nb, 0), steps.assign(std::max(nt, 0), 0);
+  auto blockSteps = [&](int b) {
+    return std::max(T.blockDeepEnd[b] - T.blockBegin[b] + 4, T.blockPadEnd[b] - T.blockDeepEnd[b] + 6);
+  };
+  // first and last local step at which block b touches each chart row (rows as a span [y0, y0 + n))
+  struct Touch {
+    int y0 = 0;
+    std::vector<int> first, last;
+  };
+  auto touches = [&](int b) {
+    Touch tc;
+    int lo = INT32_MAX, hi = INT32_MIN;
+    for (int l = T.blockBegin[b]; l < T.blockPadEnd[b]; l++) lo = std::min(lo, T.lineRow[l]), hi = std::max(hi, T.lineRow[l]);
+    tc.y0 = lo - 1;
+    tc.first.assign(hi - lo + 3, INT32_MAX), tc.last.assign(hi - lo + 3, -1);
+    auto mark = [&](int y, int t) {
+      for (int dy = -1; dy <= 1; dy++) {
+        const int k = y + dy - tc.y0;
+        tc.first[k] = std::min(tc.first[k], t), tc.last[k] = std::max(tc.last[k], t);
+      }
+    };
+    for (int i = 0; i < T.blockDeepEnd[b] - T.blockBegin[b]; i++)
+      for (int w = 0; w < 3; w++) mark(T.lineRow[T.blockBegin[b] + i], i + 2 * w);
+    for (int i = 0; i < T.blockPadEnd[b] - T.blockDeepEnd[b]; i++) mark(T.lineRow[T.blockDeepEnd[b] + i], i + 6);
+    return tc;
+  };
+  for (int t = 0; t < nt; t++) {
+    Touch prev;
+    for (int b = T.taskPtr[t]; b < T.taskPtr[t + 1]; b++) {
+      Touch cur = touches(b);
+      int s0 = 0;
+      if (b > T.taskPtr[t]) {
+        s0 = start[b - 1];
+        for (size_t k = 0; k < cur.first.size(); k++) {
+          const int y = cur.y0 + (int)k, kp = y - prev.y0;
+          if (cur.last[k] < 0 || kp < 0 || kp >= (int)prev.last.size() || prev.last[kp] < 0) continue;
+          s0 = std::max(s0, start[b - 1] + prev.last[kp] - cur.first[k] + 1);
+        }
+        if (b - kBandGroups >= T.taskPtr[t]) s0 = std::max(s0, start[b - kBandGroups] + blockSteps(b - kBandGroups));
+      }
+      start[b] = s0;
+      steps[t] = std::max(steps[t], s0 + blockSteps(b));
+      prev = std::move(cur);
+    }
+  }
 }
 
 }  // namespace gdtp

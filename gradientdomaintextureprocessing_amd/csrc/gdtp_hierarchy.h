@@ -79,10 +79,19 @@ struct ProlongationLine { int start, length, center, prev, next, aligned; };
 // Flattened band tasks: task t owns blocks [taskPtr[t], taskPtr[t+1]); block b owns deep rows
 // [lineBegin[b], deepEnd[b]) then padded rows [deepEnd[b], padEnd[b]); row r owns segments
 // [linePtr[r], linePtr[r+1]).
+// Wave groups of the GPU band kernel: blocks of one band task in flight at once.
+constexpr int kBandGroups = 3;
+
 struct BandTasks {
   std::vector<int> taskPtr{0}, blockBegin, blockDeepEnd, blockPadEnd, linePtr{0};
   std::vector<Segment> segments;
+  std::vector<int> lineRow;    // chart grid row of every line (deep and padded)
+  std::vector<int> blockStart; // GPU pipeline: first task step of every block (BandSchedule)
+  std::vector<int> taskSteps;  // GPU pipeline: task steps of every task
 };
+
+// The GPU band kernel's pipeline schedule of T (fills blockStart and taskSteps).
+void BandSchedule(BandTasks& T);
 
 struct Level {
   std::vector<GridChart> charts;

@@ -60,6 +60,30 @@ def test_band_relax_with_residual(case):
     assert relerr(r_gpu, r) < 1e-10
 
 
+def test_band_relax_pipelined_blocks():
+    """A level whose band launches overlap the blocks of a task in time (few tasks, several blocks
+    each: the cube's 341-texel-wide charts at 1024^2) gives the oracle's sequential sweep."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _, H, S = cube_system(1024, 2, max_cols=3)
+    info = H.level_info(0)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert (info["tasks"] + 1) // 2 <= cus and info["blocks"] / info["tasks"] >= 3, info  # the pipelined launch
+    L0 = oracle.level_from_product(H, S, 0)
+    x0, b = noise_texture_rhs(H, L0, K=3)
+    xg, bg, rg = dev(x0), dev(b), torch.zeros((3, L0.N), dtype=torch.float64, device="cuda")
+    print(f"[pipelined band] N={L0.N} tasks={info['tasks']} blocks={info['blocks']}", flush=True)
+    S.relax(0, xg, bg, rg)
+    x_gpu, r_gpu = host(xg), host(rg)
+    x = x0.copy()
+    x[:, L0.bgi] = x_gpu[:, L0.bgi]
+    r = np.zeros_like(x)
+    oracle.band_relax(L0, x, b, r)
+    oracle.boundary_residual(L0, x, b, r)
+    assert relerr(x_gpu, x) < 1e-11
+    assert relerr(r_gpu, r) < 1e-10
+
+
 def test_boundary_and_coarse_solves(case):
     name, H, S, L = case
     rng = np.random.default_rng(11)
