@@ -53,10 +53,11 @@ struct DevLevel {
   DevCsr bd, bb, brestr;       // A_BD, A_BB, boundary restriction to level+1
   // band tasks
   int nTasks = 0;
-  int *taskPtr = nullptr, *blockBegin = nullptr, *blockDeepEnd = nullptr, *blockPadEnd = nullptr, *linePtr = nullptr;
+  int *taskPtr = nullptr, *blockBegin = nullptr, *blockDeepEnd = nullptr, *blockPadEnd = nullptr;
   int *blockStart = nullptr, *taskSteps = nullptr;  // pipeline schedule: first task step of each block, steps per task
   int bandGroups = 1;                                // wave groups of the band launches (1 or kBandGroups)
-  int* segs = nullptr;         // 5 ints per segment
+  int* lineElem = nullptr;                           // row elements of every band line (offsets)
+  int4* elems = nullptr;                             // (g, above, below, deep index or -d-1 at a segment start)
   int nRaster = 0, nRestr = 0, nProl = 0;
   int *raster = nullptr, *restr = nullptr, *prol = nullptr;  // 5, 5, 6 ints per line
   DevLdlt factor;              // boundary factor (levels < L-1) or full coarse factor (level L-1)

@@ -17,56 +17,20 @@ static constexpr int kDenseTopMax = 4096;  // largest LDL^T top solved through a
 // The reference smoother (IterativeSolvers.inl:398-558 with residual, :174-395 without):
 // per band task, blocks left to right; inside a block, three lagged passes over the block's deep
 // rows (at step j rows j, j-1, j-2), each row a left-to-right Gauss-Seidel sweep of its deep
-// segments, and (pre-smoother) the true residual of padded row j-3.  One wavefront runs one task.
+// segments, and (pre-smoother) the true residual of padded row j-3.
 // The in-row dependence x_i = beta_i - c3_i x_{i-1} is a first-order linear recurrence, evaluated
 // with a wavefront-wide scan (two texels per lane, 128-texel rows).
 
-struct SegRef {
-  const int* s;
-  __device__ int cs(int i) const { return s[5 * i + 0]; }
-  __device__ int ce(int i) const { return s[5 * i + 1]; }
-  __device__ int ps(int i) const { return s[5 * i + 2]; }
-  __device__ int ns(int i) const { return s[5 * i + 3]; }
-  __device__ int ds(int i) const { return s[5 * i + 4]; }
-};
-
-// Locate element e of a row made of segments [s0, s1); returns the segment or -1.
-__device__ __forceinline__ int FindElem(SegRef S, int s0, int s1, int e, int& off) {
-  int base = 0;
-  for (int s = s0; s < s1; s++) {
-    int len = S.ce(s) - S.cs(s) + 1;
-    if (e < base + len) { off = e - base; return s; }
-    base += len;
-  }
-  return -1;
-}
-
 // Where one lane's texels of a row are: up to three elements (a pass takes elements 2 lane and
 // 2 lane + 1, for the scan; a residual row lane, lane + 64 and lane + 128), each with its global
-// index g, the rows above and below (p, n), its deep index d and whether it starts a segment.
-// Locating is pure index work on the task arrays, independent of x: with the blocks of a task
-// overlapped in time, a wave locates the row of its next step before the barrier that ends the
-// current one (measured no faster, and 4 % slower at C2 level 0, when blocks run one at a time).
+// index g, the rows above and below (p, n), its deep index d and whether it starts a segment.  The
+// host expands every band line into a table of such elements (BandTasks::elems), so that a wave
+// finds its elements with one load that waits on nothing, issued a step ahead (LoadRow); searching
+// the line's segments instead took three or four dependent memory round trips per step.
 struct RowLoc {
   int g[3], p[3], n[3], d[3];
   bool first[3];
 };
-template <int kElems, int kStride, int kLaneMul>
-__device__ __forceinline__ RowLoc LocateRow(SegRef S, int s0, int s1) {
-  const int lane = threadIdx.x % kWave;
-  RowLoc L;
-  for (int q = 0; q < 3; q++) L.g[q] = -1, L.p[q] = L.n[q] = L.d[q] = 0, L.first[q] = false;
-#pragma unroll
-  for (int q = 0; q < kElems; q++) {
-    int off = 0;
-    const int sg = FindElem(S, s0, s1, kLaneMul * lane + kStride * q, off);
-    if (sg >= 0) {
-      L.g[q] = S.cs(sg) + off, L.p[q] = S.ps(sg) + off, L.n[q] = S.ns(sg) + off, L.d[q] = S.ds(sg) + off;
-      L.first[q] = off == 0;
-    }
-  }
-  return L;
-}
 
 // One left-to-right Gauss-Seidel sweep of a row (UpdateRow, IterativeSolvers.inl:448-470): the
 // in-row dependence x_i = beta_i - c3_i x_{i-1} as a wavefront scan over the lanes' element pairs,
@@ -162,78 +126,108 @@ struct BandBlock {  // one block's rows as the band kernel walks them
   int r0 = 0, p0 = 0, nDeep = 0, nPad = 0, steps = 0;
 };
 __device__ __forceinline__ BandBlock LoadBlock(const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
-                                               const int* __restrict__ blockPadEnd, int blk) {
+                                               const int* __restrict__ blockPadEnd, int blk, int withResidual) {
   BandBlock B;
   B.r0 = blockBegin[blk], B.p0 = blockDeepEnd[blk];
   B.nDeep = B.p0 - B.r0, B.nPad = blockPadEnd[blk] - B.p0;
-  B.steps = max(B.nDeep + 4, B.nPad + 6);
+  B.steps = withResidual ? max(B.nDeep + 4, B.nPad + 6) : B.nDeep + 4;
   return B;
 }
-// The row wave gw (0..3 in its group) works on at local step ls of block B: 1 pass, 2 residual, 0 none.
-__device__ __forceinline__ int LocateStep(SegRef S, const int* __restrict__ linePtr, const BandBlock& B, int gw, int ls,
-                                          int withResidual, RowLoc& L) {
+// The line wave gw (0..3 in its group) works on at local step ls of block B, or -1; kind 1 pass, 2 residual.
+__device__ __forceinline__ int LineOfStep(const BandBlock& B, int gw, int ls, int withResidual, int& kind) {
+  kind = 0;
   if (gw < 3) {
     const int row = ls - 2 * gw;
-    if (row < 0 || row >= B.nDeep) return 0;
-    L = LocateRow<2, 1, 2>(S, linePtr[B.r0 + row], linePtr[B.r0 + row + 1]);
-    return 1;
+    if (row < 0 || row >= B.nDeep) return -1;
+    kind = 1;
+    return B.r0 + row;
   }
   const int pr = ls - 6;
-  if (!withResidual || pr < 0 || pr >= B.nPad) return 0;
-  L = LocateRow<3, kWave, 1>(S, linePtr[B.p0 + pr], linePtr[B.p0 + pr + 1]);
-  return 2;
+  if (!withResidual || pr < 0 || pr >= B.nPad) return -1;
+  kind = 2;
+  return B.p0 + pr;
 }
+// A lane's elements of a line from the row element table (no search, no load that waits on another):
+// a pass takes elements 2 lane and 2 lane + 1, a residual row lane, lane + 64 and lane + 128.
+__device__ __forceinline__ RowLoc LoadRow(const int4* __restrict__ elems, int o0, int o1, int kind) {
+  const int lane = threadIdx.x % kWave;
+  RowLoc L;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int e = kind == 1 ? 2 * lane + q : lane + kWave * q;
+    const bool in = kind != 0 && (kind == 2 || q < 2) && o0 + e < o1;
+    const int4 v = elems[in ? o0 + e : 0];  // unconditional (element 0 where not in), then selected: no branch
+    L.g[q] = in ? v.x : -1, L.p[q] = v.y, L.n[q] = v.z, L.d[q] = v.w >= 0 ? v.w : -v.w - 1, L.first[q] = v.w < 0;
+  }
+  return L;
+}
+// Where a group's blocks stand at a task step: the block it works on (or past the task's end) and
+// the task step that block starts at.  G = 1: the blocks one after another; G > 1: every G-th block
+// of the task, at the host schedule's blockStart.
+template <int G>
+struct BandCursor {
+  int blk, b1, start;
+  BandBlock B;
+  __device__ __forceinline__ void Init(int first, int last, const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
+                                       const int* __restrict__ blockPadEnd, const int* __restrict__ blockStart, int withResidual) {
+    blk = first, b1 = last, start = 0;
+    if (blk < b1) {
+      B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk, withResidual);
+      if (G > 1) start = blockStart[blk];
+    }
+  }
+  // move to task step st (non-decreasing); returns the line of wave gw there, or -1
+  __device__ __forceinline__ int At(int st, int gw, const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
+                                    const int* __restrict__ blockPadEnd, const int* __restrict__ blockStart, int withResidual, int& kind) {
+    while (blk < b1 && st >= start + B.steps) {
+      const int end = start + B.steps;
+      blk += G;
+      if (blk < b1) B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk, withResidual), start = G == 1 ? end : blockStart[blk];
+    }
+    kind = 0;
+    if (blk >= b1 || st < start) return -1;
+    return LineOfStep(B, gw, st - start, withResidual, kind);
+  }
+};
 template <int G, int KC>
 __global__ __launch_bounds__(G * 4 * kWave) void KBandGs(int color, int nTasks, const int* __restrict__ taskPtr,
                                                          const int* __restrict__ blockBegin, const int* __restrict__ blockDeepEnd,
-                                                         const int* __restrict__ blockPadEnd, const int* __restrict__ linePtr,
-                                                         const int* __restrict__ blockStart, const int* __restrict__ taskSteps,
-                                                         const int* __restrict__ segs, const double* __restrict__ coef, int D, double* x,
-                                                         const double* __restrict__ b, double* r, int K, int ld, int withResidual) {
+                                                         const int* __restrict__ blockPadEnd, const int* __restrict__ lineElem,
+                                                         const int4* __restrict__ elems, const int* __restrict__ blockStart,
+                                                         const int* __restrict__ taskSteps, const double* __restrict__ coef, int D,
+                                                         double* x, const double* __restrict__ b, double* r, int K, int ld,
+                                                         int withResidual) {
   const int t = 2 * blockIdx.x + color;
   if (t >= nTasks) return;
   const int wave = threadIdx.x / kWave, group = wave / 4, gw = wave % 4;
-  SegRef S{segs};
-  const int b1 = taskPtr[t + 1];
-  if constexpr (G == 1) {  // blocks one after another, each step's row located at the step
-    for (int blk = taskPtr[t]; blk < b1; blk++) {
-      const BandBlock B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk);
-      const int steps = withResidual ? B.steps : B.nDeep + 4;
-      for (int st = 0; st < steps; st++) {
-        RowLoc L;
-        const int kind = LocateStep(S, linePtr, B, gw, st, withResidual, L);
-        if (kind == 1) GsRow<KC>(L, coef, D, x, b, K, ld);
-        else if (kind == 2) ResRow(L, coef, D, x, b, r, K, ld);
-        __syncthreads();
-      }
-    }
-    return;
+  const int b0 = taskPtr[t], b1 = taskPtr[t + 1];
+  int total = 0;  // task steps
+  if constexpr (G == 1) {
+    for (int k = b0; k < b1; k++) total += LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, k, withResidual).steps;
+  } else {
+    total = taskSteps[t];
   }
-  // this group's current block (every G-th block of the task) and the task step it starts at
-  int blk = taskPtr[t] + group;
-  BandBlock B;
-  int bStart = 0;
-  if (blk < b1) B = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, blk), bStart = blockStart[blk];
-  const int total = taskSteps[t];
-  RowLoc cur, nxt;
-  int curKind = blk < b1 && bStart == 0 ? LocateStep(S, linePtr, B, gw, 0, withResidual, cur) : 0;
-  for (int st = 0;; st++) {
-    // the group's block and local step at task step st + 1 (located now, used after the barrier)
-    int nblk = blk, nStart = bStart;
-    BandBlock NB = B;
-    if (blk < b1 && st + 1 >= bStart + B.steps) {
-      nblk = blk + G;
-      if (nblk < b1) NB = LoadBlock(blockBegin, blockDeepEnd, blockPadEnd, nblk), nStart = blockStart[nblk];
-    }
-    const bool more = st + 1 < total;
+  // Two steps of lookahead per wave: the line of step st + 2 (its element range, scalar loads) and
+  // the elements of step st + 1 (vector loads) are fetched while step st computes, so a step waits
+  // only for its own stencil and x.
+  BandCursor<G> ahead;
+  ahead.Init(b0 + group, b1, blockBegin, blockDeepEnd, blockPadEnd, blockStart, withResidual);
+  int kind0, kind1;
+  const int l0 = ahead.At(0, gw, blockBegin, blockDeepEnd, blockPadEnd, blockStart, withResidual, kind0);
+  const int l1 = ahead.At(1, gw, blockBegin, blockDeepEnd, blockPadEnd, blockStart, withResidual, kind1);
+  RowLoc cur = LoadRow(elems, kind0 ? lineElem[l0] : 0, kind0 ? lineElem[l0 + 1] : 0, kind0);
+  int curKind = kind0, nextKind = kind1;
+  int n0 = kind1 ? lineElem[l1] : 0, n1 = kind1 ? lineElem[l1 + 1] : 0;
+  for (int st = 0; st < total; st++) {
+    int kind2;
+    const int l2 = ahead.At(st + 2, gw, blockBegin, blockDeepEnd, blockPadEnd, blockStart, withResidual, kind2);
+    const int m0 = kind2 ? lineElem[l2] : 0, m1 = kind2 ? lineElem[l2 + 1] : 0;
+    const RowLoc nxt = LoadRow(elems, n0, n1, nextKind);
     if (curKind == 1) GsRow<KC>(cur, coef, D, x, b, K, ld);
     else if (curKind == 2) ResRow(cur, coef, D, x, b, r, K, ld);
-    // locate the next step's row before the barrier: its index loads overlap the other waves' work
-    int nxtKind = 0;
-    if (more && nblk < b1 && st + 1 >= nStart) nxtKind = LocateStep(S, linePtr, NB, gw, st + 1 - nStart, withResidual, nxt);
     __syncthreads();
-    if (!more) break;
-    blk = nblk, bStart = nStart, B = NB, cur = nxt, curKind = nxtKind;
+    cur = nxt, curKind = nextKind;
+    nextKind = kind2, n0 = m0, n1 = m1;
   }
 }
 
@@ -772,14 +766,11 @@ void UploadSystem(const Hierarchy& h, const OperatorHierarchy& oh, const FineOpe
     d.blockBegin = Upload(ds, H.tasks.blockBegin);
     d.blockDeepEnd = Upload(ds, H.tasks.blockDeepEnd);
     d.blockPadEnd = Upload(ds, H.tasks.blockPadEnd);
-    d.linePtr = Upload(ds, H.tasks.linePtr);
     d.blockStart = Upload(ds, H.tasks.blockStart);
     d.taskSteps = Upload(ds, H.tasks.taskSteps);
     d.bandGroups = BandGroups(H.tasks);
-    std::vector<int> segs;
-    segs.reserve(H.tasks.segments.size() * 5);
-    for (auto& s : H.tasks.segments) segs.insert(segs.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart});
-    d.segs = Upload(ds, segs);
+    d.lineElem = Upload(ds, H.tasks.lineElem);
+    d.elems = reinterpret_cast<int4*>(Upload(ds, H.tasks.elems));
     std::vector<int> rl;
     for (auto& x : H.rasterLines) rl.insert(rl.end(), {x.start, x.end, x.prev, x.next, x.coeff});
     d.nRaster = (int)H.rasterLines.size();
@@ -858,8 +849,9 @@ void BandSweepLevel(DeviceSystem& ds, int l, double* x, const double* b, double*
   if (d.nTasks == 0) return;
   const int withResidual = r != nullptr ? 1 : 0;
 #define GDTP_BAND(G, KC)                                                                                                       \
-  KBandGs<G, KC><<<blocks, G * 4 * kWave, 0, s>>>(color, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd, d.linePtr, \
-                                                  d.blockStart, d.taskSteps, d.segs, d.coef, d.D, x, b, r, K, ld, withResidual)
+  KBandGs<G, KC><<<blocks, G * 4 * kWave, 0, s>>>(color, d.nTasks, d.taskPtr, d.blockBegin, d.blockDeepEnd, d.blockPadEnd,  \
+                                                  d.lineElem, d.elems, d.blockStart, d.taskSteps, d.coef, d.D, x, b, r, K, ld,     \
+                                                  withResidual)
   // column chunks of K up to three, else of two (a chunk of four needs 168-179 registers a lane:
   // occupancy 2 or spills)
 #define GDTP_BAND_KC(G)             \

@@ -584,6 +584,17 @@ int BuildHierarchy(const TexturedMesh& mesh, int W, int H, int numLevels, const 
 // further apart share no columns), and once the wave group it takes (every kBandGroups-th block)
 // is free.  taskSteps[t]: task steps of task t.
 void BandSchedule(BandTasks& T) {
+  // row elements: every line's segments expanded texel by texel
+  const int nl = (int)T.linePtr.size() - 1;
+  T.lineElem.assign(1, 0), T.elems.clear();
+  for (int l = 0; l < nl; l++) {
+    for (int s = T.linePtr[l]; s < T.linePtr[l + 1]; s++) {
+      const Segment& sg = T.segments[s];
+      for (int o = 0; o <= sg.currentEnd - sg.currentStart; o++)
+        T.elems.insert(T.elems.end(), {sg.currentStart + o, sg.previousStart + o, sg.nextStart + o, o == 0 ? -(sg.deepStart + o) - 1 : sg.deepStart + o});
+    }
+    T.lineElem.push_back((int)T.elems.size() / 4);
+  }
   const int nb = (int)T.blockBegin.size(), nt = (int)T.taskPtr.size() - 1;
   std::vector<int>& start = T.blockStart;
   std::vector<int>& steps = T.taskSteps;

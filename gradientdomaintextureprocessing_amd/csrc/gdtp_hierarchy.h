@@ -88,9 +88,14 @@ struct BandTasks {
   std::vector<int> lineRow;    // chart grid row of every line (deep and padded)
   std::vector<int> blockStart; // GPU pipeline: first task step of every block (BandSchedule)
   std::vector<int> taskSteps;  // GPU pipeline: task steps of every task
+  // GPU row elements: line l's elements (its segments' texels in order) are lineElem[l] ..
+  // lineElem[l + 1] - 1, element e stored as 4 ints (global index, index above, index below, deep
+  // index d; -d - 1 where the element starts a segment)
+  std::vector<int> lineElem, elems;
 };
 
-// The GPU band kernel's pipeline schedule of T (fills blockStart and taskSteps).
+// The GPU band kernel's pipeline schedule and row element table of T (fills blockStart, taskSteps,
+// lineElem and elems).
 void BandSchedule(BandTasks& T);
 
 struct Level {
