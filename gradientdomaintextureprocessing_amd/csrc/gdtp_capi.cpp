@@ -283,6 +283,8 @@ int gdtp_hierarchy_array_i32(const gdtp_hierarchy* h, int level, const char* cna
   else if (name == "tasks.lineRow") v = L.tasks.lineRow;
   else if (name == "tasks.blockStart") v = L.tasks.blockStart;
   else if (name == "tasks.taskSteps") v = L.tasks.taskSteps;
+  else if (name == "tasks.lineElem") v = L.tasks.lineElem;
+  else if (name == "tasks.elems") v = L.tasks.elems;
   else if (name == "tasks.segments") { for (auto& s : L.tasks.segments) v.insert(v.end(), {s.currentStart, s.currentEnd, s.previousStart, s.nextStart, s.deepStart}); }
   else if (name == "charts") {
     for (auto& c : L.charts)

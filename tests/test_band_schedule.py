@@ -114,3 +114,24 @@ def test_early_block_breaks_order():
     early = start.copy()
     early[b:tp[t + 1]] -= (start[b] - start[b - 1] + 1) // 2
     assert not check_schedule(H, 0, early)
+
+
+@pytest.mark.parametrize("case", ["icosphere512", "julius512"])
+def test_band_row_elements(case):
+    """The band kernel's row element table restates every line's segments texel by texel: global
+    index, indices above and below, deep index (-d-1 where the element starts a segment)."""
+    if case == "icosphere512":
+        mesh = g.TexturedMesh.icosphere(4, 512, 512, 0.05, 7)
+    else:
+        import os
+        d = np.load(os.path.join(os.path.dirname(__file__), "golden", "julius_mesh.npz"))
+        mesh = g.TexturedMesh.from_arrays(d["vertices"], d["triangles"], d["corner_uv"])
+    H = g.InitializeHierarchy(mesh, 512, 512, 2)
+    A = lambda n: H.array(0, n).astype(np.int64)
+    seg, lp, le = A("tasks.segments").reshape(-1, 5), A("tasks.linePtr"), A("tasks.lineElem")
+    el = A("tasks.elems").reshape(-1, 4)
+    assert le[-1] == len(el)
+    for l in range(len(lp) - 1):
+        want = [(s[0] + o, s[2] + o, s[3] + o, -(s[4] + o) - 1 if o == 0 else s[4] + o) for s in seg[lp[l]:lp[l + 1]]
+                for o in range(s[1] - s[0] + 1)]
+        assert [tuple(e) for e in el[le[l]:le[l + 1]]] == want, l
