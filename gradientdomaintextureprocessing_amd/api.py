@@ -435,11 +435,12 @@ class MultigridSystem:
         return tile_shapes()
 
     TILE_SHAPE_AUTO = -3
+    TILE_SHAPE_PAIRS = -4
 
     def tile_shape(self, shape=None, level=None):
-        """Set (0..6 fused, -1 one launch per colour, TILE_SHAPE_AUTO per level by size, the
-        default) and/or return the deep-sweep tile shape setting of every level, or the shape one
-        level uses."""
+        """Set (0..6 fused, -1 one launch per colour, TILE_SHAPE_PAIRS one launch per colour pair,
+        TILE_SHAPE_AUTO per level by size, the default) and/or return the deep-sweep tile shape
+        setting of every level, or the shape one level uses."""
         cur = ctypes.c_int(0)
         arg = -2 if shape is None else int(shape)
         if level is None:
@@ -499,7 +500,8 @@ _PREFETCH = {0: "fetch per colour", 1: "one colour ahead", 2: "all colours ahead
 
 def tile_shapes():
     """The fused deep-sweep tile shapes the library offers, index -> label ("64x4 tile, 256
-    threads, one colour ahead, XCD-contiguous"), and -1 -> "per-colour" (one launch per colour)."""
+    threads, one colour ahead, XCD-contiguous"), -1 -> "per-colour" (one launch per colour) and
+    -4 -> "colour pairs" (one launch per colour pair, a workgroup per row)."""
     out, lib = {}, _lib.load()
     w, h, t, v = (ctypes.c_int() for _ in range(4))
     k = 0
@@ -507,6 +509,7 @@ def tile_shapes():
         out[k] = f"{w.value}x{h.value} tile, {t.value} threads, {_PREFETCH[v.value & 3]}" + (", XCD-contiguous" if v.value & 4 else "")
         k += 1
     out[-1] = "per-colour"
+    out[MultigridSystem.TILE_SHAPE_PAIRS] = "colour pairs"
     return out
 
 

@@ -568,7 +568,12 @@ int gdtp_residual_norms_noalloc(gdtp_system* s, const double* x, const double* b
 // fused one does not keep up: C2 level 0, 3.7 M texels per plane, 186 against 188-192 us per sweep),
 // else the fused 64x4 tiles of 384 threads, one colour ahead, XCD-contiguous (C2 levels 1-3: 46.7,
 // 17.6 and 8.3 us per sweep against 58.0, 19.7 and 14.6 us per colour).
-static int AutoTileShape(const DevMixLevel& d) { return d.plane > (1L << 21) ? -1 : 2; }
+// Measured at C2 (us per sweep; texels per dense colour plane): level 0 (6.2 M) one launch per
+// colour 187, fused 190, pairs 211; level 1 (1.6 M) pairs 42, fused shape 2 48; level 2 (0.41 M)
+// pairs 16, shape 2 18; level 3 (0.12 M) fused 8, pairs 10.
+static int AutoTileShape(const DevMixLevel& d) {
+  return d.plane > (1L << 21) ? kShapePerColour : d.plane > (1L << 17) ? kShapePairs : 2;
+}
 
 static MixedSystem& Mixed(gdtp_system* s) {
   if (!s->mixed) {
@@ -597,8 +602,14 @@ int gdtp_mixed_fuse_transfers(gdtp_system* s, int flags) {
   return GDTP_OK;
 }
 
+// A deep-sweep shape setting: a fused shape, one launch per colour, colour pairs, automatic, or
+// (report only) -2.
+static bool ValidShape(int shape) {
+  return (shape >= -2 && shape < kFusedShapes) || shape == GDTP_TILE_SHAPE_AUTO || shape == GDTP_TILE_SHAPE_PAIRS;
+}
+
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current) {
-  if (!s || shape < GDTP_TILE_SHAPE_AUTO || shape >= kFusedShapes) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-3, 7)");
+  if (!s || !ValidShape(shape)) return Fail(GDTP_ERROR_INVALID, "tile shape outside [-4, 7)");
   if (shape != -2) {
     s->tileShape = shape;
     s->levelShape.clear();
@@ -622,7 +633,7 @@ int gdtp_mixed_tile_shape_info(int shape, int* width, int* height, int* threads,
 }
 
 int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current) {
-  if (!s || !s->ds || level < 0 || level >= s->ds->levels || shape < GDTP_TILE_SHAPE_AUTO || shape >= kFusedShapes)
+  if (!s || !s->ds || level < 0 || level >= s->ds->levels || !ValidShape(shape))
     return Fail(GDTP_ERROR_INVALID, "level or tile shape out of range");
   GDTP_TRY({
     MixedSystem& m = Mixed(s);  // the automatic shape depends on the level's size

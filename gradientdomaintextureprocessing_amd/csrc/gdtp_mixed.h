@@ -35,6 +35,12 @@ constexpr FusedShapeInfo kFusedShape[kFusedShapes] = {
     {64, 4, 256, kFusedAllAhead | kFusedXcd},
 };
 
+// Deep sweeps that are not fused: one launch per colour on 64 x 4 tiles (kShapePerColour), or
+// two launches per sweep, one per colour pair, with a workgroup per sub-grid row
+// (kShapePairs; active row spans of up to kPairMaxSpan x kPairThreads sub-grid texels).
+constexpr int kShapePerColour = -1, kShapePairs = -4;
+constexpr int kPairThreads = 512, kPairMaxSpan = 4;
+
 struct DevMixLevel {
   int N = 0, B = 0, nCharts = 0;
   long plane = 0, Nd = 0;          // dense colour-plane size and total dense size (4 planes)
@@ -52,7 +58,11 @@ struct DevMixLevel {
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
   int* ftiles[kFusedShapes] = {};  // the same for each fused-sweep tile shape
   int nFTiles[kFusedShapes] = {};
-  int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, -1: one launch per colour)
+  // colour-pair sweep: 4 ints (chart, b, first texel, span end) per wave of packed rows (pairSpan 1:
+  // kPairThreads / 64 per workgroup) or per row (wider spans: one workgroup each); nRows workgroups
+  int* rows = nullptr;
+  int nRows = 0, pairSpan = 0;     // sub-grid texels per thread of the widest span (1, 2, 4; 0: too wide)
+  int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, kShapePerColour, kShapePairs)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;
   int *bRowptr = nullptr, *bCol = nullptr;

@@ -177,6 +177,126 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __
   }
 }
 
+// ------------------------------------------------------------------ colour pairs
+// The colours of a sweep in pairs, (0, 1) then (2, 3) (post: (3, 2) then (1, 0)).  The second
+// colour qb = qa ^ 1 of a pair reads the first only at its two horizontal neighbours, which lie in
+// the same sub-grid row, and neither colour reads a value the other pair writes in this launch.  A
+// workgroup owns one sub-grid row of one chart, so a pair takes one barrier and no halo: the first
+// colour's row (new values at deep texels, current ones at the other active texels) goes to LDS,
+// the second colour reads its two horizontal neighbours there and its other six from global
+// memory.  Every plane is then read twice per sweep instead of three times, and the iterate is
+// bitwise that of one launch per colour (same neighbour values, GsValue).  A row is the span
+// [a0, a1) of sub-grid columns holding an active texel of any plane (the horizontal neighbours of a
+// deep texel are active, so they lie in it).  Spans of up to kPairThreads texels are packed whole
+// into workgroups, 64 texels per wave (P = 1); wider ones take a workgroup each, thread t their
+// texels a0 + t, a0 + t + kPairThreads, ... (P of them).  Workgroups in XCD-contiguous order
+// (XcdTile), so the rows above and below share an L2.  (Measured at C2 level 0: one row per
+// workgroup, 221 us per sweep with 256 threads of two texels each and the second colour's loads
+// issued after the barrier, 210 us with 512 threads of one texel each, against 187 us for one
+// launch per colour: the rows of triangular charts leave half the waves without texels.)
+// One texel of a colour pair: its stencil, then its neighbour values (the second colour's two
+// horizontal neighbours come from LDS later) and rhs; a texel that is not deep takes its current
+// value (zero outside the chart), which the first colour puts into LDS.
+template <bool kHalf, int KC>
+struct PairTexel {
+  float c4, corr, cf[9], v[KC][9], r[KC];
+  int idx;
+  bool act;
+};
+template <bool kHalf, int KC>
+__device__ __forceinline__ void PairStencil(PairTexel<kHalf, KC>& T, const MixChart& c, long plane, long Nd, int q, int a, int a1, int b,
+                                            const float* __restrict__ st, const float2* __restrict__ stc, const uint4* __restrict__ sth) {
+  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  T.act = a < a1 && i < c.w && j < c.h;
+  T.idx = T.act ? (int)((long)q * plane + c.base2 + (long)b * c.w2 + a) : 0;
+  T.c4 = T.act ? GsStencil<kHalf>(T.idx, Nd, st, stc, sth, T.corr, T.cf) : 0.0f;
+}
+template <bool kHalf, int KC>
+__device__ __forceinline__ void PairValues(PairTexel<kHalf, KC>& T, bool second, const MixChart& c, long plane, long Nd, int q, int a,
+                                           int b, const float* e, const float* __restrict__ rhs) {
+  if (T.c4 != 0.0f) {
+    const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int dx = k % 3 - 1, dy = k / 3 - 1;
+      if (k == 4 || (second && dy == 0)) continue;
+      const int nb = DenseByte(c, plane, i + dx, j + dy);
+#pragma unroll
+      for (int ch = 0; ch < KC; ch++) T.v[ch][k] = Ld(e + (long)ch * Nd, nb);
+    }
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) T.r[ch] = Ld(rhs + (long)ch * Nd, 4 * T.idx);
+  } else if (!second) {
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) T.v[ch][4] = T.act ? Ld(e + (long)ch * Nd, 4 * T.idx) : 0.0f;
+  }
+}
+// The texel's new value (deep texels) written to e; the first colour's row value into LDS slot t;
+// the second colour's horizontal neighbours read from LDS slots lo, lo + 1 first.
+template <bool kHalf, int KC, int S>
+__device__ __forceinline__ void PairUpdate(PairTexel<kHalf, KC>& T, bool second, int q, int t, float (&sm)[KC][S], float* e, long Nd) {
+  if (second && T.c4 != 0.0f) {
+    const int lo = (q & 1) ? t : t - 1;  // span columns of i - 1 and i + 1 in the first colour's plane
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) T.v[ch][3] = sm[ch][lo], T.v[ch][5] = sm[ch][lo + 1];
+  }
+#pragma unroll
+  for (int ch = 0; ch < KC; ch++) {
+    if (T.c4 != 0.0f) {
+      const float val = GsValue<kHalf>(T.cf, T.v[ch], T.c4, T.corr, T.r[ch]);
+      e[(long)ch * Nd + T.idx] = val;
+      if (!second) sm[ch][t] = val;
+    } else if (!second) {
+      sm[ch][t] = T.v[ch][4];
+    }
+  }
+}
+
+template <bool kHalf, int KC, int P>
+__global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __restrict__ rows, int nRows, const MixChart* __restrict__ charts,
+                                                           long plane, long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+                                                           const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs) {
+  __shared__ float sm[KC][P * kPairThreads];
+  const int qb = qa ^ 1;
+  if constexpr (P == 1) {
+    // packed rows: wave w takes the 64 texels [R.z, R.z + 64) of row (chart R.x, R.y), clipped to
+    // the row's span end R.w; a row's waves are consecutive, so its texels have consecutive LDS
+    // slots.  One texel per thread and colour: both colours' loads in flight before the barrier
+    // (two memory round trips per workgroup: the stencils, then the values).
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+    const int4 R = reinterpret_cast<const int4*>(rows)[XcdTile(blockIdx.x, nRows) * (kPairThreads / 64) + w];
+    const MixChart c = charts[R.x];
+    const int b = R.y, a1 = R.w, t = threadIdx.x, a = R.z + t % 64;
+    PairTexel<kHalf, KC> A, B;
+    PairStencil(A, c, plane, Nd, qa, a, a1, b, st, stc, sth);
+    PairStencil(B, c, plane, Nd, qb, a, a1, b, st, stc, sth);
+    PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
+    PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
+    PairUpdate(A, false, qa, t, sm, e, Nd);
+    __syncthreads();
+    PairUpdate(B, true, qb, t, sm, e, Nd);
+  } else {  // one row per workgroup: span [R.z, R.w) of row (chart R.x, R.y)
+    const int4 R = reinterpret_cast<const int4*>(rows)[XcdTile(blockIdx.x, nRows)];
+    const MixChart c = charts[R.x];
+    const int b = R.y, a0 = R.z, a1 = R.w;
+    for (int p = 0; p < P; p++) {
+      const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
+      PairTexel<kHalf, KC> A;
+      PairStencil(A, c, plane, Nd, qa, a, a1, b, st, stc, sth);
+      PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
+      PairUpdate(A, false, qa, t, sm, e, Nd);
+    }
+    __syncthreads();
+    for (int p = 0; p < P; p++) {
+      const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
+      PairTexel<kHalf, KC> B;
+      PairStencil(B, c, plane, Nd, qb, a, a1, b, st, stc, sth);
+      PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
+      PairUpdate(B, true, qb, t, sm, e, Nd);
+    }
+  }
+}
+
 __device__ __forceinline__ int FloorHalf(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
 // The bilinear prolongation of the coarse column E (dense coarse planes) at fine texel
@@ -478,7 +598,9 @@ __global__ __launch_bounds__(NT) void KMixDeepGsFused(const int* __restrict__ ti
 // workgroup covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are
 // fetched once rather than once per plane.  (Measured and not kept: a thread owning the 2x2 block of
 // texels of one sub-grid position, its 4x4 window of e and all four stencils loaded before any is
-// used: one memory round trip per thread, but 138 registers and 232 instead of 221 us at C2.)
+// used: one memory round trip per thread, but 138 registers and 232 instead of 221 us at C2; the
+// tile's window of e in all four planes staged in LDS by row-contiguous loads, seven instead of 36
+// loads per column per thread: 220 us, as the HBM bytes, not the load count, bound the launch.)
 __device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
                                                  long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                  const float* __restrict__ rhs, float* res, int K, int zeroQ) {
@@ -634,6 +756,8 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
 // ------------------------------------------------------------------ transfers
 // Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c (all four
 // coarse colour planes of a tile per workgroup: they read overlapping fine neighbourhoods).
+// (Measured and not kept: the tile's (4 x 64 + 1) x (4 x 4 + 1) fine window staged in LDS one
+// column at a time by row-contiguous loads: 101 instead of 95 us at C2 level 0.)
 __device__ __forceinline__ void RestrictDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ cch,
                                                  const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
                                                  const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
@@ -781,20 +905,33 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
 }
 
 // Coarsest level: e = A^{-1} rhs with the precomputed dense inverse (FP64), rhs / e in the dense
-// FP32 planes; one workgroup per row.
+// FP32 planes; one workgroup per row, which reads the row once for up to four columns (the small
+// coarsest level's rhs stays in the L2).
 __global__ __launch_bounds__(256) void KMixDenseCoarse(int N, const double* __restrict__ Ainv, const int* __restrict__ g2d,
                                                        const float* __restrict__ rhs, float* e, long Nd, int K) {
   const int i = blockIdx.x;
   const double* row = Ainv + (long)i * N;
-  for (int ch = 0; ch < K; ch++) {
-    const float* R = rhs + (long)ch * Nd;
-    double s = 0.0;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) s += row[j] * (double)R[g2d[j]];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    __shared__ double part[4];
-    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = s;
+  __shared__ double part[4][4];
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      const double a = row[j];
+      const long gj = g2d[j];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < kc) s[q] += a * (double)rhs[(long)(c0 + q) * Nd + gj];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o, 64);
+      if (threadIdx.x % 64 == 0) part[q][threadIdx.x / 64] = s[q];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) e[(long)ch * Nd + g2d[i]] = (float)(part[0] + part[1] + part[2] + part[3]);
+    if ((int)threadIdx.x < kc) {
+      const int q = threadIdx.x;
+      e[(long)(c0 + q) * Nd + g2d[i]] = (float)(part[q][0] + part[q][1] + part[q][2] + part[q][3]);
+    }
     __syncthreads();
   }
 }
@@ -1370,6 +1507,47 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       d.nTiles = (int)tiles.size() / 3;
       d.tiles = Up(ms, tiles);
     }
+    if (l + 1 < nl && !d.chartSweeps) {  // sub-grid rows for the colour-pair sweep: their active spans
+      std::vector<int> rows;
+      int widest = 0;
+      for (int c = 0; c < d.nCharts; c++) {
+        const MixChart& m = d.hcharts[c];
+        for (int bb = 0; bb < m.h2; bb++) {
+          int lo = m.w2, hi = -1;
+          for (int q = 0; q < 4; q++)
+            for (int aa = 0; aa < m.w2; aa++) {
+              int i = 2 * aa + (q & 1), j = 2 * bb + (q >> 1);
+              if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)bb * m.w2 + aa] >= 0) lo = std::min(lo, aa), hi = std::max(hi, aa);
+            }
+          if (hi < 0) continue;
+          rows.insert(rows.end(), {c, bb, lo, hi + 1});
+          widest = std::max(widest, hi + 1 - lo);
+        }
+      }
+      d.pairSpan = 1;
+      while (d.pairSpan <= kPairMaxSpan && d.pairSpan * kPairThreads < widest) d.pairSpan *= 2;
+      if (d.pairSpan > kPairMaxSpan) d.pairSpan = 0;
+      if (d.pairSpan == 1) {  // pack whole rows into workgroups, 64 texels per wave
+        constexpr int kWaves = kPairThreads / 64;
+        std::vector<int> waves;
+        int used = 0;
+        for (size_t r = 0; r < rows.size(); r += 4) {
+          const int nw = (rows[r + 3] - rows[r + 2] + 63) / 64;
+          if (used + nw > kWaves) {
+            for (; used < kWaves; used++) waves.insert(waves.end(), {0, 0, 0, 0});
+            used = 0;
+          }
+          for (int k = 0; k < nw; k++) waves.insert(waves.end(), {rows[r], rows[r + 1], rows[r + 2] + 64 * k, rows[r + 3]});
+          used += nw;
+        }
+        for (; used > 0 && used < kWaves; used++) waves.insert(waves.end(), {0, 0, 0, 0});
+        rows.swap(waves);
+        d.nRows = (int)rows.size() / (4 * kWaves);
+      } else {
+        d.nRows = (int)rows.size() / 4;
+      }
+      d.rows = Up(ms, rows);
+    }
     if (l + 1 < nl && !d.chartSweeps)
       for (int sh = 0; sh < kFusedShapes; sh++) {
         int same = -1;  // shapes of equal tile size share one tile list
@@ -1524,12 +1702,17 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
   ms.bytes += UploadByteCounter() - before;
   const int Nc = ms.L[nl - 1].N;
   if (Nc <= kDenseCoarseMax) {  // dense inverse of the coarsest operator, by columns of the LDL^T solve
-    std::vector<double> inv((size_t)Nc * Nc), unit((size_t)Nc, 0.0), col((size_t)Nc);
-    for (int j = 0; j < Nc; j++) {
-      unit[j] = 1.0;
-      SolveLdlt(oh.coarseFactor, unit.data(), col.data(), 1, Nc);
-      unit[j] = 0.0;
-      for (int i = 0; i < Nc; i++) inv[(size_t)i * Nc + j] = col[i];
+    std::vector<double> inv((size_t)Nc * Nc);
+#pragma omp parallel
+    {
+      std::vector<double> unit((size_t)Nc, 0.0), col((size_t)Nc);
+#pragma omp for schedule(dynamic, 16)
+      for (int j = 0; j < Nc; j++) {
+        unit[j] = 1.0;
+        SolveLdlt(oh.coarseFactor, unit.data(), col.data(), 1, Nc);
+        unit[j] = 0.0;
+        for (int i = 0; i < Nc; i++) inv[(size_t)i * Nc + j] = col[i];
+      }
     }
     ms.coarseInv = Up(ms, inv);
   }
@@ -1555,6 +1738,32 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
   if (d.stencilH) KMixDeepGs<true><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
   else KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+}
+
+// One colour pair (first colour qa) of the deep sweep over the level's rows, columns in chunks of up
+// to four.
+template <bool kHalf>
+static void PairLaunch(DevMixLevel& d, int qa, int K, hipStream_t s) {
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const long off = (long)c0 * d.Nd;
+#define GDTP_PAIR(KC, P)                                                                                                             \
+  KMixPairGs<kHalf, KC, P><<<d.nRows, kPairThreads, 0, s>>>(qa, d.rows, d.nRows, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, \
+                                                            d.stencilH, d.e + off, d.rhs + off)
+#define GDTP_PAIR_SPAN(KC)                    \
+  switch (d.pairSpan) {                       \
+    case 1: GDTP_PAIR(KC, 1); break;          \
+    case 2: GDTP_PAIR(KC, 2); break;          \
+    default: GDTP_PAIR(KC, 4); break;         \
+  }
+    switch (std::min(4, K - c0)) {
+      case 4: GDTP_PAIR_SPAN(4); break;
+      case 3: GDTP_PAIR_SPAN(3); break;
+      case 2: GDTP_PAIR_SPAN(2); break;
+      default: GDTP_PAIR_SPAN(1); break;
+    }
+#undef GDTP_PAIR_SPAN
+#undef GDTP_PAIR
+  }
 }
 
 // One fused four-colour sweep (KMixDeepGsFused) over the level's tiles of shape `shape`, columns in
@@ -1601,7 +1810,8 @@ static void FusedShapeLaunch(DevMixLevel& d, int shape, int K, hipStream_t s, Fu
 static bool Fused(const DevMixLevel& d) { return d.shape >= 0 && !d.chartSweeps && d.nFTiles[d.shape] > 0; }
 
 // One deep four-colour sweep of a tiled level (colours 0..3, post: 3..0): fused when the system
-// uses a fused tile shape, else one launch per colour.  With `prolong` (post, fused only) the
+// uses a fused tile shape, one launch per colour pair with kShapePairs (where the widest row fits),
+// else one launch per colour.  With `prolong` (post, fused only) the
 // sweep first adds the prolongation of the coarse correction.  Returns the launches made.
 static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStream_t s, const FusedProlong* prolong = nullptr) {
   const int shape = d.shape;
@@ -1616,6 +1826,10 @@ static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStrea
       post ? FusedShapeLaunch<false, true, false>(d, shape, K, s, pr) : FusedShapeLaunch<false, false, false>(d, shape, K, s, pr);
     }
     return (K + 3) / 4;
+  }
+  if (d.shape == kShapePairs && d.pairSpan > 0 && d.nRows > 0) {  // pairs (0, 1), (2, 3); post (3, 2), (1, 0)
+    for (const int qa : {post ? 3 : 0, post ? 1 : 2}) d.stencilH ? PairLaunch<true>(d, qa, K, s) : PairLaunch<false>(d, qa, K, s);
+    return 2 * ((K + 3) / 4);
   }
   for (int qq = 0; qq < 4; qq++) DeepGsLaunch(d, post ? 3 - qq : qq, K, s);
   return 4;

@@ -32,7 +32,7 @@ enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS
 enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
 enum { GDTP_ASSEMBLY_DEVICE = 1 };
 enum { GDTP_FUSE_PROLONGATION = 1 };
-enum { GDTP_TILE_SHAPE_AUTO = -3 };
+enum { GDTP_TILE_SHAPE_AUTO = -3, GDTP_TILE_SHAPE_PAIRS = -4 };
 
 typedef struct gdtp_mesh gdtp_mesh;
 typedef struct gdtp_hierarchy gdtp_hierarchy;
@@ -215,8 +215,11 @@ int gdtp_mixed_cycles(gdtp_system* s, int cycles);
 /* Tile shape of the mixed solver's deep sweep on levels swept by tiles: 0..6 = one fused launch per
  * sweep (all four colours, the tile and its halo staged in LDS; the shapes' tile sizes, threads
  * and variants are listed by gdtp_mixed_tile_shape_info); -1 = one launch per colour on 64x4
- * tiles; GDTP_TILE_SHAPE_AUTO (the default) = per level, one launch per colour where a colour plane
- * holds more than 2^21 texels, else shape 2.  Every shape gives bitwise the same iterate.  Setting a
+ * tiles; GDTP_TILE_SHAPE_PAIRS = one launch per colour pair ((0, 1) then (2, 3); post-smoothing
+ * (3, 2) then (1, 0)), a workgroup per row of the colour planes (levels whose widest chart row
+ * exceeds 2048 sub-grid texels fall back to one launch per colour);
+ * GDTP_TILE_SHAPE_AUTO (the default) = per level, one launch per colour where a colour plane
+ * holds more than 2^21 texels, colour pairs above 2^17, else shape 2.  Every shape gives bitwise the same iterate.  Setting a
  * shape here replaces any per-level shapes.  shape -2 only reports; `current` (may be NULL)
  * receives the setting (GDTP_TILE_SHAPE_AUTO included).  No counterpart in the reference. */
 int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
@@ -226,7 +229,7 @@ int gdtp_mixed_tile_shape(gdtp_system* s, int shape, int* current);
  * start), 4 = XCD-contiguous tile order.  Any pointer may be NULL. */
 int gdtp_mixed_tile_shape_info(int shape, int* width, int* height, int* threads, int* variant);
 /* The same for one level only (the best shape differs between levels); shape -2 only reports;
- * `current` receives the shape the level uses (-1..6, the automatic choice resolved). */
+ * `current` receives the shape the level uses (-4, -1..6, the automatic choice resolved). */
 int gdtp_mixed_level_tile_shape(gdtp_system* s, int level, int shape, int* current);
 /* Transfers fused into the mixed solver's relaxation kernels: GDTP_FUSE_PROLONGATION lets the first
  * post-smoothing sweep of every level swept by fused tiles add the bilinear prolongation of the

@@ -1,9 +1,11 @@
 """Fused four-colour deep sweep (one launch per sweep, tile and halo staged in LDS, ping-pong
-correction buffers, optionally adding the prolongation at the start of post-smoothing) against one
-launch per colour and a separate prolongation pass: every tile shape gives bitwise the same iterate
-of the mixed solver, on screened operators (FP16 smoother stencil) and on a nearly singular one
-(FP32 smoother stencil), for column counts that fill one and two column chunks.  The FP32 fallback
-for FP16-range stencils (forced by lowering the FP16 ratio limit) solves to the direct solution."""
+correction buffers, optionally adding the prolongation at the start of post-smoothing) and the
+colour-pair sweep (one launch per colour pair, a workgroup per sub-grid row) against one launch per
+colour and a separate prolongation pass: every tile shape gives bitwise the same iterate of the
+mixed solver, on screened operators (FP16 smoother stencil) and on a nearly singular one (FP32
+smoother stencil), for column counts that fill one and two column chunks, and for rows of every
+width the pair sweep handles.  The FP32 fallback for FP16-range stencils (forced by lowering the
+FP16 ratio limit) solves to the direct solution."""
 import os
 
 import numpy as np
@@ -50,7 +52,7 @@ def test_fused_sweep_is_bitwise_four_launches(case, K):
     print(f"[fused {case} K={K}] per-colour launches", flush=True)
     want = iterate(S, b, -1, 2, False)
     assert np.all(np.isfinite(want)) and np.abs(want).max() > 0
-    for shape in range(7):
+    for shape in [*range(7), S.TILE_SHAPE_PAIRS]:
         for use_graph in (False, True):
             print(f"[fused {case} K={K}] shape {shape} graph {use_graph}", flush=True)
             got = iterate(S, b, shape, 2, use_graph)
@@ -62,6 +64,36 @@ def test_fused_sweep_is_bitwise_four_launches(case, K):
     S.tile_shape(0)
     S.mixed_deep_sweep(0, K)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("width,span", [(2048, 1), (4096, 2), (8192, 4)])
+def test_pair_sweep_wide_rows_is_bitwise_four_launches(width, span):
+    """Rows of about 500, 1000 and 2000 active sub-grid texels: each thread of the pair sweep (512
+    threads) takes 1, 2 and 4 texels of its row on level 0 (a torus in two charts side by side,
+    256 texels high)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mesh = g.TexturedMesh.torus(48, 24, 1, 2, width, 256)
+    H = g.InitializeHierarchy(mesh, width, 256, 4)
+    # widest active span of a sub-grid row (texel (i, j) of a chart lies in row j >> 1, column i >> 1)
+    ni = H.array(0, "nodeInfo").reshape(-1, 4)
+    key = ni[:, 0].astype(np.int64) * 1000000 + (ni[:, 2] >> 1)
+    col = ni[:, 1] >> 1
+    order = np.argsort(key, kind="stable")
+    k, c = key[order], col[order]
+    starts = np.flatnonzero(np.r_[True, k[1:] != k[:-1]])
+    widest = int((np.maximum.reduceat(c, starts) - np.minimum.reduceat(c, starts) + 1).max())
+    assert span == max(1, 1 << int(np.ceil(np.log2(widest / 512))))
+    S = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=3)
+    N = H.level_info(0)["texels"]
+    b = torch.from_numpy(np.random.default_rng(span).standard_normal((3, N))).cuda()
+    print(f"[pairs {width}x256 span {span}] per-colour launches", flush=True)
+    want = iterate(S, b, -1, 2, False)
+    assert np.all(np.isfinite(want)) and np.abs(want).max() > 0
+    for use_graph in (False, True):
+        print(f"[pairs {width}x256 span {span}] graph {use_graph}", flush=True)
+        got = iterate(S, b, S.TILE_SHAPE_PAIRS, 2, use_graph)
+        assert np.array_equal(got, want), f"graph {use_graph}: max diff {np.abs(got - want).max()}"
 
 
 def test_fp32_smoother_fallback_solves_to_direct():
