@@ -79,6 +79,13 @@ __device__ __forceinline__ void ForColumnChunks(int K, F&& f) {
 // its stores (few memory round trips per texel: small levels are latency bound).
 __device__ __forceinline__ float Half(unsigned bits) { return (float)__builtin_bit_cast(_Float16, (unsigned short)bits); }
 
+// The eight FP16 off-diagonal ratios of a packed stencil (slot 4 unused).
+template <bool kHalf>
+__device__ __forceinline__ void Unpack(const uint4& w, float (&cf)[9]) {
+  cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
+  cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
+}
+
 // The Gauss-Seidel value of one texel and column from its eight neighbours v (slot 4 unused):
 // c4 rhs - sum_{k!=4} c_k v_k, with the FP16 stencil's row-sum correction - corr * sum_k v_k.  The
 // fused-multiply-adds are explicit so that every kernel that calls this evaluates it identically.
@@ -149,32 +156,68 @@ __device__ __forceinline__ void DeepGsTexel(const MixChart& c, long plane, long 
   });
 }
 
+// One colour plane of a tiled level: wave w of the workgroup takes row w of its 64 x 4 tile, whose
+// deep texels it reads from a host-built bit mask (one uniform 64-bit load per wave) instead of
+// from the stencil, so that a deep texel's stencil, neighbour and rhs loads all go out together
+// (one memory round trip instead of two).  Tiles in XCD-contiguous order (XcdTile).
 template <bool kHalf>
-__global__ __launch_bounds__(256) void KMixDeepGs(int q, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                  long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+__global__ __launch_bounds__(256) void KMixDeepGs(int q, int nTiles, const int* __restrict__ tiles,
+                                                  const unsigned long long* __restrict__ deepMask, const MixChart* __restrict__ charts,
+                                                  long plane, long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
                                                   const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
-  const int* T = tiles + 3 * blockIdx.x;
+  const int tile = XcdTile(blockIdx.x, nTiles), row = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTA), tx = threadIdx.x % kTA;
+  const unsigned long long m = deepMask[((long)tile * 4 + q) * kTB + row];
+  if (!((m >> tx) & 1ull)) return;
+  const int* T = tiles + 3 * tile;
   const MixChart c = charts[T[0]];
-  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  if (a >= c.w2 || b >= c.h2) return;
-  DeepGsTexel<kHalf>(c, plane, Nd, q, a, b, st, stc, sth, e, rhs, K);
+  const int a = T[1] + tx, b = T[2] + row, i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  const int idx = (int)((long)q * plane + c.base2 + (long)b * c.w2 + a);
+  float c4, corr = 0.0f, cf[9];
+  if constexpr (kHalf) {
+    const float2 cd = stc[idx];
+    const uint4 w = sth[idx];
+    c4 = cd.x, corr = 0.125f * cd.y;
+    Unpack<true>(w, cf);
+  } else {
+    c4 = st[4 * Nd + idx];
+#pragma unroll
+    for (int k = 0; k < 9; k++) cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+  }
+  int nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
+  ForColumnChunks(K, [&](int c0, auto kc) {
+    constexpr int KC = decltype(kc)::value;
+    float v[KC][9], r[KC];
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) {
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+        if (k != 4) v[ch][k] = Ld(e + (c0 + ch) * Nd, nb[k]);
+      r[ch] = Ld(rhs + (c0 + ch) * Nd, nb[4]);
+    }
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) e[(long)(c0 + ch) * Nd + idx] = GsValue<kHalf>(cf, v[ch], c4, corr, r[ch]);
+  });
 }
 
 // The same four-colour sweep with one 1024-thread workgroup per chart, colours in order (post:
 // 3..0) separated by barriers: deep stencils never reach across charts, so this is exactly the
 // four launches of KMixDeepGs in one, for levels whose charts are small enough that launch
-// latency, not memory, bounds a sweep.  All threads reach every barrier.
+// latency, not memory, bounds a sweep.  `sweeps` consecutive sweeps (post-smoothing runs its sweeps
+// back to back) in one launch.  All threads reach every barrier.
 template <bool kHalf>
-__global__ __launch_bounds__(1024) void KMixChartGs(int post, const MixChart* __restrict__ charts, long plane, long Nd,
+__global__ __launch_bounds__(1024) void KMixChartGs(int post, int sweeps, const MixChart* __restrict__ charts, long plane, long Nd,
                                                     const float* __restrict__ st, const float2* __restrict__ stc,
                                                     const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs, int K) {
   const MixChart c = charts[blockIdx.x];
   const int aw = (c.w + 1) / 2, n = aw * c.h2;
-  for (int qq = 0; qq < 4; qq++) {
-    const int q = post ? 3 - qq : qq;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel<kHalf>(c, plane, Nd, q, t % aw, t / aw, st, stc, sth, e, rhs, K);
-    __syncthreads();
-  }
+  for (int sw = 0; sw < sweeps; sw++)
+    for (int qq = 0; qq < 4; qq++) {
+      const int q = post ? 3 - qq : qq;
+      for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel<kHalf>(c, plane, Nd, q, t % aw, t / aw, st, stc, sth, e, rhs, K);
+      __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ colour pairs
@@ -361,12 +404,6 @@ struct FusedOperands {
   typename std::conditional<kHalf, uint4, Stencil9>::type sten[kIt];
   int idx[kIt], w[kIt], state[kIt];  // state 0: outside the chart or region, 1: deep, 2: active not deep, 3: inactive
 };
-
-template <bool kHalf>
-__device__ __forceinline__ void Unpack(const uint4& w, float (&cf)[9]) {
-  cf[0] = Half(w.x & 0xffff), cf[1] = Half(w.x >> 16), cf[2] = Half(w.y & 0xffff), cf[3] = Half(w.y >> 16), cf[4] = 0.0f;
-  cf[5] = Half(w.z & 0xffff), cf[6] = Half(w.z >> 16), cf[7] = Half(w.w & 0xffff), cf[8] = Half(w.w >> 16);
-}
 
 // Thread t's texels of the region [LX, TA + HX) x [LY, TB + HY) of plane Q: t, t + NT, ...
 template <bool kHalf, bool kFirst, bool kProlong, int TA, int TB, int NT, int KC, int Q, int LX, int HX, int LY, int HY, int kIt>
@@ -713,6 +750,8 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
 // accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual
 // staged in LDS.  Cores of different blocks are disjoint.
 static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
+static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch
+template <bool kPrefetch>
 __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
                                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
                                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
@@ -724,6 +763,19 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
   const float* M = mat + matPtr[b];
   const int i = threadIdx.x % 64, part = threadIdx.x / 64;
+  // kPrefetch: this lane's entries of the block inverse (row i, columns part, part + kSchwarzWaves,
+  // ...) are loaded before the residual is staged, so the two are in flight together (fewer round
+  // trips for levels with fewer blocks than CUs, where latency bounds the step; elsewhere the
+  // registers this takes halve the resident workgroups: level 0 at C2 44 -> 61 us)
+  constexpr int kPerLane = kBlockExtMax / kSchwarzWaves;
+  static_assert(kBlockExtMax <= 64 * kSchwarzWaves, "one extended row per thread");
+  float mv[kPrefetch ? kPerLane : 1];
+  if constexpr (kPrefetch)
+#pragma unroll
+    for (int jj = 0; jj < kPerLane; jj++) {
+      const int j = part + kSchwarzWaves * jj;
+      mv[jj] = i < nc && j < nx ? M[(long)j * nc + i] : 0.0f;
+    }
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
     for (int j = threadIdx.x; j < nx; j += 64 * kSchwarzWaves)
@@ -731,13 +783,25 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
         R[q * kBlockExtMax + j] = resB ? resB[(long)(ch0 + q) * B + ext[x0 + j]] : rhs[(long)(ch0 + q) * Nd + bdIdx[ext[x0 + j]]];
     __syncthreads();
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (i < nc)
-      for (int j = part; j < nx; j += kSchwarzWaves) {
-        const float m = M[(long)j * nc + i];
+    if (i < nc) {
+      if constexpr (kPrefetch) {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
+        for (int jj = 0; jj < kPerLane; jj++) {
+          const int j = part + kSchwarzWaves * jj;
+          if (j >= nx) break;
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (q < kc) acc[q] += mv[jj] * R[q * kBlockExtMax + j];
+        }
+      } else {
+        for (int j = part; j < nx; j += kSchwarzWaves) {
+          const float m = M[(long)j * nc + i];
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
+        }
       }
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++) P[part][q][i] = acc[q];
     __syncthreads();
@@ -1506,6 +1570,21 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       const std::vector<int> tiles = buildTiles(kTA, kTB);
       d.nTiles = (int)tiles.size() / 3;
       d.tiles = Up(ms, tiles);
+      // deep texels of each tile: [tile][plane][tile row] 64-bit masks, bit = column in the tile
+      std::vector<unsigned long long> mask((size_t)d.nTiles * 4 * kTB, 0ull);
+      static_assert(kTA == 64, "one 64-bit mask per tile row");
+      for (int t = 0; t < d.nTiles; t++) {
+        const MixChart& m = d.hcharts[tiles[3 * t]];
+        const int a0 = tiles[3 * t + 1], b0 = tiles[3 * t + 2];
+        for (int q = 0; q < 4; q++)
+          for (int r = 0; r < kTB && b0 + r < m.h2; r++)
+            for (int x = 0; x < kTA && a0 + x < m.w2; x++) {
+              const int i = 2 * (a0 + x) + (q & 1), j = 2 * (b0 + r) + (q >> 1);
+              if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)(b0 + r) * m.w2 + a0 + x] == 2)
+                mask[((size_t)t * 4 + q) * kTB + r] |= 1ull << x;
+            }
+      }
+      d.deepMask = Up(ms, mask);
     }
     if (l + 1 < nl && !d.chartSweeps) {  // sub-grid rows for the colour-pair sweep: their active spans
       std::vector<int> rows;
@@ -1736,8 +1815,10 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 // One colour of the deep sweep: FP16 stencil where the level holds one (well-screened operators),
 // else FP32.
 static void DeepGsLaunch(DevMixLevel& d, int q, int K, hipStream_t s) {
-  if (d.stencilH) KMixDeepGs<true><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
-  else KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.tiles, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+  if (d.stencilH)
+    KMixDeepGs<true><<<d.nTiles, 256, 0, s>>>(q, d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+  else
+    KMixDeepGs<false><<<d.nTiles, 256, 0, s>>>(q, d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
 }
 
 // One colour pair (first colour qa) of the deep sweep over the level's rows, columns in chunks of up
@@ -1844,8 +1925,12 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
     if (!fromRhs)
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
                                                                             d.tmpB, 1, d.Nd, K);
-    KMixSchwarz<<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
-                                                         fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+    if (d.nBlocks <= kSchwarzPrefetchBlocks)
+      KMixSchwarz<true><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+                                                                 fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+    else
+      KMixSchwarz<false><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+                                                                  fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
   }
 }
 
@@ -1860,8 +1945,13 @@ static void Smooth(MixedSystem& ms, int l, int K, bool post, hipStream_t s, bool
   for (int it = 0; it < nu; it++) {
     if (!post) BoundarySweeps(ms, d, K, s, zeroE && it == 0);
     if (d.nTiles > 0 && d.chartSweeps) {
-      if (d.stencilH) KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
-      else KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+      // post-smoothing's deep sweeps follow one another with nothing between them: one launch
+      const int sweeps = post ? nu : 1;
+      if (d.stencilH)
+        KMixChartGs<true><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, sweeps, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+      else
+        KMixChartGs<false><<<d.nCharts, 1024, 0, s>>>(post ? 1 : 0, sweeps, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, d.stencilH, d.e, d.rhs, K);
+      if (post) it = nu - 1;
     } else if (d.nTiles > 0) {
       DeepSweep(ms, d, post, K, s, it == 0 ? prolong : nullptr);
     }
