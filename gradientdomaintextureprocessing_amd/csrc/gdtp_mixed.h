@@ -56,7 +56,8 @@ struct DevMixLevel {
   int nTiles = 0;
   bool chartSweeps = false;         // small charts: one workgroup per chart runs all four colours of a sweep
   int* tiles = nullptr;            // 3 ints per tile: chart, a0, b0 (sub-grid origin)
-  unsigned long long* deepMask = nullptr;  // [tile][plane][tile row]: bit x set where texel (a0 + x, b0 + row) is deep
+  unsigned long long* deepMask = nullptr;    // [tile][plane][tile row]: bit x set where texel (a0 + x, b0 + row) is deep
+  unsigned long long* activeMask = nullptr;  // the same for active texels (deep or boundary)
   int* ftiles[kFusedShapes] = {};  // the same for each fused-sweep tile shape
   int nFTiles[kFusedShapes] = {};
   // colour-pair sweep: 4 ints (chart, b, first texel, span end) per wave of packed rows (pairSpan 1:

@@ -633,28 +633,30 @@ __global__ __launch_bounds__(NT) void KMixDeepGsFused(const int* __restrict__ ti
 
 // res = rhs - A e at deep texels (raw A: the symmetric five-word stencil, a4 the diagonal).  A
 // workgroup covers its tile in all four colour planes, so the planes' shared neighbourhoods of e are
-// fetched once rather than once per plane.  (Measured and not kept: a thread owning the 2x2 block of
-// texels of one sub-grid position, its 4x4 window of e and all four stencils loaded before any is
-// used: one memory round trip per thread, but 138 registers and 232 instead of 221 us at C2; the
-// tile's window of e in all four planes staged in LDS by row-contiguous loads, seven instead of 36
-// loads per column per thread: 220 us, as the HBM bytes, not the load count, bound the launch.)
-__device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
-                                                 long Nd, const float* __restrict__ sym, const float* __restrict__ e,
-                                                 const float* __restrict__ rhs, float* res, int K, int zeroQ) {
+// fetched once rather than once per plane; its deep texels come from the tile's bit masks (one
+// uniform load per wave and plane), so a texel's stencil, neighbour and rhs loads go out together.
+// (Measured and not kept: a thread owning the 2x2 block of texels of one sub-grid position, its 4x4
+// window of e and all four stencils loaded before any is used: one memory round trip per thread,
+// but 138 registers and 232 instead of 221 us at C2; the tile's window of e in all four planes
+// staged in LDS by row-contiguous loads, seven instead of 36 loads per column per thread: 220 us,
+// as the HBM bytes, not the load count, bound the launch.)
+__device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const unsigned long long* __restrict__ deepMask,
+                                                 const MixChart* __restrict__ charts, long plane, long Nd, const float* __restrict__ sym,
+                                                 const float* __restrict__ e, const float* __restrict__ rhs, float* res, int K, int zeroQ) {
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTA), tx = threadIdx.x % kTA;
+  const unsigned long long* mk = deepMask + (long)vb * 4 * kTB + row;
   const int* T = tiles + 3 * vb;
   const MixChart c = charts[T[0]];
-  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  if (a >= c.w2 || b >= c.h2) return;
+  const int a = T[1] + tx, b = T[2] + row;
   for (int q = 0; q < 4; q++) {
+    if (!((mk[q * kTB] >> tx) & 1ull)) continue;
     const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
-    if (i >= c.w || j >= c.h) continue;
     const long idx = (long)q * plane + c.base2 + (long)b * c.w2 + a;
-    const float a4 = sym[4 * Nd + idx];
-    if (a4 == 0.0f) continue;
     if (q == zeroQ) {  // relaxed last by the FP32 smoother: its deep rows hold exactly, so the residual is zero
       for (int ch = 0; ch < K; ch++) res[(long)ch * Nd + idx] = 0.0f;
       continue;
     }
+    const float a4 = sym[4 * Nd + idx];
     int nb[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
@@ -736,12 +738,13 @@ __global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __
 
 // The residual of a level after pre-smoothing in one launch: blocks [0, nTiles) take the deep tiles,
 // the others the boundary rows (disjoint outputs, so the two parts overlap on the device).
-__global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __restrict__ tiles, const MixChart* __restrict__ charts, long plane,
+__global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __restrict__ tiles, const unsigned long long* __restrict__ deepMask,
+                                                    const MixChart* __restrict__ charts, long plane,
                                                     long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                     const float* __restrict__ rhs, float* res, int K, int zeroQ, int B,
                                                     const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
                                                     const float* __restrict__ vl, const float* __restrict__ dg) {
-  if ((int)blockIdx.x < nTiles) ResidualDeepTile(XcdTile(blockIdx.x, nTiles), tiles, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
+  if ((int)blockIdx.x < nTiles) ResidualDeepTile(XcdTile(blockIdx.x, nTiles), tiles, deepMask, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
   else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
 }
 
@@ -819,22 +822,22 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
 
 // ------------------------------------------------------------------ transfers
 // Coarse deep rhs: 3x3 full weighting of the fine residual around the fine texel 2c (all four
-// coarse colour planes of a tile per workgroup: they read overlapping fine neighbourhoods).
+// coarse colour planes of a tile per workgroup: they read overlapping fine neighbourhoods; coarse
+// deep texels from the tile's bit masks, so no load waits on the texel type).
 // (Measured and not kept: the tile's (4 x 64 + 1) x (4 x 4 + 1) fine window staged in LDS one
 // column at a time by row-contiguous loads: 101 instead of 95 us at C2 level 0.)
-__device__ __forceinline__ void RestrictDeepTile(int vb, const int* __restrict__ tiles, const MixChart* __restrict__ cch,
-                                                 const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
-                                                 const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
-                                                 int K) {
+__device__ __forceinline__ void RestrictDeepTile(int vb, const int* __restrict__ tiles, const unsigned long long* __restrict__ cDeepMask,
+                                                 const MixChart* __restrict__ cch, const MixChart* __restrict__ fch, long cplane, long cNd,
+                                                 long fplane, long fNd, const float* __restrict__ fres, float* crhs, float* ce, int K) {
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTA), tx = threadIdx.x % kTA;
+  const unsigned long long* mk = cDeepMask + (long)vb * 4 * kTB + row;
   const int* T = tiles + 3 * vb;
   const MixChart c = cch[T[0]], f = fch[T[0]];
-  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  if (a >= c.w2 || b >= c.h2) return;
+  const int a = T[1] + tx, b = T[2] + row;
   for (int q = 0; q < 4; q++) {
+    if (!((mk[q * kTB] >> tx) & 1ull)) continue;  // coarse deep texels from the tile's bit masks
     const int ci = 2 * a + (q & 1), cj = 2 * b + (q >> 1);
-    if (ci >= c.w || cj >= c.h) continue;
     const long idx = (long)q * cplane + c.base2 + (long)b * c.w2 + a;
-    if (ctype[idx] != 2) continue;
     const int fi = f.cofx + 2 * (ci - c.cofx), fj = f.cofy + 2 * (cj - c.cofy);
     int n[9];
 #pragma unroll
@@ -887,13 +890,13 @@ __device__ __forceinline__ void RestrictBoundaryRows(int vb, int B, const int* _
 
 // Restriction into a level in one launch: blocks [0, nTiles) take the coarse deep tiles, the others
 // the coarse boundary rows (disjoint outputs).
-__global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __restrict__ tiles, const MixChart* __restrict__ cch,
-                                                    const MixChart* __restrict__ fch, long cplane, long cNd, long fplane, long fNd,
-                                                    const int8_t* __restrict__ ctype, const float* __restrict__ fres, float* crhs, float* ce,
-                                                    int K, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                    const int* __restrict__ cl, const float* __restrict__ vl) {
+__global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __restrict__ tiles, const unsigned long long* __restrict__ cDeepMask,
+                                                    const MixChart* __restrict__ cch, const MixChart* __restrict__ fch, long cplane, long cNd,
+                                                    long fplane, long fNd, const float* __restrict__ fres, float* crhs, float* ce, int K, int B,
+                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
+                                                    const float* __restrict__ vl) {
   if ((int)blockIdx.x < nTiles)
-    RestrictDeepTile(XcdTile(blockIdx.x, nTiles), tiles, cch, fch, cplane, cNd, fplane, fNd, ctype, fres, crhs, ce, K);
+    RestrictDeepTile(XcdTile(blockIdx.x, nTiles), tiles, cDeepMask, cch, fch, cplane, cNd, fplane, fNd, fres, crhs, ce, K);
   else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, fres, fNd, crhs, ce, cNd, K);
 }
 
@@ -905,14 +908,16 @@ __global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __res
 // mean of two (bx, bx + 1): bx = 0 for both when ui0 is even, bx = qi when ui0 is odd.  Deep texels of
 // plane `skipQ` are left alone: the Gauss-Seidel sweep that follows overwrites them without reading
 // them (skipQ = -1 when no sweep follows).
-__global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ tiles, const MixChart* __restrict__ fch,
+__global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ tiles, const unsigned long long* __restrict__ fDeepMask,
+                                                      const unsigned long long* __restrict__ fActiveMask, const MixChart* __restrict__ fch,
                                                       const MixChart* __restrict__ cch, long fplane, long fNd, long cplane, long cNd,
-                                                      const int8_t* __restrict__ ftype, const float* __restrict__ ce, float* fe, int K,
-                                                      int skipQ) {
-  const int* T = tiles + 3 * XcdTile(blockIdx.x, gridDim.x);
+                                                      const float* __restrict__ ce, float* fe, int K, int skipQ) {
+  // active (and deep) fine texels from the tile's bit masks: no load waits on the texel type
+  const int tile = XcdTile(blockIdx.x, gridDim.x), row = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kTA), tx = threadIdx.x % kTA;
+  const long mo = (long)tile * 4 * kTB + row;
+  const int* T = tiles + 3 * tile;
   const MixChart f = fch[T[0]], c = cch[T[0]];
-  const int a = T[1] + (threadIdx.x % kTA), b = T[2] + (threadIdx.x / kTA);
-  if (a >= f.w2 || b >= f.h2) return;
+  const int a = T[1] + tx, b = T[2] + row;
   const int ui0 = 2 * a - f.cofx, uj0 = 2 * b - f.cofy;
   const int ci0 = FloorHalf(ui0) + c.cofx, cj0 = FloorHalf(uj0) + c.cofy;
   const bool ei = (ui0 & 1) == 0, ej = (uj0 & 1) == 0;
@@ -924,9 +929,8 @@ __global__ __launch_bounds__(256) void KMixProlongAdd(const int* __restrict__ ti
     const int qi = q & 1, qj = q >> 1;
     const long idx = (long)q * fplane + f.base2 + (long)b * f.w2 + a;
     fidx[q] = (int)(4 * idx);
-    const bool in = 2 * a + qi < f.w && 2 * b + qj < f.h;
-    const int t = in ? ftype[idx] : -1;
-    act[q] = t >= 0 && !(q == skipQ && t == 2);
+    const bool active = (fActiveMask[mo + q * kTB] >> tx) & 1ull, deep = (fDeepMask[mo + q * kTB] >> tx) & 1ull;
+    act[q] = active && !(q == skipQ && deep);
     const int bx = ei ? 0 : qi, by = ej ? 0 : qj;
     const bool ox = ei ? qi == 1 : qi == 0, oy = ej ? qj == 1 : qj == 0;
     if (act[q]) need |= ((1 | (ox ? 2 : 0)) * (1 | (oy ? 4 : 0))) << (bx + 2 * by);
@@ -1570,8 +1574,8 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       const std::vector<int> tiles = buildTiles(kTA, kTB);
       d.nTiles = (int)tiles.size() / 3;
       d.tiles = Up(ms, tiles);
-      // deep texels of each tile: [tile][plane][tile row] 64-bit masks, bit = column in the tile
-      std::vector<unsigned long long> mask((size_t)d.nTiles * 4 * kTB, 0ull);
+      // deep and active texels of each tile: [tile][plane][tile row] 64-bit masks, bit = column in the tile
+      std::vector<unsigned long long> deep((size_t)d.nTiles * 4 * kTB, 0ull), active(deep.size(), 0ull);
       static_assert(kTA == 64, "one 64-bit mask per tile row");
       for (int t = 0; t < d.nTiles; t++) {
         const MixChart& m = d.hcharts[tiles[3 * t]];
@@ -1580,11 +1584,13 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
           for (int r = 0; r < kTB && b0 + r < m.h2; r++)
             for (int x = 0; x < kTA && a0 + x < m.w2; x++) {
               const int i = 2 * (a0 + x) + (q & 1), j = 2 * (b0 + r) + (q >> 1);
-              if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)(b0 + r) * m.w2 + a0 + x] == 2)
-                mask[((size_t)t * 4 + q) * kTB + r] |= 1ull << x;
+              const int tp = i < m.w && j < m.h ? type[(long)q * d.plane + m.base2 + (long)(b0 + r) * m.w2 + a0 + x] : -1;
+              if (tp == 2) deep[((size_t)t * 4 + q) * kTB + r] |= 1ull << x;
+              if (tp >= 0) active[((size_t)t * 4 + q) * kTB + r] |= 1ull << x;
             }
       }
-      d.deepMask = Up(ms, mask);
+      d.deepMask = Up(ms, deep);
+      d.activeMask = Up(ms, active);
     }
     if (l + 1 < nl && !d.chartSweeps) {  // sub-grid rows for the colour-pair sweep: their active spans
       std::vector<int> rows;
@@ -1987,10 +1993,10 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
     Smooth(ms, l, K, false, s, l > 0 || zeroE0);
     const long rb = d.B > 0 ? NBlk((long)d.B * kRowLanes, 256) : 0, cb = c.B > 0 ? NBlk((long)c.B * kRowLanes, 256) : 0;
     if (d.nTiles + rb > 0)
-      KMixResidual<<<d.nTiles + rb, 256, 0, s>>>(d.nTiles, d.tiles, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
+      KMixResidual<<<d.nTiles + rb, 256, 0, s>>>(d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
                                                  ms.nuPre > 0 && !d.stencilH ? 3 : -1, d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
     if (c.nTiles + cb > 0)
-      KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, c.type, d.res, c.rhs,
+      KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.deepMask, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, d.res, c.rhs,
                                                  c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);
   }
   {
@@ -2009,8 +2015,8 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
     // the first post-smoothing sweep of a fused level adds the prolongation itself
     const bool fuse = ms.fuseProlong && ms.nuPost > 0 && Fused(d);
     if (d.nTiles > 0 && !fuse)
-      KMixProlongAdd<<<d.nTiles, 256, 0, s>>>(d.tiles, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, d.type, c.e, d.e, K,
-                                              ms.nuPost > 0 ? 3 : -1);
+      KMixProlongAdd<<<d.nTiles, 256, 0, s>>>(d.tiles, d.deepMask, d.activeMask, d.charts, c.charts, d.plane, d.Nd, c.plane, c.Nd, c.e, d.e,
+                                              K, ms.nuPost > 0 ? 3 : -1);
     FusedProlong pr;
     pr.cch = c.charts, pr.cplane = c.plane, pr.cNd = c.Nd, pr.ce = c.e;
     Smooth(ms, l, K, true, s, false, fuse ? &pr : nullptr);
