@@ -63,6 +63,7 @@ struct DevMixLevel {
   // colour-pair sweep: 4 ints (chart, b, first texel, span end) per wave of packed rows (pairSpan 1:
   // kPairThreads / 64 per workgroup) or per row (wider spans: one workgroup each); nRows workgroups
   int* rows = nullptr;
+  unsigned long long* rowMask = nullptr;  // packed rows: [wave][plane] deep texels of the wave's 64 texels (bit = lane)
   int nRows = 0, pairSpan = 0;     // sub-grid texels per thread of the widest span (1, 2, 4; 0: too wide)
   int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, kShapePerColour, kShapePairs)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal

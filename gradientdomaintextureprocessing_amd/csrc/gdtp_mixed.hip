@@ -244,20 +244,42 @@ template <bool kHalf, int KC>
 struct PairTexel {
   float c4, corr, cf[9], v[KC][9], r[KC];
   int idx;
-  bool act;
+  bool act, deep;
 };
+// deep: 1 / 0 where the caller knows whether the texel is deep (its stencil and the other loads then
+// go out together), -1 where the stencil tells (c4 != 0)
 template <bool kHalf, int KC>
 __device__ __forceinline__ void PairStencil(PairTexel<kHalf, KC>& T, const MixChart& c, long plane, long Nd, int q, int a, int a1, int b,
-                                            const float* __restrict__ st, const float2* __restrict__ stc, const uint4* __restrict__ sth) {
+                                            int deep, const float* __restrict__ st, const float2* __restrict__ stc,
+                                            const uint4* __restrict__ sth) {
   const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
   T.act = a < a1 && i < c.w && j < c.h;
   T.idx = T.act ? (int)((long)q * plane + c.base2 + (long)b * c.w2 + a) : 0;
-  T.c4 = T.act ? GsStencil<kHalf>(T.idx, Nd, st, stc, sth, T.corr, T.cf) : 0.0f;
+  T.corr = 0.0f;
+  if (deep < 0) {
+    T.c4 = T.act ? GsStencil<kHalf>(T.idx, Nd, st, stc, sth, T.corr, T.cf) : 0.0f;
+    T.deep = T.c4 != 0.0f;
+  } else {
+    T.deep = deep != 0;
+    T.c4 = 0.0f;
+    if (T.deep) {
+      if constexpr (kHalf) {
+        const float2 cd = stc[T.idx];
+        const uint4 w = sth[T.idx];
+        T.c4 = cd.x, T.corr = 0.125f * cd.y;
+        Unpack<true>(w, T.cf);
+      } else {
+        T.c4 = st[4 * Nd + T.idx];
+#pragma unroll
+        for (int k = 0; k < 9; k++) T.cf[k] = k == 4 ? 0.0f : st[k * Nd + T.idx];
+      }
+    }
+  }
 }
 template <bool kHalf, int KC>
 __device__ __forceinline__ void PairValues(PairTexel<kHalf, KC>& T, bool second, const MixChart& c, long plane, long Nd, int q, int a,
                                            int b, const float* e, const float* __restrict__ rhs) {
-  if (T.c4 != 0.0f) {
+  if (T.deep) {
     const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -278,14 +300,14 @@ __device__ __forceinline__ void PairValues(PairTexel<kHalf, KC>& T, bool second,
 // the second colour's horizontal neighbours read from LDS slots lo, lo + 1 first.
 template <bool kHalf, int KC, int S>
 __device__ __forceinline__ void PairUpdate(PairTexel<kHalf, KC>& T, bool second, int q, int t, float (&sm)[KC][S], float* e, long Nd) {
-  if (second && T.c4 != 0.0f) {
+  if (second && T.deep) {
     const int lo = (q & 1) ? t : t - 1;  // span columns of i - 1 and i + 1 in the first colour's plane
 #pragma unroll
     for (int ch = 0; ch < KC; ch++) T.v[ch][3] = sm[ch][lo], T.v[ch][5] = sm[ch][lo + 1];
   }
 #pragma unroll
   for (int ch = 0; ch < KC; ch++) {
-    if (T.c4 != 0.0f) {
+    if (T.deep) {
       const float val = GsValue<kHalf>(T.cf, T.v[ch], T.c4, T.corr, T.r[ch]);
       e[(long)ch * Nd + T.idx] = val;
       if (!second) sm[ch][t] = val;
@@ -296,8 +318,9 @@ __device__ __forceinline__ void PairUpdate(PairTexel<kHalf, KC>& T, bool second,
 }
 
 template <bool kHalf, int KC, int P>
-__global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __restrict__ rows, int nRows, const MixChart* __restrict__ charts,
-                                                           long plane, long Nd, const float* __restrict__ st, const float2* __restrict__ stc,
+__global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __restrict__ rows, const unsigned long long* __restrict__ rowMask,
+                                                           int nRows, const MixChart* __restrict__ charts, long plane, long Nd,
+                                                           const float* __restrict__ st, const float2* __restrict__ stc,
                                                            const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs) {
   __shared__ float sm[KC][P * kPairThreads];
   const int qb = qa ^ 1;
@@ -306,13 +329,15 @@ __global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __
     // the row's span end R.w; a row's waves are consecutive, so its texels have consecutive LDS
     // slots.  One texel per thread and colour: both colours' loads in flight before the barrier
     // (two memory round trips per workgroup: the stencils, then the values).
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
-    const int4 R = reinterpret_cast<const int4*>(rows)[XcdTile(blockIdx.x, nRows) * (kPairThreads / 64) + w];
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64), slot = XcdTile(blockIdx.x, nRows) * (kPairThreads / 64) + w;
+    const int4 R = reinterpret_cast<const int4*>(rows)[slot];
     const MixChart c = charts[R.x];
     const int b = R.y, a1 = R.w, t = threadIdx.x, a = R.z + t % 64;
+    // the wave's deep texels in each plane (bit = lane): known before any stencil is loaded
+    const int da = (int)((rowMask[4 * slot + qa] >> (t % 64)) & 1ull), db = (int)((rowMask[4 * slot + qb] >> (t % 64)) & 1ull);
     PairTexel<kHalf, KC> A, B;
-    PairStencil(A, c, plane, Nd, qa, a, a1, b, st, stc, sth);
-    PairStencil(B, c, plane, Nd, qb, a, a1, b, st, stc, sth);
+    PairStencil(A, c, plane, Nd, qa, a, a1, b, da, st, stc, sth);
+    PairStencil(B, c, plane, Nd, qb, a, a1, b, db, st, stc, sth);
     PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
     PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
     PairUpdate(A, false, qa, t, sm, e, Nd);
@@ -325,7 +350,7 @@ __global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __
     for (int p = 0; p < P; p++) {
       const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
       PairTexel<kHalf, KC> A;
-      PairStencil(A, c, plane, Nd, qa, a, a1, b, st, stc, sth);
+      PairStencil(A, c, plane, Nd, qa, a, a1, b, -1, st, stc, sth);
       PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
       PairUpdate(A, false, qa, t, sm, e, Nd);
     }
@@ -333,7 +358,7 @@ __global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __
     for (int p = 0; p < P; p++) {
       const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
       PairTexel<kHalf, KC> B;
-      PairStencil(B, c, plane, Nd, qb, a, a1, b, st, stc, sth);
+      PairStencil(B, c, plane, Nd, qb, a, a1, b, -1, st, stc, sth);
       PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
       PairUpdate(B, true, qb, t, sm, e, Nd);
     }
@@ -1628,6 +1653,18 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         for (; used > 0 && used < kWaves; used++) waves.insert(waves.end(), {0, 0, 0, 0});
         rows.swap(waves);
         d.nRows = (int)rows.size() / (4 * kWaves);
+        // deep texels of each wave's 64 texels, per plane (bit = lane)
+        std::vector<unsigned long long> mask(rows.size(), 0ull);
+        for (size_t wv = 0; wv < rows.size() / 4; wv++) {
+          const MixChart& m = d.hcharts[rows[4 * wv]];
+          const int bb = rows[4 * wv + 1], a0 = rows[4 * wv + 2], a1 = rows[4 * wv + 3];
+          for (int q = 0; q < 4; q++)
+            for (int x = 0; x < 64 && a0 + x < a1; x++) {
+              const int i = 2 * (a0 + x) + (q & 1), j = 2 * bb + (q >> 1);
+              if (i < m.w && j < m.h && type[(long)q * d.plane + m.base2 + (long)bb * m.w2 + a0 + x] == 2) mask[4 * wv + q] |= 1ull << x;
+            }
+        }
+        d.rowMask = Up(ms, mask);
       } else {
         d.nRows = (int)rows.size() / 4;
       }
@@ -1834,8 +1871,8 @@ static void PairLaunch(DevMixLevel& d, int qa, int K, hipStream_t s) {
   for (int c0 = 0; c0 < K; c0 += 4) {
     const long off = (long)c0 * d.Nd;
 #define GDTP_PAIR(KC, P)                                                                                                             \
-  KMixPairGs<kHalf, KC, P><<<d.nRows, kPairThreads, 0, s>>>(qa, d.rows, d.nRows, d.charts, d.plane, d.Nd, d.stencil, d.stencilC, \
-                                                            d.stencilH, d.e + off, d.rhs + off)
+  KMixPairGs<kHalf, KC, P><<<d.nRows, kPairThreads, 0, s>>>(qa, d.rows, d.rowMask, d.nRows, d.charts, d.plane, d.Nd, d.stencil,   \
+                                                            d.stencilC, d.stencilH, d.e + off, d.rhs + off)
 #define GDTP_PAIR_SPAN(KC)                    \
   switch (d.pairSpan) {                       \
     case 1: GDTP_PAIR(KC, 1); break;          \
