@@ -514,6 +514,9 @@ def main():
                     help="short request-loop leg (three meshes, 20 QPS) reported beside the solve throughput")
     ap.add_argument("--profile-kernel", default=None, choices=["mixed", "fused", "band"])
     ap.add_argument("--tile-shape", type=int, default=1, help="fused tile shape index for --profile-kernel fused")
+    ap.add_argument("--tile-autotune", default="on", choices=["on", "off"],
+                    help="time every deep-sweep tile shape per level before the timed steps (off: the library's automatic shapes, "
+                         "e.g. under a profiler, whose overhead would skew the choice)")
     ap.add_argument("--serving", action="store_true", help="request-loop benchmark (resident meshes, Poisson arrivals)")
     ap.add_argument("--workload", default=None, choices=["c3", "c4", "assembly"],
                     help="other workloads: c3 batched geodesics, c4 line integral convolution, assembly = mesh admission "
@@ -549,7 +552,7 @@ def main():
         f, b = texture_rhs(H, S, K)
         x, r = torch.zeros_like(b), torch.zeros_like(b)
         tiles = None
-        if mode == "mixed":  # pick the fused deep-sweep tile shape on this system (level 0, K columns)
+        if mode == "mixed" and args.tile_autotune == "on":  # pick each level's deep-sweep tile shape on this system (K columns)
             solve(S, x, b, args.tol, max_cycles, mode)
             tiles = {f"level {l}": {str(S.TILE_SHAPES[k]): round(v, 2) for k, v in t.items()} for l, t in S.autotune_tiles(K).items()}
             log(f"[bench] tile shapes per level (us per sweep, fastest first): {tiles}")
