@@ -37,9 +37,9 @@ constexpr FusedShapeInfo kFusedShape[kFusedShapes] = {
 
 // Deep sweeps that are not fused: one launch per colour on 64 x 4 tiles (kShapePerColour), or
 // two launches per sweep, one per colour pair, with a workgroup per sub-grid row
-// (kShapePairs; active row spans of up to kPairMaxSpan x kPairThreads sub-grid texels).
+// (kShapePairs; active row spans of up to kPairMaxSpan x kPairThreadsMax sub-grid texels).
 constexpr int kShapePerColour = -1, kShapePairs = -4;
-constexpr int kPairThreads = 512, kPairMaxSpan = 4;
+constexpr int kPairThreadsMax = 1024, kPairMaxSpan = 2;
 
 struct DevMixLevel {
   int N = 0, B = 0, nCharts = 0;
@@ -61,10 +61,11 @@ struct DevMixLevel {
   int* ftiles[kFusedShapes] = {};  // the same for each fused-sweep tile shape
   int nFTiles[kFusedShapes] = {};
   // colour-pair sweep: 4 ints (chart, b, first texel, span end) per wave of packed rows (pairSpan 1:
-  // kPairThreads / 64 per workgroup) or per row (wider spans: one workgroup each); nRows workgroups
+  // pairThreads / 64 per workgroup) or per row (wider spans: one workgroup each); nRows workgroups
   int* rows = nullptr;
   unsigned long long* rowMask = nullptr;  // packed rows: [wave][plane] deep texels of the wave's 64 texels (bit = lane)
-  int nRows = 0, pairSpan = 0;     // sub-grid texels per thread of the widest span (1, 2, 4; 0: too wide)
+  int nRows = 0, pairSpan = 0;     // sub-grid texels per thread of the widest span (1, 2; 0: too wide)
+  int pairThreads = 0;             // threads per workgroup of the pair sweep (512, kPairThreadsMax)
   int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, kShapePerColour, kShapePairs)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;

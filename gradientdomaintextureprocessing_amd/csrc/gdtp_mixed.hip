@@ -230,9 +230,10 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, int sweeps, const 
 // memory.  Every plane is then read twice per sweep instead of three times, and the iterate is
 // bitwise that of one launch per colour (same neighbour values, GsValue).  A row is the span
 // [a0, a1) of sub-grid columns holding an active texel of any plane (the horizontal neighbours of a
-// deep texel are active, so they lie in it).  Spans of up to kPairThreads texels are packed whole
-// into workgroups, 64 texels per wave (P = 1); wider ones take a workgroup each, thread t their
-// texels a0 + t, a0 + t + kPairThreads, ... (P of them).  Workgroups in XCD-contiguous order
+// deep texel are active, so they lie in it).  Where the level's widest span has at most
+// kPairThreadsMax texels, spans are packed whole into workgroups of NT = 512 or 1024 threads, 64
+// texels per wave (P = 1); wider ones take a workgroup of kPairThreadsMax threads each, thread t
+// their texels a0 + t, a0 + t + NT, ... (P of them).  Workgroups in XCD-contiguous order
 // (XcdTile), so the rows above and below share an L2.  (Measured at C2 level 0: one row per
 // workgroup, 221 us per sweep with 256 threads of two texels each and the second colour's loads
 // issued after the barrier, 210 us with 512 threads of one texel each, against 187 us for one
@@ -317,19 +318,19 @@ __device__ __forceinline__ void PairUpdate(PairTexel<kHalf, KC>& T, bool second,
   }
 }
 
-template <bool kHalf, int KC, int P>
-__global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __restrict__ rows, const unsigned long long* __restrict__ rowMask,
+template <bool kHalf, int KC, int NT, int P>
+__global__ __launch_bounds__(NT) void KMixPairGs(int qa, const int* __restrict__ rows, const unsigned long long* __restrict__ rowMask,
                                                            int nRows, const MixChart* __restrict__ charts, long plane, long Nd,
                                                            const float* __restrict__ st, const float2* __restrict__ stc,
                                                            const uint4* __restrict__ sth, float* e, const float* __restrict__ rhs) {
-  __shared__ float sm[KC][P * kPairThreads];
+  __shared__ float sm[KC][P * NT];
   const int qb = qa ^ 1;
   if constexpr (P == 1) {
     // packed rows: wave w takes the 64 texels [R.z, R.z + 64) of row (chart R.x, R.y), clipped to
     // the row's span end R.w; a row's waves are consecutive, so its texels have consecutive LDS
     // slots.  One texel per thread and colour: both colours' loads in flight before the barrier
     // (two memory round trips per workgroup: the stencils, then the values).
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64), slot = XcdTile(blockIdx.x, nRows) * (kPairThreads / 64) + w;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64), slot = XcdTile(blockIdx.x, nRows) * (NT / 64) + w;
     const int4 R = reinterpret_cast<const int4*>(rows)[slot];
     const MixChart c = charts[R.x];
     const int b = R.y, a1 = R.w, t = threadIdx.x, a = R.z + t % 64;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __
     const MixChart c = charts[R.x];
     const int b = R.y, a0 = R.z, a1 = R.w;
     for (int p = 0; p < P; p++) {
-      const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
+      const int t = (int)threadIdx.x + NT * p, a = a0 + t;
       PairTexel<kHalf, KC> A;
       PairStencil(A, c, plane, Nd, qa, a, a1, b, -1, st, stc, sth);
       PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kPairThreads) void KMixPairGs(int qa, const int* __
     }
     __syncthreads();
     for (int p = 0; p < P; p++) {
-      const int t = (int)threadIdx.x + kPairThreads * p, a = a0 + t;
+      const int t = (int)threadIdx.x + NT * p, a = a0 + t;
       PairTexel<kHalf, KC> B;
       PairStencil(B, c, plane, Nd, qb, a, a1, b, -1, st, stc, sth);
       PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
@@ -778,8 +779,9 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
 // accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual
 // staged in LDS.  Cores of different blocks are disjoint.
 static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
-static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch
-template <bool kPrefetch>
+static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch every entry
+static constexpr int kSchwarzPrefetchLarge = 0;     // entries per lane prefetched on larger levels
+template <int kPre>
 __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
                                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
                                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
@@ -791,19 +793,20 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
   const float* M = mat + matPtr[b];
   const int i = threadIdx.x % 64, part = threadIdx.x / 64;
-  // kPrefetch: this lane's entries of the block inverse (row i, columns part, part + kSchwarzWaves,
-  // ...) are loaded before the residual is staged, so the two are in flight together (fewer round
-  // trips for levels with fewer blocks than CUs, where latency bounds the step; elsewhere the
-  // registers this takes halve the resident workgroups: level 0 at C2 44 -> 61 us)
+  // The first kPre of this lane's entries of the block inverse (row i, columns part, part +
+  // kSchwarzWaves, ...) are loaded before the residual is staged, so they are in flight with it;
+  // the rest stream in the loop.  All of them (kPre = kPerLane) on levels with at most one block
+  // per CU, where latency bounds the step; none elsewhere (C2 level 0: 44 us; 61 us prefetching
+  // all, 58 us prefetching 8 at the same occupancy).
   constexpr int kPerLane = kBlockExtMax / kSchwarzWaves;
   static_assert(kBlockExtMax <= 64 * kSchwarzWaves, "one extended row per thread");
-  float mv[kPrefetch ? kPerLane : 1];
-  if constexpr (kPrefetch)
+  static_assert(kPre >= 0 && kPre <= kPerLane, "prefetch depth");
+  float mv[kPre > 0 ? kPre : 1];
 #pragma unroll
-    for (int jj = 0; jj < kPerLane; jj++) {
-      const int j = part + kSchwarzWaves * jj;
-      mv[jj] = i < nc && j < nx ? M[(long)j * nc + i] : 0.0f;
-    }
+  for (int jj = 0; jj < kPre; jj++) {
+    const int j = part + kSchwarzWaves * jj;
+    mv[jj] = i < nc && j < nx ? M[(long)j * nc + i] : 0.0f;
+  }
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
     for (int j = threadIdx.x; j < nx; j += 64 * kSchwarzWaves)
@@ -812,22 +815,19 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
     __syncthreads();
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (i < nc) {
-      if constexpr (kPrefetch) {
 #pragma unroll
-        for (int jj = 0; jj < kPerLane; jj++) {
-          const int j = part + kSchwarzWaves * jj;
-          if (j >= nx) break;
+      for (int jj = 0; jj < kPre; jj++) {
+        const int j = part + kSchwarzWaves * jj;
+        if (j >= nx) break;
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (q < kc) acc[q] += mv[jj] * R[q * kBlockExtMax + j];
-        }
-      } else {
-        for (int j = part; j < nx; j += kSchwarzWaves) {
-          const float m = M[(long)j * nc + i];
+        for (int q = 0; q < 4; q++)
+          if (q < kc) acc[q] += mv[jj] * R[q * kBlockExtMax + j];
+      }
+      for (int j = part + kSchwarzWaves * kPre; j < nx; j += kSchwarzWaves) {
+        const float m = M[(long)j * nc + i];
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
-        }
+        for (int q = 0; q < 4; q++)
+          if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
       }
     }
 #pragma unroll
@@ -1634,11 +1634,12 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
           widest = std::max(widest, hi + 1 - lo);
         }
       }
+      d.pairThreads = widest <= 512 ? 512 : kPairThreadsMax;
       d.pairSpan = 1;
-      while (d.pairSpan <= kPairMaxSpan && d.pairSpan * kPairThreads < widest) d.pairSpan *= 2;
+      while (d.pairSpan <= kPairMaxSpan && d.pairSpan * d.pairThreads < widest) d.pairSpan *= 2;
       if (d.pairSpan > kPairMaxSpan) d.pairSpan = 0;
       if (d.pairSpan == 1) {  // pack whole rows into workgroups, 64 texels per wave
-        constexpr int kWaves = kPairThreads / 64;
+        const int kWaves = d.pairThreads / 64;
         std::vector<int> waves;
         int used = 0;
         for (size_t r = 0; r < rows.size(); r += 4) {
@@ -1870,15 +1871,13 @@ template <bool kHalf>
 static void PairLaunch(DevMixLevel& d, int qa, int K, hipStream_t s) {
   for (int c0 = 0; c0 < K; c0 += 4) {
     const long off = (long)c0 * d.Nd;
-#define GDTP_PAIR(KC, P)                                                                                                             \
-  KMixPairGs<kHalf, KC, P><<<d.nRows, kPairThreads, 0, s>>>(qa, d.rows, d.rowMask, d.nRows, d.charts, d.plane, d.Nd, d.stencil,   \
-                                                            d.stencilC, d.stencilH, d.e + off, d.rhs + off)
-#define GDTP_PAIR_SPAN(KC)                    \
-  switch (d.pairSpan) {                       \
-    case 1: GDTP_PAIR(KC, 1); break;          \
-    case 2: GDTP_PAIR(KC, 2); break;          \
-    default: GDTP_PAIR(KC, 4); break;         \
-  }
+#define GDTP_PAIR(KC, NT, P)                                                                                                \
+  KMixPairGs<kHalf, KC, NT, P><<<d.nRows, NT, 0, s>>>(qa, d.rows, d.rowMask, d.nRows, d.charts, d.plane, d.Nd, d.stencil,      \
+                                                      d.stencilC, d.stencilH, d.e + off, d.rhs + off)
+#define GDTP_PAIR_SPAN(KC)                                                                \
+  if (d.pairSpan == 1 && d.pairThreads == 512) GDTP_PAIR(KC, 512, 1);                    \
+  else if (d.pairSpan == 1) GDTP_PAIR(KC, kPairThreadsMax, 1);                          \
+  else GDTP_PAIR(KC, kPairThreadsMax, kPairMaxSpan);
     switch (std::min(4, K - c0)) {
       case 4: GDTP_PAIR_SPAN(4); break;
       case 3: GDTP_PAIR_SPAN(3); break;
@@ -1969,10 +1968,10 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
                                                                             d.tmpB, 1, d.Nd, K);
     if (d.nBlocks <= kSchwarzPrefetchBlocks)
-      KMixSchwarz<true><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+      KMixSchwarz<kBlockExtMax / kSchwarzWaves><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
                                                                  fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
     else
-      KMixSchwarz<false><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+      KMixSchwarz<kSchwarzPrefetchLarge><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
                                                                   fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
   }
 }
