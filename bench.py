@@ -64,9 +64,9 @@ def texture_rhs(H, S, K, seed=1):
 
 
 def solve(S, x, b, tol, max_cycles, mode):
-    """One step: from x = 0 until every column reaches tol (reference V-cycles or mixed refinement)."""
-    x.zero_()
-    its, rel = S.solve(x, b, tol=tol, max_iters=max_cycles, mode=mode)
+    """One step: from x = 0 until every column reaches tol (reference V-cycles or mixed refinement).
+    The solver writes the zeros itself (zero_guess), and the mixed path's first residual is b."""
+    its, rel = S.solve(x, b, tol=tol, max_iters=max_cycles, mode=mode, zero_guess=True)
     return its, float(rel.max())
 
 

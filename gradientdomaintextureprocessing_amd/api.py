@@ -345,14 +345,15 @@ class MultigridSystem:
         (pu, pr), K = self._pair(potential, rhs, self._texels[0])
         _lib.check(_lib.load().gdtp_geodesic_divergence(self._h, pu, pr, K, _stream()))
 
-    def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed", between=None):
+    def solve(self, x, b, tol=1e-6, max_iters=50, mode="mixed", between=None, zero_guess=False):
         """Solve A x = b to relative residual `tol` per column; returns (iterations, rel_res).
         between: optional callable run after every unconverged iteration (the device is idle
-        then; the solve continues afterwards without restarting)."""
+        then; the solve continues afterwards without restarting).  zero_guess: start from x = 0
+        whatever x holds (the solve writes the zeros; the mixed path's first residual is b)."""
         (px, pb), K = self._pair(x, b, self._texels[0])
         rel = np.zeros(K)
         it = ctypes.c_int(0)
-        m = 1 if mode == "mixed" else 0
+        m = (1 if mode == "mixed" else 0) | (0x100 if zero_guess else 0)
         if between is None:
             _lib.check(_lib.load().gdtp_solve(self._h, px, pb, K, tol, max_iters, m, rel.ctypes.data,
                                               ctypes.byref(it), _stream()))

@@ -729,6 +729,8 @@ int gdtp_solve_yielding(gdtp_system* s, double* x, const double* b, int ncols, d
                         int* iterations, void* stream, void (*between)(void*), void* user) {
   int rc = Ready(s, ncols);
   if (rc != GDTP_OK) return rc;
+  const bool zeroGuess = (mode & GDTP_SOLVE_ZERO_GUESS) != 0;
+  mode &= ~GDTP_SOLVE_ZERO_GUESS;
   if (!x || !b || !(tol > 0) || maxIters < 0 || (mode != GDTP_SOLVE_FP64_REFERENCE && mode != GDTP_SOLVE_MIXED))
     return Fail(GDTP_ERROR_INVALID, "invalid solve arguments");
   GDTP_TRY({
@@ -739,7 +741,7 @@ int gdtp_solve_yielding(gdtp_system* s, double* x, const double* b, int ncols, d
     bool ok = false;
     if (mode == GDTP_SOLVE_MIXED) {
       MixedSystem& m = Mixed(s);
-      it = SolveMixed(ds, m, x, b, ncols, tol, maxIters, rel.data(), st, s->useGraph != 0, nullptr, between, user);
+      it = SolveMixed(ds, m, x, b, ncols, tol, maxIters, rel.data(), st, s->useGraph != 0, nullptr, between, user, zeroGuess);
       ok = true;
       for (double r : rel) ok = ok && r <= tol;
     } else {
@@ -748,6 +750,7 @@ int gdtp_solve_yielding(gdtp_system* s, double* x, const double* b, int ncols, d
         GDTP_HIP(hipHostMalloc((void**)&s->hNorms, 2 * s->kcap * sizeof(double), hipHostMallocDefault));
       }
       const int N = ds.L[0].N;
+      if (zeroGuess) GDTP_HIP(hipMemsetAsync(x, 0, (size_t)N * ncols * sizeof(double), st));  // this loop is never captured
       SquaredNorms(b, N, ncols, N, s->dNorms + s->kcap, st);
       for (;; it++) {
         ResidualLevel(ds, 0, x, b, ds.L[0].r, ncols, N, st);

@@ -144,8 +144,10 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
 // `between` (may be null) is called with `ctx` after every iteration that leaves the solve
 // unconverged, on the calling thread, with the device idle: a caller can yield the GPU there
 // without ending the solve (flexible CG keeps its search direction).
+// zeroGuess: start from x = 0 whatever x holds (x is zeroed, and the first residual is b itself).
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph, const double* weights = nullptr, void (*between)(void*) = nullptr, void* ctx = nullptr);
+               hipStream_t s, bool useGraph, const double* weights = nullptr, void (*between)(void*) = nullptr, void* ctx = nullptr,
+               bool zeroGuess = false);
 
 // Drop the captured refinement step (after the smoothing configuration changes).
 void InvalidateGraph(MixedSystem& ms);

@@ -1365,6 +1365,34 @@ __global__ __launch_bounds__(256) void K64ResidualW(int deepBlocks, int D, const
 }
 
 // x += e at every texel: one thread per texel for all columns (its dense index read once)
+// The first residual of a solve from a zero guess: r = b exactly, so no operator is applied.  x is
+// set to zero, r goes to the FP32 dense rhs of the correction (e = 0) and, for flexible CG, to r64;
+// |r|^2 = |b|^2 into both norm slots.  One thread per texel; every thread of a block reaches the
+// block sums.
+__global__ __launch_bounds__(256) void K64ZeroGuess(int N, const int* __restrict__ g2d, const double* __restrict__ b, double* x, float* rhs0,
+                                                    float* e0, long Nd, double* norms, double* bnorms, double* r64, int K) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const long di = g < N ? g2d[g] : 0;
+  for (int c0 = 0; c0 < K; c0 += 4) {
+    const int kc = min(4, K - c0);
+    double acc[4] = {0, 0, 0, 0};
+    if (g < N)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (q >= kc) break;
+        const long o = (long)(c0 + q) * N + g;
+        const double v = b[o];
+        x[o] = 0.0;
+        if (r64) r64[o] = v;
+        rhs0[(long)(c0 + q) * Nd + di] = (float)v;
+        e0[(long)(c0 + q) * Nd + di] = 0.0f;
+        acc[q] = v * v;
+      }
+    BlockAddSquares4(acc, kc, norms + (long)c0 * kSlots);  // every thread of the block
+    if (bnorms) BlockAddSquares4(acc, kc, bnorms + (long)c0 * kSlots);
+  }
+}
+
 __global__ __launch_bounds__(256) void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
@@ -2216,11 +2244,15 @@ static void HomeLevel0(MixedSystem& ms) {
 
 // residual of the level-0 FP64 system into the FP32 correction rhs, squared norms into norms[0..K)
 // bnorms (first residual of a solve): |b|^2 per column into bnorms[ch][slot] as well.
-static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, const double* x, const double* b, int K, bool weighted, hipStream_t s,
-                          double* r64 = nullptr, double* bnorms = nullptr) {
+static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, bool weighted, hipStream_t s,
+                          double* r64 = nullptr, double* bnorms = nullptr, bool zeroGuess = false) {
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
+  if (zeroGuess) {  // x = 0: r = b, whatever the operator
+    if (d.N > 0) K64ZeroGuess<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, b, x, d.rhs, d.e, d.Nd, ms.norms, bnorms, r64, K);
+    return;
+  }
   // deep rows: kDeepGroups x 256 per block; boundary rows: kRowLanes64 lanes each
   const int nt = 256, db = L0.nRaster > 0 ? (int)NBlk(L0.D, 256 * kDeepGroups) : 0;
   const int bb = L0.B > 0 ? (int)NBlk((long)L0.B * kRowLanes64, nt) : 0;
@@ -2273,7 +2305,7 @@ static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hip
 }
 
 int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, int K, double tol, int maxIter, double* relRes,
-               hipStream_t s, bool useGraph, const double* weights, void (*between)(void*), void* ctx) {
+               hipStream_t s, bool useGraph, const double* weights, void (*between)(void*), void* ctx, bool zeroGuess) {
   if (K < 1 || K > ms.kcap) throw std::runtime_error("column count outside [1, capacity]");
   if (K > 1024) throw std::runtime_error("at most 1024 columns per solve");
   // weighted columns run flexible CG: their operators differ from the one the V-cycle was built
@@ -2297,7 +2329,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
   if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
   HomeLevel0(ms);
-  OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr, bNorms);  // |b|^2 in the same pass
+  OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr, bNorms, zeroGuess);  // |b|^2 in the same pass
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
   GDTP_HIP(hipStreamSynchronize(s));
   auto slotSum = [&](const double* p) {

@@ -29,7 +29,7 @@ enum {
 
 enum { GDTP_METRIC_EMBEDDING = 0, GDTP_METRIC_UNIFORM = 1 };
 enum { GDTP_OPERATOR_SYSTEM = 0, GDTP_OPERATOR_MASS = 1, GDTP_OPERATOR_STIFFNESS = 2 };
-enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1 };
+enum { GDTP_SOLVE_FP64_REFERENCE = 0, GDTP_SOLVE_MIXED = 1, GDTP_SOLVE_ZERO_GUESS = 0x100 };
 enum { GDTP_ASSEMBLY_DEVICE = 1 };
 enum { GDTP_FUSE_PROLONGATION = 1 };
 enum { GDTP_TILE_SHAPE_AUTO = -3, GDTP_TILE_SHAPE_PAIRS = -4 };
@@ -173,8 +173,11 @@ void gdtp_modulation_destroy(gdtp_modulation* m);
  *   Gauss-Seidel on dense chart planes, restricted additive Schwarz on the boundary rows, exact FP64
  *   coarsest solve),
  *   on the system's own hierarchy; no counterpart in the reference.
- * x holds the initial guess and receives the solution; rel_res (host, ncols) and iterations
- * (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached first. */
+ * x holds the initial guess and receives the solution; with GDTP_SOLVE_ZERO_GUESS or-ed into mode
+ * the solve starts from x = 0 whatever x holds (it writes the zeros itself, and the mixed path's
+ * first residual is b, with no operator applied; no counterpart in the reference).  rel_res (host,
+ * ncols) and iterations (host) may be NULL.  Returns GDTP_ERROR_NUMERIC when max_iters is reached
+ * first. */
 int gdtp_solve(gdtp_system* s, double* x, const double* b, int ncols, double tol, int max_iters, int mode, double* rel_res,
                int* iterations, void* stream);
 /* gdtp_solve that calls between(user) on the calling thread after every iteration (mixed: outer

@@ -62,6 +62,28 @@ def test_fp64_reference_solve(case):
     assert np.all(np.linalg.norm(r, axis=1) / np.linalg.norm(b, axis=1) <= 1e-8)
 
 
+@pytest.mark.parametrize("mode,krylov", [("mixed", False), ("mixed", True), ("fp64", False)])
+def test_zero_guess_equals_solve_from_zero(case, mode, krylov):
+    """GDTP_SOLVE_ZERO_GUESS: the solve ignores what x holds and starts from x = 0.  The mixed
+    path's first residual is then b itself; b - A 0 is exactly b, so the iterate is bitwise that
+    of a solve from a zeroed x (plain refinement and flexible CG; the FP64 path zeroes x)."""
+    name, H, S, L, A = case
+    _, b = noise_texture_rhs(H, L[0], K=3, seed=9)
+    bg = torch.from_numpy(b).cuda()
+    S.configure_mixed(krylov=krylov)
+    try:
+        x0 = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+        its0, _ = S.solve(x0, bg, tol=1e-8, max_iters=40, mode=mode)
+        xg = torch.from_numpy(np.random.default_rng(1).standard_normal((3, L[0].N))).cuda()
+        its1, rel1 = S.solve(xg, bg, tol=1e-8, max_iters=40, mode=mode, zero_guess=True)
+        torch.cuda.synchronize()
+    finally:
+        S.configure_mixed()
+    print(f"[zero guess {name} {mode} krylov={krylov}] {its0} / {its1} iterations", flush=True)
+    assert its1 == its0 and np.all(rel1 <= 1e-8)
+    assert np.array_equal(xg.cpu().numpy(), x0.cpu().numpy())
+
+
 def test_graph_replays_match_eager(case):
     name, H, S, L, A = case
     _, b = noise_texture_rhs(H, L[0], K=3, seed=6)
