@@ -340,9 +340,12 @@ __global__ __launch_bounds__(NT) void KMixPairGs(int qa, const int* __restrict__
     PairStencil(A, c, plane, Nd, qa, a, a1, b, da, st, stc, sth);
     PairStencil(B, c, plane, Nd, qb, a, a1, b, db, st, stc, sth);
     PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
-    PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
+    // 1024-thread workgroups (spans over 512 texels) hold only the second colour's stencil across
+    // the barrier: its neighbour values too would take 78 registers, one such workgroup per CU
+    if constexpr (NT < kPairThreadsMax) PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
     PairUpdate(A, false, qa, t, sm, e, Nd);
     __syncthreads();
+    if constexpr (NT == kPairThreadsMax) PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
     PairUpdate(B, true, qb, t, sm, e, Nd);
   } else {  // one row per workgroup: span [R.z, R.w) of row (chart R.x, R.y)
     const int4 R = reinterpret_cast<const int4*>(rows)[XcdTile(blockIdx.x, nRows)];
