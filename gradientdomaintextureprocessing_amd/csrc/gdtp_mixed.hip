@@ -669,6 +669,10 @@ __global__ __launch_bounds__(NT) void KMixDeepGsFused(const int* __restrict__ ti
 // but 138 registers and 232 instead of 221 us at C2; the tile's window of e in all four planes
 // staged in LDS by row-contiguous loads, seven instead of 36 loads per column per thread: 220 us,
 // as the HBM bytes, not the load count, bound the launch.)
+// kChunk: columns whose loads are in flight together (2 on large levels: 50 instead of 79
+// registers, eight waves per SIMD instead of six, level 0 at C2 219 against 221 us; 4 on small
+// levels, which are latency-bound and pay for a second round trip).
+template <int kChunk>
 __device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__ tiles, const unsigned long long* __restrict__ deepMask,
                                                  const MixChart* __restrict__ charts, long plane, long Nd, const float* __restrict__ sym,
                                                  const float* __restrict__ e, const float* __restrict__ rhs, float* res, int K, int zeroQ) {
@@ -692,7 +696,7 @@ __device__ __forceinline__ void ResidualDeepTile(int vb, const int* __restrict__
     // own lower couplings and a4; upper coupling k from slot 8-k of that neighbour
     const float cf[9] = {sym[idx], sym[Nd + idx], sym[2 * Nd + idx], sym[3 * Nd + idx], a4,
                          Ld(sym + 3 * Nd, nb[5]), Ld(sym + 2 * Nd, nb[6]), Ld(sym + Nd, nb[7]), Ld(sym, nb[8])};
-    ForColumnChunks(K, [&](int c0, auto kc) {
+    ForColumnChunks<kChunk>(K, [&](int c0, auto kc) {
       constexpr int KC = decltype(kc)::value;
       float v[KC][9], r[KC];
 #pragma unroll
@@ -765,15 +769,18 @@ __global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __
   BoundaryResidualRows(blockIdx.x, B, bdIdx, rp, cl, vl, dg, e, rhs, out, compact, Nd, K);
 }
 
+static constexpr int kResidualWideTiles = 1024;  // levels with more 64 x 4 tiles load two columns at a time
 // The residual of a level after pre-smoothing in one launch: blocks [0, nTiles) take the deep tiles,
 // the others the boundary rows (disjoint outputs, so the two parts overlap on the device).
+template <int kChunk>
 __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __restrict__ tiles, const unsigned long long* __restrict__ deepMask,
                                                     const MixChart* __restrict__ charts, long plane,
                                                     long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                     const float* __restrict__ rhs, float* res, int K, int zeroQ, int B,
                                                     const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
                                                     const float* __restrict__ vl, const float* __restrict__ dg) {
-  if ((int)blockIdx.x < nTiles) ResidualDeepTile(XcdTile(blockIdx.x, nTiles), tiles, deepMask, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
+  if ((int)blockIdx.x < nTiles)
+    ResidualDeepTile<kChunk>(XcdTile(blockIdx.x, nTiles), tiles, deepMask, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
   else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
 }
 
@@ -2060,8 +2067,9 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
     Smooth(ms, l, K, false, s, l > 0 || zeroE0);
     const long rb = d.B > 0 ? NBlk((long)d.B * kRowLanes, 256) : 0, cb = c.B > 0 ? NBlk((long)c.B * kRowLanes, 256) : 0;
     if (d.nTiles + rb > 0)
-      KMixResidual<<<d.nTiles + rb, 256, 0, s>>>(d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K,
-                                                 ms.nuPre > 0 && !d.stencilH ? 3 : -1, d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
+      (d.nTiles > kResidualWideTiles ? KMixResidual<2> : KMixResidual<4>)<<<d.nTiles + rb, 256, 0, s>>>(
+          d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K, ms.nuPre > 0 && !d.stencilH ? 3 : -1,
+          d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
     if (c.nTiles + cb > 0)
       KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.deepMask, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, d.res, c.rhs,
                                                  c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);
