@@ -201,6 +201,47 @@ __global__ __launch_bounds__(256) void KMixDeepGs(int q, int nTiles, const int* 
   });
 }
 
+// DeepGsTexel for the small levels swept one chart per workgroup, which latency bounds: a texel
+// whose 3x3 neighbourhood lies inside the chart (a test on its position alone; every deep texel
+// passes it) issues its stencil, neighbour and rhs loads together and only then learns from c4
+// whether it is deep, one memory round trip instead of two.  (The other texels cannot be deep.)
+template <bool kHalf>
+__device__ __forceinline__ void ChartGsTexel(const MixChart& c, long plane, long Nd, int q, int a, int b, const float* __restrict__ st,
+                                             const float2* __restrict__ stc, const uint4* __restrict__ sth, float* e,
+                                             const float* __restrict__ rhs, int K) {
+  const int i = 2 * a + (q & 1), j = 2 * b + (q >> 1);
+  if (i < 1 || j < 1 || i + 1 >= c.w || j + 1 >= c.h) return;
+  const int idx = (int)((long)q * plane + c.base2 + (long)b * c.w2 + a);
+  float c4, corr = 0.0f, cf[9];
+  if constexpr (kHalf) {
+    const float2 cd = stc[idx];
+    const uint4 w = sth[idx];
+    c4 = cd.x, corr = 0.125f * cd.y;
+    Unpack<true>(w, cf);
+  } else {
+    c4 = st[4 * Nd + idx];
+#pragma unroll
+    for (int k = 0; k < 9; k++) cf[k] = k == 4 ? 0.0f : st[k * Nd + idx];
+  }
+  int nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = DenseByte(c, plane, i + k % 3 - 1, j + k / 3 - 1);
+  ForColumnChunks(K, [&](int c0, auto kc) {
+    constexpr int KC = decltype(kc)::value;
+    float v[KC][9], r[KC];
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) {
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+        if (k != 4) v[ch][k] = Ld(e + (c0 + ch) * Nd, nb[k]);
+      r[ch] = Ld(rhs + (c0 + ch) * Nd, nb[4]);
+    }
+    if (c4 == 0.0f) return;  // not deep: nothing to update
+#pragma unroll
+    for (int ch = 0; ch < KC; ch++) e[(long)(c0 + ch) * Nd + idx] = GsValue<kHalf>(cf, v[ch], c4, corr, r[ch]);
+  });
+}
+
 // The same four-colour sweep with one 1024-thread workgroup per chart, colours in order (post:
 // 3..0) separated by barriers: deep stencils never reach across charts, so this is exactly the
 // four launches of KMixDeepGs in one, for levels whose charts are small enough that launch
@@ -215,7 +256,7 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, int sweeps, const 
   for (int sw = 0; sw < sweeps; sw++)
     for (int qq = 0; qq < 4; qq++) {
       const int q = post ? 3 - qq : qq;
-      for (int t = threadIdx.x; t < n; t += blockDim.x) DeepGsTexel<kHalf>(c, plane, Nd, q, t % aw, t / aw, st, stc, sth, e, rhs, K);
+      for (int t = threadIdx.x; t < n; t += blockDim.x) ChartGsTexel<kHalf>(c, plane, Nd, q, t % aw, t / aw, st, stc, sth, e, rhs, K);
       __syncthreads();
     }
 }
