@@ -825,22 +825,26 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
   else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
 }
 
-// Restricted additive Schwarz over the boundary rows: one workgroup of kSchwarzWaves waves per
-// block, the waves splitting the extended rows; lane i of every wave owns core row i (< 64) and
-// accumulates sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual
-// staged in LDS.  Cores of different blocks are disjoint.
+// Restricted additive Schwarz over the boundary rows: one workgroup of W waves per block, the
+// waves splitting the extended rows; lane i of every wave owns core row i (< 64) and accumulates
+// sum_j Minv[i][j] * res[ext_j] for up to four columns at a time, the block's residual staged in
+// LDS.  Cores of different blocks are disjoint.  W = 4 on levels with at least
+// kSchwarzManyBlocks blocks, where four times as many blocks are resident (C2 level 0: 40 against
+// 45 us with 16 waves); W = 16 elsewhere, where a block's own latency bounds the step (level 3,
+// 366 blocks: 10 us with 16 waves, 19 us with 4).
 static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
 static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch every entry
-static constexpr int kSchwarzPrefetchLarge = 0;     // entries per lane prefetched on larger levels
-template <int kPre>
-__global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
+static constexpr int kSchwarzWavesLarge = 4;        // waves per block on levels with many blocks (more blocks resident)
+static constexpr int kSchwarzManyBlocks = 1024;     // from this many blocks on
+template <int W, int kPre>
+__global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
                                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
                                                                   const long* __restrict__ matPtr, const float* __restrict__ mat,
                                                                   const float* __restrict__ resB, const int* __restrict__ bdIdx,
                                                                   const float* __restrict__ rhs, int B, float* e, long Nd, int K,
                                                                   float omega) {
   __shared__ float R[4 * kBlockExtMax];
-  __shared__ float P[kSchwarzWaves][4][kBlockCore];
+  __shared__ float P[W][4][kBlockCore];
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
   const float* M = mat + matPtr[b];
   const int i = threadIdx.x % 64, part = threadIdx.x / 64;
@@ -849,18 +853,17 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
   // the rest stream in the loop.  All of them (kPre = kPerLane) on levels with at most one block
   // per CU, where latency bounds the step; none elsewhere (C2 level 0: 44 us; 61 us prefetching
   // all, 58 us prefetching 8 at the same occupancy).
-  constexpr int kPerLane = kBlockExtMax / kSchwarzWaves;
-  static_assert(kBlockExtMax <= 64 * kSchwarzWaves, "one extended row per thread");
+  constexpr int kPerLane = kBlockExtMax / W;
   static_assert(kPre >= 0 && kPre <= kPerLane, "prefetch depth");
   float mv[kPre > 0 ? kPre : 1];
 #pragma unroll
   for (int jj = 0; jj < kPre; jj++) {
-    const int j = part + kSchwarzWaves * jj;
+    const int j = part + W * jj;
     mv[jj] = i < nc && j < nx ? M[(long)j * nc + i] : 0.0f;
   }
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
-    for (int j = threadIdx.x; j < nx; j += 64 * kSchwarzWaves)
+    for (int j = threadIdx.x; j < nx; j += 64 * W)
       for (int q = 0; q < kc; q++)  // resB null: e is zero on this level, the residual is the rhs
         R[q * kBlockExtMax + j] = resB ? resB[(long)(ch0 + q) * B + ext[x0 + j]] : rhs[(long)(ch0 + q) * Nd + bdIdx[ext[x0 + j]]];
     __syncthreads();
@@ -868,13 +871,13 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
     if (i < nc) {
 #pragma unroll
       for (int jj = 0; jj < kPre; jj++) {
-        const int j = part + kSchwarzWaves * jj;
+        const int j = part + W * jj;
         if (j >= nx) break;
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (q < kc) acc[q] += mv[jj] * R[q * kBlockExtMax + j];
       }
-      for (int j = part + kSchwarzWaves * kPre; j < nx; j += kSchwarzWaves) {
+      for (int j = part + W * kPre; j < nx; j += W) {
         const float m = M[(long)j * nc + i];
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -888,7 +891,7 @@ __global__ __launch_bounds__(64 * kSchwarzWaves) void KMixSchwarz(const int* __r
       const int q = threadIdx.x / 64;
       if (q < kc && i < nc) {
         float v = 0.0f;
-        for (int p = 0; p < kSchwarzWaves; p++) v += P[p][q][i];
+        for (int p = 0; p < W; p++) v += P[p][q][i];
         e[(long)(ch0 + q) * Nd + core[c0 + i]] += omega * v;
       }
     }
@@ -2049,11 +2052,17 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
                                                                             d.tmpB, 1, d.Nd, K);
     if (d.nBlocks <= kSchwarzPrefetchBlocks)
-      KMixSchwarz<kBlockExtMax / kSchwarzWaves><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
-                                                                 fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+      KMixSchwarz<kSchwarzWaves, kBlockExtMax / kSchwarzWaves><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(
+          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
+          ms.omega);
+    else if (d.nBlocks < kSchwarzManyBlocks)
+      KMixSchwarz<kSchwarzWaves, 0><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(
+          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
+          ms.omega);
     else
-      KMixSchwarz<kSchwarzPrefetchLarge><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
-                                                                  fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+      KMixSchwarz<kSchwarzWavesLarge, 0><<<d.nBlocks, 64 * kSchwarzWavesLarge, 0, s>>>(
+          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
+          ms.omega);
   }
 }
 
