@@ -272,7 +272,7 @@ __global__ __launch_bounds__(1024) void KMixChartGs(int post, int sweeps, const 
 // bitwise that of one launch per colour (same neighbour values, GsValue).  A row is the span
 // [a0, a1) of sub-grid columns holding an active texel of any plane (the horizontal neighbours of a
 // deep texel are active, so they lie in it).  Where the level's widest span has at most
-// kPairThreadsMax texels, spans are packed whole into workgroups of NT = 512 or 1024 threads, 64
+// kPairThreadsMax texels, spans are packed whole into workgroups of NT = 256, 512 or 1024 threads, 64
 // texels per wave (P = 1); wider ones take a workgroup of kPairThreadsMax threads each, thread t
 // their texels a0 + t, a0 + t + NT, ... (P of them).  Workgroups in XCD-contiguous order
 // (XcdTile), so the rows above and below share an L2.  (Measured at C2 level 0: one row per
@@ -1718,7 +1718,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
           widest = std::max(widest, hi + 1 - lo);
         }
       }
-      d.pairThreads = widest <= 512 ? 512 : kPairThreadsMax;
+      d.pairThreads = widest <= 256 ? 256 : widest <= 512 ? 512 : kPairThreadsMax;
       d.pairSpan = 1;
       while (d.pairSpan <= kPairMaxSpan && d.pairSpan * d.pairThreads < widest) d.pairSpan *= 2;
       if (d.pairSpan > kPairMaxSpan) d.pairSpan = 0;
@@ -1959,7 +1959,8 @@ static void PairLaunch(DevMixLevel& d, int qa, int K, hipStream_t s) {
   KMixPairGs<kHalf, KC, NT, P><<<d.nRows, NT, 0, s>>>(qa, d.rows, d.rowMask, d.nRows, d.charts, d.plane, d.Nd, d.stencil,      \
                                                       d.stencilC, d.stencilH, d.e + off, d.rhs + off)
 #define GDTP_PAIR_SPAN(KC)                                                                \
-  if (d.pairSpan == 1 && d.pairThreads == 512) GDTP_PAIR(KC, 512, 1);                    \
+  if (d.pairSpan == 1 && d.pairThreads == 256) GDTP_PAIR(KC, 256, 1);                    \
+  else if (d.pairSpan == 1 && d.pairThreads == 512) GDTP_PAIR(KC, 512, 1);               \
   else if (d.pairSpan == 1) GDTP_PAIR(KC, kPairThreadsMax, 1);                          \
   else GDTP_PAIR(KC, kPairThreadsMax, kPairMaxSpan);
     switch (std::min(4, K - c0)) {
