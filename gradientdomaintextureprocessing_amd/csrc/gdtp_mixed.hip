@@ -1196,8 +1196,9 @@ __device__ __forceinline__ void SymRow(const double* __restrict__ sym, int N, in
 // together.  (C2, 12.1 M deep rows, 3 columns: 470 us per launch; one group per workgroup 463 us,
 // one column per wave 557 us, each row's side values taken from the neighbouring lanes by shuffles
 // 561 us, chunks of two columns (92 instead of 116 registers, five waves per SIMD instead of four)
-// 520 us: the launch time follows neither the atomics, the load batching, the load count nor the
-// occupancy.)
+// 520 us, each texel's row indices and dense index from a host-built int4 table instead of its
+// raster line (no dependent index loads) 475 against 467 us: the launch time follows neither the
+// atomics, the load batching, the load count, the occupancy nor the index chain.)
 static constexpr int kDeepGroups = 4;
 __device__ __forceinline__ void K64ResidualDeepRows(int vb, int D, const int* __restrict__ deepLine, const int* __restrict__ lines,
                                                     const double* __restrict__ sym, const double* __restrict__ x,
