@@ -623,12 +623,12 @@ class Geodesics:
             raise ValueError("source texel out of range")
         delta = torch.zeros(K, N, dtype=torch.float64, device="cuda")
         delta[torch.arange(K, device="cuda"), src] = 1.0
-        u = torch.zeros_like(delta)
-        self.heat.solve(u, delta, tol=tol, max_iters=max_iters, mode=_solve_mode(mode))
+        u = torch.empty_like(delta)  # both solves start from zero (zero_guess writes the zeros)
+        self.heat.solve(u, delta, tol=tol, max_iters=max_iters, mode=_solve_mode(mode), zero_guess=True)
         div = torch.empty_like(u)
         self.heat.geodesic_divergence(u, div)
-        phi = torch.zeros_like(u)
-        self.distance.solve(phi, div, tol=tol, max_iters=max_iters, mode=mode)
+        phi = torch.empty_like(u)
+        self.distance.solve(phi, div, tol=tol, max_iters=max_iters, mode=mode, zero_guess=True)
         phi -= phi[torch.arange(K, device="cuda"), src][:, None]
         return phi
 

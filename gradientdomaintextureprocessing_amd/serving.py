@@ -291,9 +291,10 @@ class SolverServer:
             self.running_lock.wait_for(lambda: self.short_pending == 0, timeout=self.yield_ms / 1e3)
 
     def _solve_yielding(self, system, x, b):
-        """Mixed solve to self.tol from x that yields to running filtering batches between its
-        iterations (flexible CG keeps its search direction across the yields)."""
-        its, _ = system.solve(x, b, tol=self.tol, max_iters=self.max_iters, mode="mixed", between=self._yield_to_filters)
+        """Mixed solve to self.tol from x = 0 (whatever x holds) that yields to running filtering
+        batches between its iterations (flexible CG keeps its search direction across the yields)."""
+        its, _ = system.solve(x, b, tol=self.tol, max_iters=self.max_iters, mode="mixed", between=self._yield_to_filters,
+                              zero_guess=True)
         return its
 
     def _run(self, m, batch, lane):
@@ -328,10 +329,8 @@ class SolverServer:
             rows = torch.arange(K, device="cuda")
             b.zero_()
             b[rows, src] = 1.0
-            x.zero_()
             its = self._solve_yielding(system, x, b)
             system.geodesic_divergence(x, t0)
-            t1.zero_()
             its2 = self._solve_yielding(m.distance, t1, t0)
             its += its2
             t1 -= t1[rows, src][:, None]
