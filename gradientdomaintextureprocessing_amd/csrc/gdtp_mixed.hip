@@ -369,8 +369,7 @@ __global__ __launch_bounds__(NT) void KMixPairGs(int qa, const int* __restrict__
   if constexpr (P == 1) {
     // packed rows: wave w takes the 64 texels [R.z, R.z + 64) of row (chart R.x, R.y), clipped to
     // the row's span end R.w; a row's waves are consecutive, so its texels have consecutive LDS
-    // slots.  One texel per thread and colour: both colours' loads in flight before the barrier
-    // (two memory round trips per workgroup: the stencils, then the values).
+    // slots.  One texel per thread and colour; both colours' stencils are loaded together.
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64), slot = XcdTile(blockIdx.x, nRows) * (NT / 64) + w;
     const int4 R = reinterpret_cast<const int4*>(rows)[slot];
     const MixChart c = charts[R.x];
@@ -381,12 +380,12 @@ __global__ __launch_bounds__(NT) void KMixPairGs(int qa, const int* __restrict__
     PairStencil(A, c, plane, Nd, qa, a, a1, b, da, st, stc, sth);
     PairStencil(B, c, plane, Nd, qb, a, a1, b, db, st, stc, sth);
     PairValues(A, false, c, plane, Nd, qa, a, b, e, rhs);
-    // 1024-thread workgroups (spans over 512 texels) hold only the second colour's stencil across
-    // the barrier: its neighbour values too would take 78 registers, one such workgroup per CU
-    if constexpr (NT < kPairThreadsMax) PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
+    // the second colour's neighbour loads go out once the first colour is done (its registers
+    // free) and overlap the barrier wait: 60 registers, eight waves per SIMD (issuing them with
+    // the first colour's took 78, six: C2 level 1 40.4 against 38.0 us per sweep)
     PairUpdate(A, false, qa, t, sm, e, Nd);
+    PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
     __syncthreads();
-    if constexpr (NT == kPairThreadsMax) PairValues(B, true, c, plane, Nd, qb, a, b, e, rhs);
     PairUpdate(B, true, qb, t, sm, e, Nd);
   } else {  // one row per workgroup: span [R.z, R.w) of row (chart R.x, R.y)
     const int4 R = reinterpret_cast<const int4*>(rows)[XcdTile(blockIdx.x, nRows)];
