@@ -117,6 +117,7 @@ struct MixedSystem {
   double* dots = nullptr;          // [3][kcap][1024]: per-block partial sums of (z,q), (r,p), (p,q)
   double *alpha = nullptr, *beta = nullptr, *pqPrev = nullptr;  // [kcap]
   int* first = nullptr;            // first step of a solve (beta = 0)
+  int* fresh = nullptr;            // x not yet written (refinement from a zero guess): the next update sets x = e
   std::vector<void*> allocations;
   size_t bytes = 0;                // device bytes held
   // captured refinement steps, one per (K, x, b, weighted) in use (least recently used evicted)

@@ -1393,7 +1393,8 @@ __global__ __launch_bounds__(256) void K64Residual(int deepBlocks, int D, const 
                                                    const int* __restrict__ rpb, const int* __restrict__ colb, const double* __restrict__ valb,
                                                    const double* __restrict__ x, const double* __restrict__ b, int N,
                                                    const int* __restrict__ g2d, float* rhs0, float* e0, long Nd, double* norms, int K,
-                                                   double* bnorms) {
+                                                   double* bnorms, int* fresh) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *fresh = 0;  // x is written from here on (read by the next update)
   if ((int)blockIdx.x < deepBlocks)
     K64ResidualDeepRows(XcdTile(blockIdx.x, deepBlocks), D, deepLine, lines, sym, x, b, N, g2d, rhs0, e0, Nd, norms, K, bnorms);
   else
@@ -1420,13 +1421,13 @@ __global__ __launch_bounds__(256) void K64ResidualW(int deepBlocks, int D, const
                              svalb, x, b, N, g2d, rhs0, e0, Nd, norms, K, w, r64, bnorms);
 }
 
-// x += e at every texel: one thread per texel for all columns (its dense index read once)
-// The first residual of a solve from a zero guess: r = b exactly, so no operator is applied.  x is
-// set to zero, r goes to the FP32 dense rhs of the correction (e = 0) and, for flexible CG, to r64;
-// |r|^2 = |b|^2 into both norm slots.  One thread per texel; every thread of a block reaches the
-// block sums.
-__global__ __launch_bounds__(256) void K64ZeroGuess(int N, const int* __restrict__ g2d, const double* __restrict__ b, double* x, float* rhs0,
-                                                    float* e0, long Nd, double* norms, double* bnorms, double* r64, int K) {
+// The first residual of a solve from a zero guess: r = b exactly, so no operator is applied.  r goes
+// to the FP32 dense rhs of the correction (e = 0) and, for flexible CG, to r64; |r|^2 = |b|^2 into
+// both norm slots.  x is set to zero when writeX is set (flexible CG); refinement leaves x unwritten
+// and its first update writes x = 0 + e instead of reading x.  One thread per texel; every thread of
+// a block reaches the block sums.
+__global__ __launch_bounds__(256) void K64ZeroGuess(int N, const int* __restrict__ g2d, const double* __restrict__ b, double* x, int writeX,
+                                                    float* rhs0, float* e0, long Nd, double* norms, double* bnorms, double* r64, int K) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const long di = g < N ? g2d[g] : 0;
   for (int c0 = 0; c0 < K; c0 += 4) {
@@ -1438,7 +1439,7 @@ __global__ __launch_bounds__(256) void K64ZeroGuess(int N, const int* __restrict
         if (q >= kc) break;
         const long o = (long)(c0 + q) * N + g;
         const double v = b[o];
-        x[o] = 0.0;
+        if (writeX) x[o] = 0.0;
         if (r64) r64[o] = v;
         rhs0[(long)(c0 + q) * Nd + di] = (float)v;
         e0[(long)(c0 + q) * Nd + di] = 0.0f;
@@ -1449,10 +1450,19 @@ __global__ __launch_bounds__(256) void K64ZeroGuess(int N, const int* __restrict
   }
 }
 
-__global__ __launch_bounds__(256) void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K) {
+// x += e at every texel: one thread per texel for all columns (its dense index read once).  While
+// *fresh is set (the first update of a solve from a zero guess, x not yet written) x = 0 + e, the
+// same bits as x += e on a zeroed x, without reading x; the residual launch after it clears *fresh.
+__global__ __launch_bounds__(256) void K64Update(int N, const int* __restrict__ g2d, const float* __restrict__ e0, long Nd, double* x, int K,
+                                                 const int* __restrict__ fresh) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
   const long di = g2d[g];
+  if (*fresh) {
+#pragma unroll 4
+    for (int ch = 0; ch < K; ch++) x[(long)ch * N + g] = 0.0 + (double)e0[(long)ch * Nd + di];
+    return;
+  }
 #pragma unroll 4
   for (int ch = 0; ch < K; ch++) x[(long)ch * N + g] += (double)e0[(long)ch * Nd + di];
 }
@@ -1934,6 +1944,7 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     ms.dots = Zeros<double>(ms, 3 * (size_t)kcap * kDotBlocks);
     ms.alpha = Zeros<double>(ms, kcap), ms.beta = Zeros<double>(ms, kcap), ms.pqPrev = Zeros<double>(ms, kcap);
     ms.first = Zeros<int>(ms, 1);
+    ms.fresh = Zeros<int>(ms, 1);
   }
   GDTP_HIP(hipHostMalloc((void**)&ms.hostNorms, 2 * (size_t)kcap * kSlots * sizeof(double), hipHostMallocDefault));
   GDTP_HIP(hipStreamCreateWithFlags(&ms.stream, hipStreamNonBlocking));
@@ -2313,8 +2324,8 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, double* x, const do
   DevLevel& L0 = ds.L[0];
   DevMixLevel& d = ms.L[0];
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, ms.norms);
-  if (zeroGuess) {  // x = 0: r = b, whatever the operator
-    if (d.N > 0) K64ZeroGuess<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, b, x, d.rhs, d.e, d.Nd, ms.norms, bnorms, r64, K);
+  if (zeroGuess) {  // x = 0: r = b, whatever the operator; refinement's first update writes x
+    if (d.N > 0) K64ZeroGuess<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, b, x, r64 != nullptr, d.rhs, d.e, d.Nd, ms.norms, bnorms, r64, K);
     return;
   }
   // deep rows: kDeepGroups x 256 per block; boundary rows: kRowLanes64 lanes each
@@ -2329,7 +2340,7 @@ static void OuterResidual(DeviceSystem& ds, MixedSystem& ms, double* x, const do
     return;
   }
   K64Residual<<<db + bb, nt, 0, s>>>(db, L0.D, ms.deepLine, L0.raster, L0.coefSym, L0.B, L0.bgi, L0.bd.rowptr, L0.bd.col, L0.bd.val,
-                                      L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, bnorms);
+                                      L0.bb.rowptr, L0.bb.col, L0.bb.val, x, b, L0.N, d.g2d, d.rhs, d.e, d.Nd, ms.norms, K, bnorms, ms.fresh);
 }
 
 // One flexible-CG step after the V-cycle left z = V(r) in the level-0 FP32 correction:
@@ -2364,7 +2375,7 @@ static void FcgStep(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStre
 
 static void OuterUpdate(DeviceSystem& ds, MixedSystem& ms, double* x, int K, hipStream_t s) {
   DevMixLevel& d = ms.L[0];
-  K64Update<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, d.e, d.Nd, x, K);
+  K64Update<<<NBlk(d.N, 256), 256, 0, s>>>(d.N, d.g2d, d.e, d.Nd, x, K, ms.fresh);
   (void)ds;
 }
 
@@ -2392,6 +2403,9 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
   double* bNorms = ms.norms + (size_t)ms.kcap * kSlots;
   KMixZeroD<<<NBlk((long)K * kSlots, 256), 256, 0, s>>>(K * kSlots, bNorms);
   if (krylov) GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.first, 1, 1, s));
+  // refinement from a zero guess leaves x unwritten until its first update (x = 0 + e)
+  const bool unwritten = zeroGuess && !krylov;
+  GDTP_HIP(hipMemsetD32Async((hipDeviceptr_t)ms.fresh, unwritten ? 1 : 0, 1, s));
   HomeLevel0(ms);
   OuterResidual(ds, ms, x, b, K, weighted, s, krylov ? ms.r64 : nullptr, bNorms, zeroGuess);  // |b|^2 in the same pass
   GDTP_HIP(hipMemcpyAsync(ms.hostNorms, ms.norms, 2 * (size_t)ms.kcap * kSlots * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2411,7 +2425,13 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     }
     return ok;
   };
-  if (converged()) return 0;
+  if (converged() || maxIter <= 0) {
+    if (unwritten) {  // no update will run: x = 0 as asked
+      GDTP_HIP(hipMemsetAsync(x, 0, (size_t)K * ms.L[0].N * sizeof(double), s));
+      GDTP_HIP(hipStreamSynchronize(s));
+    }
+    return 0;
+  }
   // automatic: two V-cycles per residual for refinement on operators that FP32 represents well;
   // one for flexible CG (a product of two V-cycles preconditions it worse once the columns'
   // weights differ from the V-cycle's: 11 iterations of 2 cycles against 11 of 1 at a weight 3x off)

@@ -84,6 +84,44 @@ def test_zero_guess_equals_solve_from_zero(case, mode, krylov):
     assert np.array_equal(xg.cpu().numpy(), x0.cpu().numpy())
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_zero_guess_first_update_writes_x(case, use_graph):
+    """Refinement from a zero guess leaves x unwritten until its first update, which writes
+    x = 0 + e; the flag that selects it must hold for exactly that update.  Zero-guess solves in a
+    row, a plain solve after them (the flag cleared) and solves that end before any update (b = 0,
+    max_iters = 0: x must still come back zero) all match a solve from a zeroed x."""
+    from gradientdomaintextureprocessing_amd._lib import GdtpError
+
+    name, H, S, L, A = case
+    _, b = noise_texture_rhs(H, L[0], K=3, seed=10)
+    bg = torch.from_numpy(b).cuda()
+    rng = np.random.default_rng(2)
+    noise = lambda: torch.from_numpy(rng.standard_normal((3, L[0].N))).cuda()  # noqa: E731
+    S.configure_mixed(use_graph=use_graph)
+    try:
+        x0 = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+        its0, _ = S.solve(x0, bg, tol=1e-8, max_iters=40)
+        want = x0.cpu().numpy()
+        for _ in range(2):
+            xg = noise()
+            its, _ = S.solve(xg, bg, tol=1e-8, max_iters=40, zero_guess=True)
+            assert its == its0 and np.array_equal(xg.cpu().numpy(), want)
+        xp = torch.zeros_like(x0)
+        its, _ = S.solve(xp, bg, tol=1e-8, max_iters=40)
+        assert its == its0 and np.array_equal(xp.cpu().numpy(), want)
+        xz = noise()
+        its, rel = S.solve(xz, torch.zeros_like(bg), tol=1e-8, max_iters=40, zero_guess=True)
+        assert its == 0 and not xz.cpu().numpy().any()
+        xm = noise()
+        with pytest.raises(GdtpError):
+            S.solve(xm, bg, tol=1e-8, max_iters=0, zero_guess=True)
+        assert not xm.cpu().numpy().any()
+        torch.cuda.synchronize()
+    finally:
+        S.configure_mixed()
+    print(f"[zero guess first update {name} graph={use_graph}] {its0} iterations", flush=True)
+
+
 def test_graph_replays_match_eager(case):
     name, H, S, L, A = case
     _, b = noise_texture_rhs(H, L[0], K=3, seed=6)
