@@ -74,11 +74,15 @@ struct DevMixLevel {
   // overlapping Schwarz blocks of boundary rows: block b updates its core rows (dense indices
   // blkCore[blkCorePtr[b]..]) from the residual at its extended rows (boundary indices
   // blkExt[blkExtPtr[b]..]) through the core rows of the inverse of A restricted to the extended
-  // rows, stored FP32 column-major at blkMat + blkMatPtr[b] ([ext][core])
+  // rows, stored column-major at blkMat + blkMatPtr[b] ([ext][core]) in FP32, or at blkMatH +
+  // blkMatPtrH[b] as FP16 pairs ([ext / 2][core], extended rows 2s | 2s + 1 << 16 in one word)
+  // scaled per core row (entries |.| <= 1, core row r's scale blkScale[r])
   int nBlocks = 0;
   int *blkCorePtr = nullptr, *blkCore = nullptr, *blkExtPtr = nullptr, *blkExt = nullptr;
-  long* blkMatPtr = nullptr;
+  long *blkMatPtr = nullptr, *blkMatPtrH = nullptr;
   float* blkMat = nullptr;
+  uint32_t* blkMatH = nullptr;
+  float* blkScale = nullptr;
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
   int *rRowptr = nullptr, *rCol = nullptr;
   float* rVal = nullptr;
@@ -103,6 +107,9 @@ struct MixedSystem {
   double screen = 1.0, stiff = 0.0;  // the system's own weights
   // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
   bool krylov = false;
+  // the Schwarz steps read a level's FP16 block-inverse pairs where it has them (set per solve:
+  // plain refinement; flexible CG reads the FP32 inverses)
+  bool schwarzHalf = true;
   // V-cycles per outer iteration (the level-0 correction carries over between them); 0 = automatic:
   // 2 for plain refinement unless the operator is nearly singular for FP32 (see rowSumRatio), else 1
   int cycles = 0;

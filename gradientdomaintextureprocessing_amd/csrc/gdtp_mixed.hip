@@ -831,56 +831,83 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
 // kSchwarzManyBlocks blocks, where four times as many blocks are resident (C2 level 0: 40 against
 // 45 us with 16 waves); W = 16 elsewhere, where a block's own latency bounds the step (level 3,
 // 366 blocks: 10 us with 16 waves, 19 us with 4).
+// TM: the inverse's entries as FP32 (float, one extended row per load) or as FP16 pairs (uint32_t:
+// extended rows 2s and 2s + 1 of a core row in one word, so a lane's load brings two entries),
+// scaled per core row to |entry| <= 1 with the row's FP32 scale in rowScale.  The step is a
+// smoother, so the FP16 rounding (2^-11 of the row's largest entry) perturbs its correction, not
+// the residuals that drive the refinement.  Levels that smooth with FP16 stencil ratios (screened
+// operators) and more than one block per CU hold FP16 pairs beside the FP32 inverses; plain
+// refinement reads the pairs, flexible CG the FP32 inverses (with the pairs, batched and single
+// geodesic distances at 2048^2 drifted apart by 0.7 % of the largest distance at rel-res 1e-6,
+// against under 0.1 % with FP32).  kPre counts loads (entries or pairs).
 static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
 static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch every entry
 static constexpr int kSchwarzWavesLarge = 4;        // waves per block on levels with many blocks (more blocks resident)
 static constexpr int kSchwarzManyBlocks = 1024;     // from this many blocks on
-template <int W, int kPre>
+__device__ __forceinline__ void Entries(const float* M, long k, float (&m)[1]) { m[0] = M[k]; }
+__device__ __forceinline__ void Entries(const uint32_t* M, long k, float (&m)[2]) {
+  const unsigned w = M[k];
+  m[0] = Half(w & 0xffff), m[1] = Half(w >> 16);
+}
+template <int W, int kPre, class TM>
 __global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
                                                                   const int* __restrict__ extPtr, const int* __restrict__ ext,
-                                                                  const long* __restrict__ matPtr, const float* __restrict__ mat,
+                                                                  const long* __restrict__ matPtr, const TM* __restrict__ mat,
+                                                                  const float* __restrict__ rowScale,
                                                                   const float* __restrict__ resB, const int* __restrict__ bdIdx,
                                                                   const float* __restrict__ rhs, int B, float* e, long Nd, int K,
                                                                   float omega) {
+  constexpr bool kScaled = std::is_same<TM, uint32_t>::value;
+  constexpr int kJ = kScaled ? 2 : 1;  // extended rows per load
   __shared__ float R[4 * kBlockExtMax];
   __shared__ float P[W][4][kBlockCore];
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
-  const float* M = mat + matPtr[b];
+  const TM* M = mat + matPtr[b];
   const int i = threadIdx.x % 64, part = threadIdx.x / 64;
-  // The first kPre of this lane's entries of the block inverse (row i, columns part, part +
-  // kSchwarzWaves, ...) are loaded before the residual is staged, so they are in flight with it;
+  // The first kPre of this lane's loads of the block inverse (row i, load slots part, part + W,
+  // ..., each kJ extended rows) are issued before the residual is staged, so they are in flight with it;
   // the rest stream in the loop.  All of them (kPre = kPerLane) on levels with at most one block
   // per CU, where latency bounds the step; none elsewhere (C2 level 0: 44 us; 61 us prefetching
   // all, 58 us prefetching 8 at the same occupancy).
-  constexpr int kPerLane = kBlockExtMax / W;
+  constexpr int kPerLane = kBlockExtMax / (kJ * W);
   static_assert(kPre >= 0 && kPre <= kPerLane, "prefetch depth");
-  float mv[kPre > 0 ? kPre : 1];
+  float mv[kPre > 0 ? kPre : 1][kJ];
 #pragma unroll
-  for (int jj = 0; jj < kPre; jj++) {
-    const int j = part + W * jj;
-    mv[jj] = i < nc && j < nx ? M[(long)j * nc + i] : 0.0f;
+  for (int ss = 0; ss < kPre; ss++) {
+    const int sl = part + W * ss;
+    if (i < nc && kJ * sl < nx) Entries(M, (long)sl * nc + i, mv[ss]);
+    else
+#pragma unroll
+      for (int t = 0; t < kJ; t++) mv[ss][t] = 0.0f;
   }
+  const int nxPad = (nx + kJ - 1) / kJ * kJ;  // a pair's second row past nx reads a zero residual
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
-    for (int j = threadIdx.x; j < nx; j += 64 * W)
+    for (int j = threadIdx.x; j < nxPad; j += 64 * W)
       for (int q = 0; q < kc; q++)  // resB null: e is zero on this level, the residual is the rhs
-        R[q * kBlockExtMax + j] = resB ? resB[(long)(ch0 + q) * B + ext[x0 + j]] : rhs[(long)(ch0 + q) * Nd + bdIdx[ext[x0 + j]]];
+        R[q * kBlockExtMax + j] = j >= nx ? 0.0f : resB ? resB[(long)(ch0 + q) * B + ext[x0 + j]] : rhs[(long)(ch0 + q) * Nd + bdIdx[ext[x0 + j]]];
     __syncthreads();
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (i < nc) {
 #pragma unroll
-      for (int jj = 0; jj < kPre; jj++) {
-        const int j = part + W * jj;
+      for (int ss = 0; ss < kPre; ss++) {
+        const int j = kJ * (part + W * ss);
         if (j >= nx) break;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (q < kc) acc[q] += mv[jj] * R[q * kBlockExtMax + j];
-      }
-      for (int j = part + W * kPre; j < nx; j += W) {
-        const float m = M[(long)j * nc + i];
+        for (int t = 0; t < kJ; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (q < kc) acc[q] += m * R[q * kBlockExtMax + j];
+          for (int q = 0; q < 4; q++)
+            if (q < kc) acc[q] += mv[ss][t] * R[q * kBlockExtMax + j + t];
+      }
+      for (int sl = part + W * kPre; kJ * sl < nx; sl += W) {
+        float m[kJ];
+        Entries(M, (long)sl * nc + i, m);
+        const int j = kJ * sl;
+#pragma unroll
+        for (int t = 0; t < kJ; t++)
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (q < kc) acc[q] += m[t] * R[q * kBlockExtMax + j + t];
       }
     }
 #pragma unroll
@@ -891,6 +918,7 @@ __global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ co
       if (q < kc && i < nc) {
         float v = 0.0f;
         for (int p = 0; p < W; p++) v += P[p][q][i];
+        if constexpr (kScaled) v *= rowScale[c0 + i];
         e[(long)(ch0 + q) * Nd + core[c0 + i]] += omega * v;
       }
     }
@@ -1878,7 +1906,37 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       BuildSchwarzBlocks(bb, bidx, blk);
       d.nBlocks = (int)blk.corePtr.size() - 1;
       d.blkCorePtr = Up(ms, blk.corePtr), d.blkCore = Up(ms, blk.core), d.blkExtPtr = Up(ms, blk.extPtr), d.blkExt = Up(ms, blk.ext);
-      d.blkMatPtr = Up(ms, blk.matPtr), d.blkMat = Up(ms, blk.mat);
+      d.blkMatPtr = Up(ms, blk.matPtr);
+      // FP16 pairs of extended rows, each core row scaled by its largest |entry|, where the step
+      // streams its matrices (C2: level 0 39.2 -> 31.9 us, level 1 26.3 -> 20.6 us); levels with at
+      // most one block per CU prefetch every entry and are latency-bound, and keep FP32 (the pairs'
+      // conversions made them 0.5-0.8 us slower).  GDTP_SCHWARZ_HALF_BLOCKS lowers the block count
+      // from which levels take the pairs (tests use it to run small systems through them).
+      int halfBlocks = kSchwarzPrefetchBlocks + 1;
+      if (const char* env = std::getenv("GDTP_SCHWARZ_HALF_BLOCKS")) halfBlocks = std::atoi(env);
+      d.blkMat = Up(ms, blk.mat);
+      if (d.stencilH && d.nBlocks >= halfBlocks) {
+        std::vector<long> hp(1, 0);
+        for (int b = 0; b < d.nBlocks; b++)
+          hp.push_back(hp.back() + (long)((blk.extPtr[b + 1] - blk.extPtr[b] + 1) / 2) * (blk.corePtr[b + 1] - blk.corePtr[b]));
+        std::vector<uint32_t> mh(hp.back());
+        std::vector<float> sc(blk.core.size(), 1.0f);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b = 0; b < d.nBlocks; b++) {
+          const int c0 = blk.corePtr[b], nc = blk.corePtr[b + 1] - c0, nx = blk.extPtr[b + 1] - blk.extPtr[b];
+          const float* M = blk.mat.data() + blk.matPtr[b];
+          uint32_t* H = mh.data() + hp[b];
+          for (int i = 0; i < nc; i++) {
+            float mx = 0.0f;
+            for (int j = 0; j < nx; j++) mx = std::max(mx, std::fabs(M[(size_t)j * nc + i]));
+            const float sr = mx > 0.0f ? mx : 1.0f;
+            sc[c0 + i] = sr;
+            auto h = [&](int j) { return j < nx ? (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(M[(size_t)j * nc + i] / sr)) : 0u; };
+            for (int sl = 0; 2 * sl < nx; sl++) H[(size_t)sl * nc + i] = h(2 * sl) | h(2 * sl + 1) << 16;
+          }
+        }
+        d.blkMatPtrH = Up(ms, hp), d.blkMatH = Up(ms, mh), d.blkScale = Up(ms, sc);
+      }
     } else {
       std::vector<int> bidx(d.B);
       for (int r = 0; r < d.B; r++) bidx[r] = g2d[H.boundaryGlobalIndex[r]];
@@ -2054,6 +2112,17 @@ static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStrea
   return 4;
 }
 
+// res: the boundary rows' residual, or null when e is zero on the level (the residual is the rhs)
+template <int W, int kPre>
+static void SchwarzLaunch(MixedSystem& ms, DevMixLevel& d, const float* res, int K, hipStream_t s) {
+  if (d.blkMatH && ms.schwarzHalf)
+    KMixSchwarz<W, kPre / 2, uint32_t><<<d.nBlocks, 64 * W, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtrH, d.blkMatH,
+                                                                   d.blkScale, res, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+  else
+    KMixSchwarz<W, kPre, float><<<d.nBlocks, 64 * W, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
+                                                            nullptr, res, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+}
+
 // zeroE: e is zero on the whole level (start of pre-smoothing), so the first sweep takes the rhs
 // as its residual without a residual pass.
 static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s, bool zeroE) {
@@ -2063,18 +2132,13 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
     if (!fromRhs)
       KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
                                                                             d.tmpB, 1, d.Nd, K);
+    const float* res = fromRhs ? nullptr : d.tmpB;
     if (d.nBlocks <= kSchwarzPrefetchBlocks)
-      KMixSchwarz<kSchwarzWaves, kBlockExtMax / kSchwarzWaves><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(
-          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
-          ms.omega);
+      SchwarzLaunch<kSchwarzWaves, kBlockExtMax / kSchwarzWaves>(ms, d, res, K, s);
     else if (d.nBlocks < kSchwarzManyBlocks)
-      KMixSchwarz<kSchwarzWaves, 0><<<d.nBlocks, 64 * kSchwarzWaves, 0, s>>>(
-          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
-          ms.omega);
+      SchwarzLaunch<kSchwarzWaves, 0>(ms, d, res, K, s);
     else
-      KMixSchwarz<kSchwarzWavesLarge, 0><<<d.nBlocks, 64 * kSchwarzWavesLarge, 0, s>>>(
-          d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat, fromRhs ? nullptr : d.tmpB, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K,
-          ms.omega);
+      SchwarzLaunch<kSchwarzWavesLarge, 0>(ms, d, res, K, s);
   }
 }
 
@@ -2393,6 +2457,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     weights = own.data();
   }
   const bool weighted = weights != nullptr;
+  ms.schwarzHalf = !krylov;  // FP16 Schwarz pairs for plain refinement only (captured with the step)
   if (weighted) {
     for (int k = 0; k < 2 * K; k++)
       if (!(weights[k] >= 0) || !std::isfinite(weights[k])) throw std::runtime_error("operator weights must be finite and non-negative");

@@ -124,3 +124,48 @@ def test_fp32_smoother_fallback_solves_to_direct():
     en = np.sqrt(np.einsum("ij,ij->i", e, (A @ e.T).T) / np.einsum("ij,ij->i", xd, (A @ xd.T).T))
     print(f"[fp32 fallback] {its} iterations, energy error {en}", flush=True)
     assert np.all(en < 1e-7)
+
+
+def test_fp16_schwarz_blocks_solve_to_direct():
+    """The boundary Schwarz step's block inverses as FP16 pairs scaled per core row (forced onto
+    every level of a screened system, whose small levels keep FP32 by default): refinement still
+    reaches 1e-9 and matches the direct solution, in at most one more iteration than FP32; flexible
+    CG on the same system reads the FP32 inverses and matches the system without pairs bitwise."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mesh = g.TexturedMesh.icosphere(4, 256, 256, 0.05, 3)
+    H = g.InitializeHierarchy(mesh, 256, 256, 4)
+    S32 = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=3)
+    S32.configure_mixed()
+    os.environ["GDTP_SCHWARZ_HALF_BLOCKS"] = "0"
+    try:
+        S16 = g.UpdateLinearSystem(H, 1e-3, 1.0, max_cols=3)
+        S16.configure_mixed()
+    finally:
+        del os.environ["GDTP_SCHWARZ_HALF_BLOCKS"]
+    # the FP16 pairs are held beside the FP32 inverses
+    assert S16.memory()[1] > S32.memory()[1]
+    L = oracle_levels(H, S16)
+    A = oracle.full_matrix(L[0]).tocsc()
+    _, b = noise_texture_rhs(H, L[0], K=3, seed=11)
+    lu = spla.splu(A)
+    xd = np.stack([lu.solve(b[k]) for k in range(3)])
+    its = {}
+    for name, S in (("fp32", S32), ("fp16", S16)):
+        x = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+        its[name], rel = S.solve(x, torch.from_numpy(b).cuda(), tol=1e-9, max_iters=40, mode="mixed")
+        e = x.cpu().numpy() - xd
+        en = np.sqrt(np.einsum("ij,ij->i", e, (A @ e.T).T) / np.einsum("ij,ij->i", xd, (A @ xd.T).T))
+        print(f"[schwarz {name}] {its[name]} iterations, energy error {en}", flush=True)
+        assert np.all(rel <= 1e-9) and np.all(en < 1e-7)
+    assert its["fp16"] <= its["fp32"] + 1
+    xs = []
+    for S in (S32, S16):
+        S.configure_mixed(krylov=True)
+        try:
+            x = torch.zeros((3, L[0].N), dtype=torch.float64, device="cuda")
+            S.solve(x, torch.from_numpy(b).cuda(), tol=1e-8, max_iters=60, mode="mixed")
+            xs.append(x.cpu().numpy())
+        finally:
+            S.configure_mixed()
+    assert np.array_equal(xs[0], xs[1])
