@@ -75,13 +75,13 @@ struct DevMixLevel {
   // blkCore[blkCorePtr[b]..]) from the residual at its extended rows (boundary indices
   // blkExt[blkExtPtr[b]..]) through the core rows of the inverse of A restricted to the extended
   // rows, stored column-major at blkMat + blkMatPtr[b] ([ext][core]) in FP32, or at blkMatH +
-  // blkMatPtrH[b] as FP16 pairs ([ext / 2][core], extended rows 2s | 2s + 1 << 16 in one word)
-  // scaled per core row (entries |.| <= 1, core row r's scale blkScale[r])
+  // blkMatPtrH[b] as FP16 quads ([ext / 4][core], extended rows 4s .. 4s + 3 in one 64-bit word,
+  // lowest first) scaled per core row (entries |.| <= 1, core row r's scale blkScale[r])
   int nBlocks = 0;
   int *blkCorePtr = nullptr, *blkCore = nullptr, *blkExtPtr = nullptr, *blkExt = nullptr;
   long *blkMatPtr = nullptr, *blkMatPtrH = nullptr;
   float* blkMat = nullptr;
-  uint32_t* blkMatH = nullptr;
+  uint2* blkMatH = nullptr;
   float* blkScale = nullptr;
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
   int *rRowptr = nullptr, *rCol = nullptr;
@@ -107,7 +107,7 @@ struct MixedSystem {
   double screen = 1.0, stiff = 0.0;  // the system's own weights
   // flexible conjugate gradients around the V-cycle (else plain iterative refinement)
   bool krylov = false;
-  // the Schwarz steps read a level's FP16 block-inverse pairs where it has them (set per solve:
+  // the Schwarz steps read a level's FP16 block-inverse quads where it has them (set per solve:
   // plain refinement; flexible CG reads the FP32 inverses)
   bool schwarzHalf = true;
   // V-cycles per outer iteration (the level-0 correction carries over between them); 0 = automatic:

@@ -831,23 +831,25 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
 // kSchwarzManyBlocks blocks, where four times as many blocks are resident (C2 level 0: 40 against
 // 45 us with 16 waves); W = 16 elsewhere, where a block's own latency bounds the step (level 3,
 // 366 blocks: 10 us with 16 waves, 19 us with 4).
-// TM: the inverse's entries as FP32 (float, one extended row per load) or as FP16 pairs (uint32_t:
-// extended rows 2s and 2s + 1 of a core row in one word, so a lane's load brings two entries),
-// scaled per core row to |entry| <= 1 with the row's FP32 scale in rowScale.  The step is a
-// smoother, so the FP16 rounding (2^-11 of the row's largest entry) perturbs its correction, not
-// the residuals that drive the refinement.  Levels that smooth with FP16 stencil ratios (screened
-// operators) and more than one block per CU hold FP16 pairs beside the FP32 inverses; plain
-// refinement reads the pairs, flexible CG the FP32 inverses (with the pairs, batched and single
-// geodesic distances at 2048^2 drifted apart by 0.7 % of the largest distance at rel-res 1e-6,
-// against under 0.1 % with FP32).  kPre counts loads (entries or pairs).
+// TM: the inverse's entries as FP32 (float, one extended row per load) or as FP16 quads (uint2:
+// extended rows 4s .. 4s + 3 of a core row in one 64-bit word, so a lane's load brings four
+// entries), scaled per core row to |entry| <= 1 with the row's FP32 scale in rowScale.  The load
+// count, not the bytes, bounds the step: FP16 with one entry per load saved nothing, two per load
+// took C2 level 0 from 39.2 to 31.9 us, four per load C2 from 12.75 to 12.65 ms per solve.  The
+// step is a smoother, so the FP16 rounding (2^-11 of the row's largest entry) perturbs its
+// correction, not the residuals that drive the refinement.  Levels that smooth with FP16 stencil
+// ratios (screened operators) and more than one block per CU hold FP16 quads beside the FP32
+// inverses; plain refinement reads the quads, flexible CG the FP32 inverses (with FP16 entries,
+// batched and single geodesic distances at 2048^2 drifted apart by 0.7 % of the largest distance
+// at rel-res 1e-6, against under 0.1 % with FP32).  kPre counts loads (entries or quads).
 static constexpr int kBlockCore = 64, kBlockExtMax = 384, kSchwarzWaves = 16;
 static constexpr int kSchwarzPrefetchBlocks = 256;  // levels with at most this many blocks (one per CU) prefetch every entry
 static constexpr int kSchwarzWavesLarge = 4;        // waves per block on levels with many blocks (more blocks resident)
 static constexpr int kSchwarzManyBlocks = 1024;     // from this many blocks on
 __device__ __forceinline__ void Entries(const float* M, long k, float (&m)[1]) { m[0] = M[k]; }
-__device__ __forceinline__ void Entries(const uint32_t* M, long k, float (&m)[2]) {
-  const unsigned w = M[k];
-  m[0] = Half(w & 0xffff), m[1] = Half(w >> 16);
+__device__ __forceinline__ void Entries(const uint2* M, long k, float (&m)[4]) {
+  const uint2 w = M[k];
+  m[0] = Half(w.x & 0xffff), m[1] = Half(w.x >> 16), m[2] = Half(w.y & 0xffff), m[3] = Half(w.y >> 16);
 }
 template <int W, int kPre, class TM>
 __global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ corePtr, const int* __restrict__ core,
@@ -857,8 +859,9 @@ __global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ co
                                                                   const float* __restrict__ resB, const int* __restrict__ bdIdx,
                                                                   const float* __restrict__ rhs, int B, float* e, long Nd, int K,
                                                                   float omega) {
-  constexpr bool kScaled = std::is_same<TM, uint32_t>::value;
-  constexpr int kJ = kScaled ? 2 : 1;  // extended rows per load
+  constexpr bool kScaled = !std::is_same<TM, float>::value;
+  constexpr int kJ = kScaled ? (int)sizeof(TM) / 2 : 1;  // extended rows per load
+  static_assert(kBlockExtMax % kJ == 0, "padded extended rows fit the staged residual");
   __shared__ float R[4 * kBlockExtMax];
   __shared__ float P[W][4][kBlockCore];
   const int b = blockIdx.x, c0 = corePtr[b], nc = corePtr[b + 1] - c0, x0 = extPtr[b], nx = extPtr[b + 1] - x0;
@@ -880,7 +883,7 @@ __global__ __launch_bounds__(64 * W) void KMixSchwarz(const int* __restrict__ co
 #pragma unroll
       for (int t = 0; t < kJ; t++) mv[ss][t] = 0.0f;
   }
-  const int nxPad = (nx + kJ - 1) / kJ * kJ;  // a pair's second row past nx reads a zero residual
+  const int nxPad = (nx + kJ - 1) / kJ * kJ;  // a quad's rows past nx read a zero residual
   for (int ch0 = 0; ch0 < K; ch0 += 4) {
     const int kc = min(4, K - ch0);
     for (int j = threadIdx.x; j < nxPad; j += 64 * W)
@@ -1907,32 +1910,33 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
       d.nBlocks = (int)blk.corePtr.size() - 1;
       d.blkCorePtr = Up(ms, blk.corePtr), d.blkCore = Up(ms, blk.core), d.blkExtPtr = Up(ms, blk.extPtr), d.blkExt = Up(ms, blk.ext);
       d.blkMatPtr = Up(ms, blk.matPtr);
-      // FP16 pairs of extended rows, each core row scaled by its largest |entry|, where the step
-      // streams its matrices (C2: level 0 39.2 -> 31.9 us, level 1 26.3 -> 20.6 us); levels with at
-      // most one block per CU prefetch every entry and are latency-bound, and keep FP32 (the pairs'
-      // conversions made them 0.5-0.8 us slower).  GDTP_SCHWARZ_HALF_BLOCKS lowers the block count
-      // from which levels take the pairs (tests use it to run small systems through them).
+      // FP16 quads of extended rows, each core row scaled by its largest |entry|, where the step
+      // streams its matrices; levels with at most one block per CU prefetch every entry and are
+      // latency-bound, and keep FP32 (FP16 pairs' conversions made them 0.5-0.8 us slower).
+      // GDTP_SCHWARZ_HALF_BLOCKS lowers the block count from which levels take the quads (tests use
+      // it to run small systems through them).
       int halfBlocks = kSchwarzPrefetchBlocks + 1;
       if (const char* env = std::getenv("GDTP_SCHWARZ_HALF_BLOCKS")) halfBlocks = std::atoi(env);
       d.blkMat = Up(ms, blk.mat);
       if (d.stencilH && d.nBlocks >= halfBlocks) {
         std::vector<long> hp(1, 0);
         for (int b = 0; b < d.nBlocks; b++)
-          hp.push_back(hp.back() + (long)((blk.extPtr[b + 1] - blk.extPtr[b] + 1) / 2) * (blk.corePtr[b + 1] - blk.corePtr[b]));
-        std::vector<uint32_t> mh(hp.back());
+          hp.push_back(hp.back() + (long)((blk.extPtr[b + 1] - blk.extPtr[b] + 3) / 4) * (blk.corePtr[b + 1] - blk.corePtr[b]));
+        std::vector<uint2> mh(hp.back());
         std::vector<float> sc(blk.core.size(), 1.0f);
 #pragma omp parallel for schedule(dynamic, 16)
         for (int b = 0; b < d.nBlocks; b++) {
           const int c0 = blk.corePtr[b], nc = blk.corePtr[b + 1] - c0, nx = blk.extPtr[b + 1] - blk.extPtr[b];
           const float* M = blk.mat.data() + blk.matPtr[b];
-          uint32_t* H = mh.data() + hp[b];
+          uint2* H = mh.data() + hp[b];
           for (int i = 0; i < nc; i++) {
             float mx = 0.0f;
             for (int j = 0; j < nx; j++) mx = std::max(mx, std::fabs(M[(size_t)j * nc + i]));
             const float sr = mx > 0.0f ? mx : 1.0f;
             sc[c0 + i] = sr;
             auto h = [&](int j) { return j < nx ? (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(M[(size_t)j * nc + i] / sr)) : 0u; };
-            for (int sl = 0; 2 * sl < nx; sl++) H[(size_t)sl * nc + i] = h(2 * sl) | h(2 * sl + 1) << 16;
+            for (int sl = 0; 4 * sl < nx; sl++)
+              H[(size_t)sl * nc + i] = make_uint2(h(4 * sl) | h(4 * sl + 1) << 16, h(4 * sl + 2) | h(4 * sl + 3) << 16);
           }
         }
         d.blkMatPtrH = Up(ms, hp), d.blkMatH = Up(ms, mh), d.blkScale = Up(ms, sc);
@@ -2116,8 +2120,8 @@ static int DeepSweep(MixedSystem& ms, DevMixLevel& d, bool post, int K, hipStrea
 template <int W, int kPre>
 static void SchwarzLaunch(MixedSystem& ms, DevMixLevel& d, const float* res, int K, hipStream_t s) {
   if (d.blkMatH && ms.schwarzHalf)
-    KMixSchwarz<W, kPre / 2, uint32_t><<<d.nBlocks, 64 * W, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtrH, d.blkMatH,
-                                                                   d.blkScale, res, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
+    KMixSchwarz<W, kPre / 4, uint2><<<d.nBlocks, 64 * W, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtrH, d.blkMatH,
+                                                                d.blkScale, res, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
   else
     KMixSchwarz<W, kPre, float><<<d.nBlocks, 64 * W, 0, s>>>(d.blkCorePtr, d.blkCore, d.blkExtPtr, d.blkExt, d.blkMatPtr, d.blkMat,
                                                             nullptr, res, d.bdIdx, d.rhs, d.B, d.e, d.Nd, K, ms.omega);
@@ -2457,7 +2461,7 @@ int SolveMixed(DeviceSystem& ds, MixedSystem& ms, double* x, const double* b, in
     weights = own.data();
   }
   const bool weighted = weights != nullptr;
-  ms.schwarzHalf = !krylov;  // FP16 Schwarz pairs for plain refinement only (captured with the step)
+  ms.schwarzHalf = !krylov;  // FP16 Schwarz quads for plain refinement only (captured with the step)
   if (weighted) {
     for (int k = 0; k < 2 * K; k++)
       if (!(weights[k] >= 0) || !std::isfinite(weights[k])) throw std::runtime_error("operator weights must be finite and non-negative");

@@ -127,7 +127,7 @@ def test_fp32_smoother_fallback_solves_to_direct():
 
 
 def test_fp16_schwarz_blocks_solve_to_direct():
-    """The boundary Schwarz step's block inverses as FP16 pairs scaled per core row (forced onto
+    """The boundary Schwarz step's block inverses as FP16 quads scaled per core row (forced onto
     every level of a screened system, whose small levels keep FP32 by default): refinement still
     reaches 1e-9 and matches the direct solution, in at most one more iteration than FP32; flexible
     CG on the same system reads the FP32 inverses and matches the system without pairs bitwise."""
@@ -143,7 +143,7 @@ def test_fp16_schwarz_blocks_solve_to_direct():
         S16.configure_mixed()
     finally:
         del os.environ["GDTP_SCHWARZ_HALF_BLOCKS"]
-    # the FP16 pairs are held beside the FP32 inverses
+    # the FP16 quads are held beside the FP32 inverses
     assert S16.memory()[1] > S32.memory()[1]
     L = oracle_levels(H, S16)
     A = oracle.full_matrix(L[0]).tocsc()
