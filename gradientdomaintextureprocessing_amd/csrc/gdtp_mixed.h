@@ -69,8 +69,9 @@ struct DevMixLevel {
   int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, kShapePerColour, kShapePairs)
   // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
   int* bdIdx = nullptr;
-  int *bRowptr = nullptr, *bCol = nullptr;
-  float *bVal = nullptr, *bDiag = nullptr;
+  int* bRowptr = nullptr;
+  int2* bEnt = nullptr;  // off-diagonal entries: (dense column, FP32 value bits)
+  float* bDiag = nullptr;
   // overlapping Schwarz blocks of boundary rows: block b updates its core rows (dense indices
   // blkCore[blkCorePtr[b]..]) from the residual at its extended rows (boundary indices
   // blkExt[blkExtPtr[b]..]) through the core rows of the inverse of A restricted to the extended

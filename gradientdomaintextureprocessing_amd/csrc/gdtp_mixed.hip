@@ -772,10 +772,10 @@ __device__ __forceinline__ T Pick(const T (&a)[4], int q) {
 }
 
 // Boundary-row residual rhs - A e (FP32).  out: compact [K][B] when `compact`, else the dense
-// vector [K][Nd] at the row's texel.
+// vector [K][Nd] at the row's texel.  A row entry is one 64-bit word (dense column, FP32 value
+// bits): one load per entry instead of two.
 __device__ __forceinline__ void BoundaryResidualRows(int vb, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                     const int* __restrict__ cl, const float* __restrict__ vl,
-                                                     const float* __restrict__ dg, const float* __restrict__ e,
+                                                     const int2* __restrict__ ent, const float* __restrict__ dg, const float* __restrict__ e,
                                                      const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
   const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
   if (r >= B) return;  // whole row groups leave together
@@ -785,8 +785,9 @@ __device__ __forceinline__ void BoundaryResidualRows(int vb, int B, const int* _
     const int kc = min(4, K - c0);
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int k = k0 + lane; k < k1; k += kRowLanes) {
-      const float v = vl[k];
-      const long c = cl[k];
+      const int2 w = ent[k];
+      const float v = __int_as_float(w.y);
+      const long c = w.x;
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (q < kc) acc[q] += v * e[(long)(c0 + q) * Nd + c];
@@ -803,10 +804,10 @@ __device__ __forceinline__ void BoundaryResidualRows(int vb, int B, const int* _
 }
 
 __global__ __launch_bounds__(256) void KMixBoundaryResidual(int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                            const int* __restrict__ cl, const float* __restrict__ vl,
-                                                            const float* __restrict__ dg, const float* __restrict__ e,
-                                                            const float* __restrict__ rhs, float* out, int compact, long Nd, int K) {
-  BoundaryResidualRows(blockIdx.x, B, bdIdx, rp, cl, vl, dg, e, rhs, out, compact, Nd, K);
+                                                            const int2* __restrict__ ent, const float* __restrict__ dg,
+                                                            const float* __restrict__ e, const float* __restrict__ rhs, float* out,
+                                                            int compact, long Nd, int K) {
+  BoundaryResidualRows(blockIdx.x, B, bdIdx, rp, ent, dg, e, rhs, out, compact, Nd, K);
 }
 
 static constexpr int kResidualWideTiles = 1024;  // levels with more 64 x 4 tiles load two columns at a time
@@ -817,11 +818,11 @@ __global__ __launch_bounds__(256) void KMixResidual(int nTiles, const int* __res
                                                     const MixChart* __restrict__ charts, long plane,
                                                     long Nd, const float* __restrict__ sym, const float* __restrict__ e,
                                                     const float* __restrict__ rhs, float* res, int K, int zeroQ, int B,
-                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                                    const float* __restrict__ vl, const float* __restrict__ dg) {
+                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int2* __restrict__ ent,
+                                                    const float* __restrict__ dg) {
   if ((int)blockIdx.x < nTiles)
     ResidualDeepTile<kChunk>(XcdTile(blockIdx.x, nTiles), tiles, deepMask, charts, plane, Nd, sym, e, rhs, res, K, zeroQ);
-  else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, dg, e, rhs, res, 0, Nd, K);
+  else BoundaryResidualRows(blockIdx.x - nTiles, B, bdIdx, rp, ent, dg, e, rhs, res, 0, Nd, K);
 }
 
 // Restricted additive Schwarz over the boundary rows: one workgroup of W waves per block, the
@@ -1904,7 +1905,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
         rp.push_back((int)cl.size());
         if (!(dg[r] > 0.0f)) throw std::runtime_error("boundary row without positive diagonal");
       }
-      d.bdIdx = Up(ms, bidx), d.bRowptr = Up(ms, rp), d.bCol = Up(ms, cl), d.bVal = Up(ms, vl), d.bDiag = Up(ms, dg);
+      std::vector<int2> ent(cl.size());
+      for (size_t k = 0; k < cl.size(); k++) ent[k] = make_int2(cl[k], __builtin_bit_cast(int, vl[k]));
+      d.bdIdx = Up(ms, bidx), d.bRowptr = Up(ms, rp), d.bEnt = Up(ms, ent), d.bDiag = Up(ms, dg);
       SchwarzBlocks blk;
       BuildSchwarzBlocks(bb, bidx, blk);
       d.nBlocks = (int)blk.corePtr.size() - 1;
@@ -2134,7 +2137,7 @@ static void BoundarySweeps(MixedSystem& ms, DevMixLevel& d, int K, hipStream_t s
   for (int sw = 0; sw < ms.boundarySweeps; sw++) {
     const bool fromRhs = zeroE && sw == 0;
     if (!fromRhs)
-      KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag, d.e, d.rhs,
+      KMixBoundaryResidual<<<NBlk((long)d.B * kRowLanes, 256), 256, 0, s>>>(d.B, d.bdIdx, d.bRowptr, d.bEnt, d.bDiag, d.e, d.rhs,
                                                                             d.tmpB, 1, d.Nd, K);
     const float* res = fromRhs ? nullptr : d.tmpB;
     if (d.nBlocks <= kSchwarzPrefetchBlocks)
@@ -2201,7 +2204,7 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
     if (d.nTiles + rb > 0)
       (d.nTiles > kResidualWideTiles ? KMixResidual<2> : KMixResidual<4>)<<<d.nTiles + rb, 256, 0, s>>>(
           d.nTiles, d.tiles, d.deepMask, d.charts, d.plane, d.Nd, d.stencilSym, d.e, d.rhs, d.res, K, ms.nuPre > 0 && !d.stencilH ? 3 : -1,
-          d.B, d.bdIdx, d.bRowptr, d.bCol, d.bVal, d.bDiag);
+          d.B, d.bdIdx, d.bRowptr, d.bEnt, d.bDiag);
     if (c.nTiles + cb > 0)
       KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.deepMask, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, d.res, c.rhs,
                                                  c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);
