@@ -67,7 +67,7 @@ struct DevMixLevel {
   int nRows = 0, pairSpan = 0;     // sub-grid texels per thread of the widest span (1, 2; 0: too wide)
   int pairThreads = 0;             // threads per workgroup of the pair sweep (512, kPairThreadsMax)
   int shape = 0;                   // deep-sweep tile shape of this level (kFusedShape index, kShapePerColour, kShapePairs)
-  // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns), diagonal
+  // boundary rows: dense index, off-diagonal row (A_BB and A_BD merged, dense columns, one word per entry), diagonal
   int* bdIdx = nullptr;
   int* bRowptr = nullptr;
   int2* bEnt = nullptr;  // off-diagonal entries: (dense column, FP32 value bits)
