@@ -5,7 +5,8 @@ colour and a separate prolongation pass: every tile shape gives bitwise the same
 mixed solver, on screened operators (FP16 smoother stencil) and on a nearly singular one (FP32
 smoother stencil), for column counts that fill one and two column chunks, and for rows of every
 width the pair sweep handles.  The FP32 fallback for FP16-range stencils (forced by lowering the
-FP16 ratio limit) solves to the direct solution."""
+FP16 ratio limit) solves to the direct solution, and so do FP16 Schwarz block inverses forced onto
+every level, which flexible CG leaves unread."""
 import os
 
 import numpy as np
