@@ -85,8 +85,8 @@ struct DevMixLevel {
   uint2* blkMatH = nullptr;
   float* blkScale = nullptr;
   // restriction into this level (coarse boundary rows: fine dense columns, weights)
-  int *rRowptr = nullptr, *rCol = nullptr;
-  float* rVal = nullptr;
+  int* rRowptr = nullptr;
+  int2* rEnt = nullptr;  // (fine dense column, FP32 weight bits)
   // vectors [K][Nd] and boundary residual scratch [K][B]; the fused sweep reads e and writes e2
   // (levels swept by tiles), after which the two pointers swap
   float *e = nullptr, *rhs = nullptr, *res = nullptr, *tmpB = nullptr, *e2 = nullptr;

@@ -970,10 +970,10 @@ __device__ __forceinline__ void RestrictDeepTile(int vb, const int* __restrict__
 }
 
 // Coarse boundary rhs: the boundary restriction matrix (fine dense columns); kRowLanes lanes per
-// coarse row read its entries coalesced, once for up to four columns, summed by shuffles.
+// coarse row read its entries coalesced, once for up to four columns, summed by shuffles.  An entry
+// is one 64-bit word (fine dense column, FP32 weight bits).
 __device__ __forceinline__ void RestrictBoundaryRows(int vb, int B, const int* __restrict__ bdIdx, const int* __restrict__ rp,
-                                                     const int* __restrict__ cl, const float* __restrict__ vl,
-                                                     const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
+                                                     const int2* __restrict__ ent, const float* __restrict__ fres, long fNd, float* crhs, float* ce, long cNd,
                                                      int K) {
   const int r = (int)(((long)vb * blockDim.x + threadIdx.x) / kRowLanes), lane = threadIdx.x % kRowLanes;
   if (r >= B) return;  // whole row groups leave together
@@ -983,8 +983,9 @@ __device__ __forceinline__ void RestrictBoundaryRows(int vb, int B, const int* _
     const int kc = min(4, K - c0);
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int k = k0 + lane; k < k1; k += kRowLanes) {
-      const float v = vl[k];
-      const long c = cl[k];
+      const int2 w = ent[k];
+      const float v = __int_as_float(w.y);
+      const long c = w.x;
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (q < kc) acc[q] += v * fres[(long)(c0 + q) * fNd + c];
@@ -1003,11 +1004,10 @@ __device__ __forceinline__ void RestrictBoundaryRows(int vb, int B, const int* _
 __global__ __launch_bounds__(256) void KMixRestrict(int nTiles, const int* __restrict__ tiles, const unsigned long long* __restrict__ cDeepMask,
                                                     const MixChart* __restrict__ cch, const MixChart* __restrict__ fch, long cplane, long cNd,
                                                     long fplane, long fNd, const float* __restrict__ fres, float* crhs, float* ce, int K, int B,
-                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int* __restrict__ cl,
-                                                    const float* __restrict__ vl) {
+                                                    const int* __restrict__ bdIdx, const int* __restrict__ rp, const int2* __restrict__ ent) {
   if ((int)blockIdx.x < nTiles)
     RestrictDeepTile(XcdTile(blockIdx.x, nTiles), tiles, cDeepMask, cch, fch, cplane, cNd, fplane, fNd, fres, crhs, ce, K);
-  else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, cl, vl, fres, fNd, crhs, ce, cNd, K);
+  else RestrictBoundaryRows(blockIdx.x - nTiles, B, bdIdx, rp, ent, fres, fNd, crhs, ce, cNd, K);
 }
 
 // e_fine += P e_coarse at every active fine texel (bilinear, chart-centred coarse grid).  A thread
@@ -1952,10 +1952,9 @@ void UploadMixed(const Hierarchy& h, const OperatorHierarchy& oh, DeviceSystem& 
     // restriction into this level (rows: this level's boundary texels; columns: fine dense)
     if (l > 0) {
       const Csr& R = h.levels[l - 1].boundaryRestriction;
-      std::vector<int> cl(R.col.size());
-      std::vector<float> vl(R.val.size());
-      for (size_t k = 0; k < R.col.size(); k++) cl[k] = (int)g2dAll[l - 1][R.col[k]], vl[k] = (float)R.val[k];
-      d.rRowptr = Up(ms, R.rowptr), d.rCol = Up(ms, cl), d.rVal = Up(ms, vl);
+      std::vector<int2> ent(R.col.size());
+      for (size_t k = 0; k < R.col.size(); k++) ent[k] = make_int2((int)g2dAll[l - 1][R.col[k]], __builtin_bit_cast(int, (float)R.val[k]));
+      d.rRowptr = Up(ms, R.rowptr), d.rEnt = Up(ms, ent);
     }
     // every active texel's bilinear parents must be active texels of the next level
     if (l + 1 < nl) {
@@ -2207,7 +2206,7 @@ void MixedVCycle(DeviceSystem& ds, MixedSystem& ms, int K, hipStream_t s, bool z
           d.B, d.bdIdx, d.bRowptr, d.bEnt, d.bDiag);
     if (c.nTiles + cb > 0)
       KMixRestrict<<<c.nTiles + cb, 256, 0, s>>>(c.nTiles, c.tiles, c.deepMask, c.charts, d.charts, c.plane, c.Nd, d.plane, d.Nd, d.res, c.rhs,
-                                                 c.e, K, c.B, c.bdIdx, c.rRowptr, c.rCol, c.rVal);
+                                                 c.e, K, c.B, c.bdIdx, c.rRowptr, c.rEnt);
   }
   {
     DevMixLevel& c = ms.L[nl - 1];
